@@ -1,0 +1,208 @@
+/*
+ * vgr.h — C ABI of the MI355X (gfx950) batched Gaussian-splat camera renderer.
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t (passed
+ * as void*), allocates nothing, and returns 0 on success or a nonzero VGR_E_*
+ * code; vgr_last_error() then names the failing argument.  All memory is owned
+ * by the caller.  Work is enqueued on the given stream; nothing synchronizes
+ * the host unless the function says so.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   vgr_render                  <- render(...)          src/isaacsim_gaussian_renderer/native/renderer.cpp:214-310
+ *                                  render_cuda(...)     src/isaacsim_gaussian_renderer/native/renderer_cuda.cu:2536-3456
+ *   vgr_precompute_covariances  <- precompute_covariances(...) renderer.cpp:89-124, renderer_cuda.cu:373-394
+ *   vgr_sort_temp_bytes         <- sort_temp_bytes / radix_sort_double_buffer_temp_bytes
+ *                                  renderer.cpp:75-88, renderer_cuda.cu:2463-2534
+ *   vgr_emitted_sort_key_end_bit<- emitted_sort_key_end_bit renderer_cuda.cu:28-46
+ *   vgr_prepare_chunked_active_ids <- prepare_chunked_active_ids renderer.cpp:126-190, renderer_cuda.cu:2397-2444
+ *   vgr_aggregate_chunk_counters   <- aggregate_chunk_counters  renderer.cpp:192-212, renderer_cuda.cu:2446-2461
+ *   vgr_last_error              <- TORCH_CHECK messages surfaced as RuntimeError (renderer.cpp:9-25)
+ */
+#ifndef VGR_H
+#define VGR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VGR_ABI_VERSION 1
+
+enum vgr_status {
+  VGR_OK = 0,
+  VGR_E_INVALID_ARGUMENT = 1,
+  VGR_E_LAUNCH = 2,
+  VGR_E_CAPACITY = 3,
+};
+
+/* Counter block written by vgr_render (int64[5]); same order and meaning as the
+ * reference's COUNTER_NAMES (cuda_backend.py:22-28). */
+enum vgr_counter {
+  VGR_COUNTER_VISIBLE = 0,
+  VGR_COUNTER_INTERSECTIONS = 1,
+  VGR_COUNTER_VISIBLE_OVERFLOW = 2,
+  VGR_COUNTER_INTERSECTION_OVERFLOW = 3,
+  VGR_COUNTER_ACTIVE_TILES = 4,
+  VGR_COUNTER_COUNT = 5,
+};
+
+/* Packed scene, one entry per Gaussian over all registered scenes
+ * (reference scene tensor list, renderer_cuda.cu:48-56). */
+typedef struct vgr_scene {
+  const float* means;               /* [G,3] */
+  const float* covariances;         /* [G,6] upper triangle 00,01,02,11,12,22 */
+  const float* opacities;           /* [G] activated */
+  const float* features;            /* [G,F]; F = 3 * (sh_degree+1)^2, k-major */
+  const int64_t* semantic_ids;      /* [G] */
+  const int64_t* registered_scene_ids; /* [S] */
+  const int64_t* scene_offsets;     /* [S+1] */
+  int64_t gaussian_count;           /* G */
+  int64_t max_scene_gaussians;      /* largest registered scene */
+  int32_t feature_width;            /* F >= 3 */
+  int32_t scene_count;              /* S */
+  int32_t sh_degree;                /* 0..3; bands beyond F/3 are absent */
+  int32_t reserved;
+} vgr_scene;
+
+/* Camera batch (reference camera tensor list, renderer_cuda.cu:58-64). */
+typedef struct vgr_cameras {
+  const float* viewmats;            /* [B,4,4] row-major world->OpenCV camera */
+  const float* intrinsics;          /* [B,3,3] */
+  const float* env_xforms;          /* [B,4,4] scene->world */
+  const int64_t* scene_ids;         /* [B] */
+  const int64_t* active_ids;        /* [A] or NULL (all cameras active) */
+  int32_t batch;                    /* B */
+  int32_t active_count;             /* A (== B when active_ids is NULL) */
+  /* Per-camera resolution.  NULL means every camera is width x height. */
+  const int32_t* widths;            /* [B] or NULL */
+  const int32_t* heights;           /* [B] or NULL */
+  const int64_t* pixel_offsets;     /* [B] first output pixel of each camera, or NULL */
+  /* Sum of tile counts over all B cameras at options.tile_size; required when
+   * widths/heights are given (the host cannot read device resolutions). */
+  int64_t max_global_tiles;
+} vgr_cameras;
+
+/* Outputs (reference output tensor list, renderer_cuda.cu:66-71).  With
+ * pixel_offsets NULL the layout is [B,H,W,C]; otherwise camera b owns pixels
+ * [pixel_offsets[b], pixel_offsets[b] + H_b*W_b) of a flat [P,C] buffer.
+ * depth/alpha/semantic may be NULL when full_sensor_output is 0. */
+typedef struct vgr_outputs {
+  float* rgb;                       /* [.,3] */
+  float* depth;                     /* [.,1] */
+  float* alpha;                     /* [.,1] */
+  int64_t* semantic;                /* [.,1] */
+} vgr_outputs;
+
+/* Visible-record layout (48 bytes, 16-byte aligned).  One record per visible
+ * (camera, Gaussian) pair, in camera-then-Gaussian order for each camera. */
+typedef struct vgr_record {
+  float mean_x, mean_y;             /* pixel-space centre */
+  float conic_0, conic_1, conic_2;  /* inverse 2D covariance (xx, xy, yy) */
+  float opacity;                    /* clamped to [0,1] */
+  float depth;                      /* camera-space z */
+  float red, green, blue;           /* view-dependent colour (SH evaluated) */
+  int32_t gaussian_id;              /* packed global Gaussian index */
+  int32_t camera_id;                /* camera index in the batch */
+} vgr_record;
+
+/* Workspace: caller-owned device buffers (sizes from vgr_workspace_sizes). */
+typedef struct vgr_workspace {
+  vgr_record* records;              /* [visible_capacity] */
+  int32_t* radii;                   /* [visible_capacity,2] */
+  uint64_t* keys_in;                /* [intersection_capacity] */
+  uint64_t* keys_out;
+  int32_t* values_in;
+  int32_t* values_out;
+  int32_t* tile_starts;             /* [tile_capacity] */
+  int32_t* tile_ends;
+  int64_t* counters;                /* [5] */
+  void* scratch;                    /* [scratch_bytes] */
+  int64_t visible_capacity;
+  int64_t intersection_capacity;
+  int64_t tile_capacity;            /* global tiles the tile arrays can hold */
+  int64_t scratch_bytes;
+} vgr_workspace;
+
+/* Render options (reference scalar arguments, renderer.cpp:214-238). */
+typedef struct vgr_options {
+  int32_t height;                   /* used when cameras.heights is NULL */
+  int32_t width;
+  float near_plane;
+  float far_plane;
+  float gaussian_support_sigma;
+  float covariance_epsilon;
+  float semantic_min_alpha;
+  int32_t antialiased;
+  int32_t tile_size;                /* 8 or 16 */
+  int32_t output_srgb;
+  int32_t deterministic;            /* accepted; ordering is always deterministic */
+  int32_t full_sensor_output;       /* 1: rgb+depth+alpha+semantic, 0: rgb only */
+  int32_t fixed_capacity_sort;      /* accepted; the sort never reads the count on the host */
+  int32_t sort_digit_bits;          /* 0 = default (8); 4..8 */
+  int32_t raster_batch;             /* Gaussians staged per LDS batch: 0 = default (256); 64..512 */
+  int32_t cameras_per_block;        /* projection camera group: 0 = default (8); 1..64 */
+  int32_t reuse_projection;         /* 1: composite only, reusing the sorted intersections and
+                                       tile ranges of the previous render into this workspace
+                                       (reference projection cache, cuda_backend.py:607-619) */
+} vgr_options;
+
+typedef struct vgr_workspace_sizes {
+  int64_t scratch_bytes;            /* bytes vgr_render needs in workspace.scratch */
+  int64_t tile_capacity;            /* global tiles for the batch */
+  int32_t sort_key_bits;            /* significant bits of the compressed sort key */
+  int32_t sort_passes;              /* radix passes the sort will run */
+} vgr_workspace_sizes;
+
+/* Library ABI version (VGR_ABI_VERSION). */
+int32_t vgr_abi_version(void);
+
+/* Message for the most recent nonzero return on this host thread. */
+const char* vgr_last_error(void);
+
+/* 32 + bit length of (global_tile_capacity - 1); -1 when the capacity is not in
+ * [1, INT32_MAX].  Same table as renderer_cuda.cu:28-46. */
+int32_t vgr_emitted_sort_key_end_bit(int64_t global_tile_capacity);
+
+/* Scratch requirement of vgr_render for the given capacities. */
+int32_t vgr_workspace_sizes_query(const vgr_cameras* cameras,
+                                  const vgr_options* options,
+                                  int64_t scene_gaussians,
+                                  int64_t intersection_capacity,
+                                  vgr_workspace_sizes* out);
+
+/* Bytes of sort scratch for n_items keys over n_segments tiles (the caller may
+ * size workspace.scratch with vgr_workspace_sizes_query instead). */
+int64_t vgr_sort_temp_bytes(int64_t n_items, int64_t n_segments);
+
+/* covariances[g] = R(q_g) diag(s_g^2) R(q_g)^T, upper triangle (renderer_cuda.cu:492-540). */
+int32_t vgr_precompute_covariances(const float* scales, const float* rotations,
+                                   int64_t count, float* covariances, void* stream);
+
+/* Full forward pass: projection + culling, tile emission, radix ordering, tile
+ * ranges, fused RGB/depth/alpha/semantic compositing.  *selector_out receives
+ * 1 when keys_out/values_out hold the sorted intersections, 0 when keys_in /
+ * values_in do (reference DoubleBuffer selector, cuda_backend.py:718-736).
+ * The host is never synchronized. */
+int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras,
+                   const vgr_outputs* outputs, const vgr_workspace* workspace,
+                   const vgr_options* options, void* stream, int64_t* selector_out);
+
+/* Logical-batch chunking helpers (renderer_cuda.cu:2307-2360). */
+int32_t vgr_prepare_chunked_active_ids(const int64_t* active_ids, int64_t active_count,
+                                       int32_t* active_mask, int64_t* chunk_active_ids,
+                                       int64_t logical_batch, int64_t physical_batch,
+                                       void* stream);
+int32_t vgr_aggregate_chunk_counters(const int64_t* chunk_counters, int64_t counter_stride,
+                                     int64_t chunk_count, int64_t* counters,
+                                     int64_t* physical_capacity_counters, void* stream);
+
+/* Fills n int64 values (graph-safe replacement for hipMemsetAsync). */
+int32_t vgr_fill_i64(int64_t* data, int64_t n, int64_t value, void* stream);
+int32_t vgr_fill_f32(float* data, int64_t n, float value, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VGR_H */

@@ -1,0 +1,399 @@
+// C ABI entry points of libvgr_hip: argument validation, scratch planning and
+// the launch sequence of one render submission.
+//
+// Launch order mirrors render_cuda (renderer_cuda.cu:2536-3456) for the
+// general 16x16 / 8x8 tile path: camera table -> projection count -> block
+// scan (writes the counter block) -> projection write + emission -> radix
+// ordering -> tile ranges -> fused compositing.  Unlike the reference there
+// is no device-to-host count copy: every stage reads the counts it needs from
+// the device, so one submission never synchronizes the host.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "vgr_internal.h"
+
+namespace vgr {
+
+namespace {
+thread_local char g_error[512] = "";
+
+size_t align_up(size_t value) { return (value + 255u) & ~static_cast<size_t>(255u); }
+
+int bit_length_u64(uint64_t value) {
+  int bits = 0;
+  while (value != 0) {
+    ++bits;
+    value >>= 1;
+  }
+  return bits;
+}
+
+int64_t global_tile_count(const vgr_cameras& cameras, const vgr_options& options) {
+  if (cameras.widths != nullptr || cameras.heights != nullptr) return cameras.max_global_tiles;
+  const int64_t tiles_x = (options.width + options.tile_size - 1) / options.tile_size;
+  const int64_t tiles_y = (options.height + options.tile_size - 1) / options.tile_size;
+  return static_cast<int64_t>(cameras.batch) * tiles_x * tiles_y;
+}
+
+int resolve_digit_bits(const vgr_options& options) {
+  return options.sort_digit_bits == 0 ? 8 : options.sort_digit_bits;
+}
+
+int resolve_cameras_per_block(const vgr_options& options) {
+  return options.cameras_per_block == 0 ? 8 : options.cameras_per_block;
+}
+
+ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const vgr_options& options,
+                             int64_t global_tiles, int64_t intersection_capacity) {
+  ScratchLayout layout{};
+  layout.cameras_per_block = resolve_cameras_per_block(options);
+  layout.projection_chunks = (gaussian_count + kBlockThreads - 1) / kBlockThreads;
+  layout.projection_groups = (active_count + layout.cameras_per_block - 1) / layout.cameras_per_block;
+  layout.projection_blocks = layout.projection_chunks * layout.projection_groups;
+  const int64_t scan_blocks = (layout.projection_blocks + kBlockThreads * 8 - 1) / (kBlockThreads * 8);
+
+  const uint32_t near_bits = vgr_float_bits(options.near_plane);
+  const uint32_t far_bits = vgr_float_bits(options.far_plane);
+  layout.depth_base = near_bits;
+  layout.depth_bits = bit_length_u64(static_cast<uint64_t>(far_bits - near_bits) + 1u);
+  layout.tile_bits = global_tiles > 1 ? bit_length_u64(static_cast<uint64_t>(global_tiles - 1)) : 0;
+  layout.sort_key_bits = layout.tile_bits + layout.depth_bits;
+  layout.sort_digit_bits = resolve_digit_bits(options);
+  layout.sort_passes = (layout.sort_key_bits + layout.sort_digit_bits - 1) / layout.sort_digit_bits;
+  layout.sort_tiles = (intersection_capacity + kSortTile - 1) / kSortTile;
+  const int64_t radix = 1 << layout.sort_digit_bits;
+
+  size_t offset = 0;
+  layout.camera_table = offset;
+  offset = align_up(offset + sizeof(CameraParams) * static_cast<size_t>(active_count > 0 ? active_count : 1));
+  layout.block_counts = offset;
+  offset = align_up(offset + sizeof(int64_t) * 2 * static_cast<size_t>(layout.projection_blocks + 1));
+  layout.block_bases = offset;
+  offset = align_up(offset + sizeof(int64_t) * 2 * static_cast<size_t>(layout.projection_blocks + scan_blocks + 1));
+  layout.sort_histogram = offset;
+  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(layout.sort_passes * radix));
+  layout.sort_digit_base = offset;
+  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(layout.sort_passes * radix));
+  layout.sort_tickets = offset;
+  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(layout.sort_passes));
+  layout.sort_status = offset;
+  offset = align_up(offset + sizeof(uint32_t) * 4);
+  layout.sort_flags = offset;
+  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(layout.sort_passes) *
+                                 static_cast<size_t>(layout.sort_tiles) * static_cast<size_t>(radix));
+  layout.total = offset;
+  return layout;
+}
+
+int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
+  if (options.tile_size != 8 && options.tile_size != 16) {
+    set_error("tile_size must be 8 or 16 on this path; got %d", options.tile_size);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const bool uniform = cameras.widths == nullptr && cameras.heights == nullptr;
+  if (uniform && (options.width <= 0 || options.height <= 0)) {
+    set_error("width and height must be positive; got %dx%d", options.width, options.height);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (!uniform && (cameras.widths == nullptr || cameras.heights == nullptr || cameras.max_global_tiles <= 0)) {
+    set_error("per-camera resolutions need widths, heights and max_global_tiles > 0");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (!(options.near_plane > 0.0f) || !(options.far_plane > options.near_plane)) {
+    set_error("expected 0 < near_plane < far_plane; got %g, %g", options.near_plane, options.far_plane);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (!(options.gaussian_support_sigma > 0.0f) || !(options.covariance_epsilon >= 0.0f)) {
+    set_error("gaussian_support_sigma must be positive and covariance_epsilon non-negative");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const int digit_bits = resolve_digit_bits(options);
+  if (digit_bits < 4 || digit_bits > 8) {
+    set_error("sort_digit_bits must be in [4, 8]; got %d", options.sort_digit_bits);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const int batch = options.raster_batch == 0 ? 256 : options.raster_batch;
+  if (batch != 64 && batch != 128 && batch != 256 && batch != 512) {
+    set_error("raster_batch must be 64, 128, 256 or 512; got %d", options.raster_batch);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const int group = resolve_cameras_per_block(options);
+  if (group < 1 || group > 64) {
+    set_error("cameras_per_block must be in [1, 64]; got %d", options.cameras_per_block);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  return 0;
+}
+
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+  va_list args;
+  va_start(args, fmt);
+  vsnprintf(g_error, sizeof(g_error), fmt, args);
+  va_end(args);
+}
+
+}  // namespace vgr
+
+using namespace vgr;
+
+extern "C" int32_t vgr_abi_version(void) { return VGR_ABI_VERSION; }
+
+extern "C" const char* vgr_last_error(void) { return g_error; }
+
+extern "C" int32_t vgr_emitted_sort_key_end_bit(int64_t global_tile_capacity) {
+  if (global_tile_capacity <= 0 || global_tile_capacity > 0x7fffffffLL) return -1;
+  return 32 + bit_length_u64(static_cast<uint64_t>(global_tile_capacity - 1));
+}
+
+extern "C" int32_t vgr_workspace_sizes_query(const vgr_cameras* cameras, const vgr_options* options,
+                                             int64_t scene_gaussians, int64_t intersection_capacity,
+                                             vgr_workspace_sizes* out) {
+  if (cameras == nullptr || options == nullptr || out == nullptr || scene_gaussians < 0 ||
+      intersection_capacity < 0) {
+    set_error("vgr_workspace_sizes_query: null or negative argument");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const int status = validate_options(*options, *cameras);
+  if (status != 0) return status;
+  const int64_t tiles = global_tile_count(*cameras, *options);
+  const ScratchLayout layout =
+      compute_layout(cameras->active_count, scene_gaussians, *options, tiles, intersection_capacity);
+  out->scratch_bytes = static_cast<int64_t>(layout.total);
+  out->tile_capacity = tiles;
+  out->sort_key_bits = layout.sort_key_bits;
+  out->sort_passes = layout.sort_passes;
+  return 0;
+}
+
+extern "C" int64_t vgr_sort_temp_bytes(int64_t n_items, int64_t n_segments) {
+  if (n_items < 0 || n_segments <= 0) return -1;
+  vgr_options options{};
+  options.tile_size = 16;
+  options.near_plane = 0.01f;
+  options.far_plane = 100.0f;
+  const ScratchLayout layout = compute_layout(0, 0, options, n_segments, n_items);
+  return static_cast<int64_t>(layout.total - layout.sort_histogram);
+}
+
+extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras, const vgr_outputs* outputs,
+                              const vgr_workspace* workspace, const vgr_options* options, void* stream_handle,
+                              int64_t* selector_out) {
+  if (scene == nullptr || cameras == nullptr || outputs == nullptr || workspace == nullptr || options == nullptr) {
+    set_error("vgr_render: null argument struct");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const vgr_scene& sc = *scene;
+  const vgr_cameras& cams = *cameras;
+  const vgr_outputs& out = *outputs;
+  const vgr_workspace& ws = *workspace;
+  const vgr_options& opt = *options;
+  hipStream_t stream = static_cast<hipStream_t>(stream_handle);
+  int status = validate_options(opt, cams);
+  if (status != 0) return status;
+  if (sc.gaussian_count < 0 || sc.gaussian_count > 0x7fffffffLL) {
+    set_error("gaussian_count must be in [0, INT32_MAX]; got %lld", static_cast<long long>(sc.gaussian_count));
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (sc.gaussian_count > 0 &&
+      (sc.means == nullptr || sc.covariances == nullptr || sc.opacities == nullptr || sc.features == nullptr ||
+       sc.semantic_ids == nullptr)) {
+    set_error("scene tensors must be non-null");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (sc.scene_count < 0 || (sc.scene_count > 0 && (sc.registered_scene_ids == nullptr || sc.scene_offsets == nullptr))) {
+    set_error("registered scene ids and offsets must be non-null");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (sc.feature_width < 3) {
+    set_error("features must provide at least RGB channels; got width %d", sc.feature_width);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (sc.sh_degree < 0 || sc.sh_degree > 3 || 3 * (sc.sh_degree + 1) * (sc.sh_degree + 1) > sc.feature_width) {
+    set_error("sh_degree %d needs %d feature channels; features have %d", sc.sh_degree,
+              3 * (sc.sh_degree + 1) * (sc.sh_degree + 1), sc.feature_width);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (cams.batch < 0 || cams.active_count < 0 || (cams.active_ids == nullptr && cams.active_count != cams.batch)) {
+    set_error("active_count must equal batch when active_ids is NULL");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (cams.active_count > 0 && cams.batch <= 0) {
+    set_error("active cameras given for an empty batch");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (cams.batch > 0 &&
+      (cams.viewmats == nullptr || cams.intrinsics == nullptr || cams.env_xforms == nullptr ||
+       cams.scene_ids == nullptr)) {
+    set_error("camera tensors must be non-null");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (out.rgb == nullptr ||
+      (opt.full_sensor_output && (out.depth == nullptr || out.alpha == nullptr || out.semantic == nullptr))) {
+    set_error("output tensors must be non-null for the requested output set");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (ws.records == nullptr || ws.keys_in == nullptr || ws.keys_out == nullptr || ws.values_in == nullptr ||
+      ws.values_out == nullptr || ws.tile_starts == nullptr || ws.tile_ends == nullptr ||
+      ws.counters == nullptr || ws.scratch == nullptr) {
+    set_error("workspace buffers must be non-null");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (ws.visible_capacity <= 0 || ws.visible_capacity > 0x7fffffffLL) {
+    set_error("visible capacity must be in [1, INT32_MAX]; got %lld", static_cast<long long>(ws.visible_capacity));
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (ws.intersection_capacity <= 0 || ws.intersection_capacity >= (1LL << 30)) {
+    set_error("intersection capacity must be in [1, 2^30); got %lld",
+              static_cast<long long>(ws.intersection_capacity));
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const int64_t tiles = global_tile_count(cams, opt);
+  if (tiles > 0x7fffffffLL) {
+    set_error("global tile count %lld exceeds the int32/uint32 key range", static_cast<long long>(tiles));
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (ws.tile_capacity < tiles) {
+    set_error("tile range workspace holds %lld tiles; the batch needs %lld", static_cast<long long>(ws.tile_capacity),
+              static_cast<long long>(tiles));
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const ScratchLayout layout =
+      compute_layout(cams.active_count, sc.gaussian_count, opt, tiles, ws.intersection_capacity);
+  if (ws.scratch_bytes < static_cast<int64_t>(layout.total)) {
+    set_error("scratch holds %lld bytes; the render needs %lld", static_cast<long long>(ws.scratch_bytes),
+              static_cast<long long>(layout.total));
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (layout.projection_blocks > 0x7fffffffLL || layout.sort_tiles > 0x7fffffffLL) {
+    set_error("projection or sort grid exceeds the launch limit");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  vgr_options resolved = opt;
+  resolved.sort_digit_bits = layout.sort_digit_bits;
+  resolved.cameras_per_block = layout.cameras_per_block;
+  if (resolved.raster_batch == 0) resolved.raster_batch = 256;
+
+  char* scratch = static_cast<char*>(ws.scratch);
+  CameraParams* table = reinterpret_cast<CameraParams*>(scratch + layout.camera_table);
+  if ((status = launch_camera_table(sc, cams, resolved, table, stream)) != 0) return status;
+  int sorted_in_out = 1;
+  if (!opt.reuse_projection) {
+    if ((status = launch_projection(sc, cams, resolved, ws, layout, scratch, stream)) != 0) return status;
+    if ((status = launch_sort(ws, layout, scratch, stream, &sorted_in_out)) != 0) return status;
+    if ((status = launch_ranges(ws, tiles, sorted_in_out != 0, stream)) != 0) return status;
+  }
+  if ((status = launch_composite(sc, cams, out, resolved, ws, layout, scratch, sorted_in_out != 0, stream)) != 0)
+    return status;
+  if (selector_out != nullptr) *selector_out = sorted_in_out;
+  return 0;
+}
+
+namespace vgr {
+namespace {
+
+__global__ void mark_logical_active_ids_kernel(const int64_t* __restrict__ active_ids, int64_t active_count,
+                                               int32_t* __restrict__ active_mask, int64_t logical_batch) {
+  const int64_t index = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (index >= active_count) return;
+  const int64_t camera = active_ids[index];
+  if (camera >= 0 && camera < logical_batch) active_mask[camera] = 1;
+}
+
+__global__ void build_chunk_active_ids_kernel(const int32_t* __restrict__ active_mask, int64_t logical_batch,
+                                              int64_t physical_batch, int64_t slot_count,
+                                              int64_t* __restrict__ chunk_active_ids) {
+  const int64_t slot = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (slot >= slot_count) return;
+  chunk_active_ids[slot] = (slot < logical_batch && active_mask[slot] != 0) ? slot % physical_batch : -1;
+}
+
+__global__ void aggregate_chunk_counters_kernel(const int64_t* __restrict__ chunk_counters, int64_t stride,
+                                                int64_t chunk_count, int64_t* __restrict__ counters,
+                                                int64_t* __restrict__ maxima) {
+  const int counter = threadIdx.x;
+  if (counter >= VGR_COUNTER_COUNT) return;
+  int64_t total = 0, maximum = 0;
+  for (int64_t chunk = 0; chunk < chunk_count; ++chunk) {
+    const int64_t value = chunk_counters[chunk * stride + counter];
+    total += value;
+    maximum = value > maximum ? value : maximum;
+  }
+  counters[counter] = total;
+  maxima[counter] = maximum;
+}
+
+__global__ void clear_mask_kernel(int32_t* __restrict__ mask, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) mask[i] = 0;
+}
+
+}  // namespace
+}  // namespace vgr
+
+extern "C" int32_t vgr_prepare_chunked_active_ids(const int64_t* active_ids, int64_t active_count,
+                                                  int32_t* active_mask, int64_t* chunk_active_ids,
+                                                  int64_t logical_batch, int64_t physical_batch, void* stream_handle) {
+  if (logical_batch <= 0 || physical_batch <= 0 || active_mask == nullptr || chunk_active_ids == nullptr ||
+      (active_count > 0 && active_ids == nullptr)) {
+    set_error("vgr_prepare_chunked_active_ids: invalid arguments");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  hipStream_t stream = static_cast<hipStream_t>(stream_handle);
+  const int64_t chunks = (logical_batch + physical_batch - 1) / physical_batch;
+  const int64_t slots = chunks * physical_batch;
+  hipLaunchKernelGGL(clear_mask_kernel, dim3(static_cast<unsigned>((logical_batch + 255) / 256)), dim3(256), 0,
+                     stream, active_mask, logical_batch);
+  if (active_count > 0) {
+    hipLaunchKernelGGL(mark_logical_active_ids_kernel, dim3(static_cast<unsigned>((active_count + 255) / 256)),
+                       dim3(256), 0, stream, active_ids, active_count, active_mask, logical_batch);
+  }
+  hipLaunchKernelGGL(build_chunk_active_ids_kernel, dim3(static_cast<unsigned>((slots + 255) / 256)), dim3(256), 0,
+                     stream, active_mask, logical_batch, physical_batch, slots, chunk_active_ids);
+  const hipError_t error = hipGetLastError();
+  if (error != hipSuccess) {
+    set_error("chunk active-id launch failed: %s", hipGetErrorString(error));
+    return VGR_E_LAUNCH;
+  }
+  return 0;
+}
+
+extern "C" int32_t vgr_aggregate_chunk_counters(const int64_t* chunk_counters, int64_t counter_stride,
+                                                int64_t chunk_count, int64_t* counters,
+                                                int64_t* physical_capacity_counters, void* stream_handle) {
+  if (chunk_counters == nullptr || counters == nullptr || physical_capacity_counters == nullptr ||
+      chunk_count <= 0 || counter_stride < VGR_COUNTER_COUNT) {
+    set_error("vgr_aggregate_chunk_counters: invalid arguments");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  hipLaunchKernelGGL(aggregate_chunk_counters_kernel, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream_handle),
+                     chunk_counters, counter_stride, chunk_count, counters, physical_capacity_counters);
+  const hipError_t error = hipGetLastError();
+  if (error != hipSuccess) {
+    set_error("counter aggregation launch failed: %s", hipGetErrorString(error));
+    return VGR_E_LAUNCH;
+  }
+  return 0;
+}
+
+extern "C" int32_t vgr_fill_i64(int64_t* data, int64_t n, int64_t value, void* stream) {
+  if (n < 0 || (n > 0 && data == nullptr)) {
+    set_error("vgr_fill_i64: invalid arguments");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  return launch_fill_i64(data, n, value, static_cast<hipStream_t>(stream));
+}
+
+namespace vgr {
+int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream);
+}
+
+extern "C" int32_t vgr_fill_f32(float* data, int64_t n, float value, void* stream) {
+  if (n < 0 || (n > 0 && data == nullptr)) {
+    set_error("vgr_fill_f32: invalid arguments");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  return vgr::launch_fill_f32(data, n, value, static_cast<hipStream_t>(stream));
+}

@@ -1,0 +1,276 @@
+// Fused RGB / expected-depth / alpha / semantic compositing for gfx950.
+//
+// Reference behaviour: composite_tiles_kernel<false,false,FULL>
+// (renderer_cuda.cu:1973-2305).  One workgroup renders one (active camera,
+// tile): tile_size^2 lanes, one pixel each (16x16 = four wave64s).  The
+// depth-sorted intersection list of the tile is consumed in batches: every
+// lane gathers whole 48-byte records (three 16-byte loads) for the next batch
+// into registers while the current batch is composited out of LDS, so the
+// gather latency overlaps the blend loop.  Inside the loop every lane reads
+// the same LDS address, which the LDS serves as a broadcast.
+//
+// The per-pixel arithmetic is the reference's, term by term, with vgr_expf in
+// place of __expf.  The semantic winner is tracked as a Gaussian index and the
+// int64 semantic label is gathered once per pixel at the end.
+#include "vgr_internal.h"
+
+namespace vgr {
+namespace {
+
+// Native 4-wide vector (first-class in LLVM, so per-thread arrays of it stay in
+// registers; HIP's float4 wrapper struct defeats scalar replacement).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct CompositeArgs {
+  const CameraParams* table;
+  const int32_t* tile_starts;
+  const int32_t* tile_ends;
+  const int32_t* sorted_values;
+  const vgr_record* records;
+  const int64_t* semantic_ids;
+  float* rgb;
+  float* depth;
+  float* alpha;
+  int64_t* semantic;
+  int64_t visible_capacity;
+  int64_t gaussian_count;
+  int active_count;
+  int uniform_tiles;     // tiles per camera when every camera has the same resolution, else 0
+  int output_srgb;
+  float semantic_min_alpha;
+};
+
+__device__ __forceinline__ int find_active_camera(const CameraParams* table, int active_count, int64_t block) {
+  int lo = 0, hi = active_count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (table[mid].active_tile_begin <= block) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
+}
+
+// Loads the records of one batch of the sorted list into registers.
+template <int kThreads, int kBatch, int kPerThread>
+__device__ __forceinline__ void gather_batch(const CompositeArgs& args, int64_t batch_start, int64_t range_end,
+                                             int lane, f32x4 (&a)[kPerThread], f32x4 (&b)[kPerThread],
+                                             f32x4 (&c)[kPerThread]) {
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const int64_t index = batch_start + k * kThreads + lane;
+    if (k * kThreads + lane < kBatch && index < range_end) {
+      const int32_t record = args.sorted_values[index];
+      if (record >= 0 && record < args.visible_capacity) {
+        const f32x4* src = reinterpret_cast<const f32x4*>(args.records + record);
+        a[k] = src[0];
+        b[k] = src[1];
+        c[k] = src[2];
+      } else {
+        // Only reachable with a corrupted list: contributes nothing (opacity 0).
+        a[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        b[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        c[k] = f32x4{0.0f, 0.0f, __int_as_float(-1), 0.0f};
+      }
+    }
+  }
+}
+
+template <int kTileSize, int kBatch, bool kFull>
+__global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(CompositeArgs args) {
+  constexpr int kThreads = kTileSize * kTileSize;
+  constexpr int kPerThread = (kBatch + kThreads - 1) / kThreads;
+  __shared__ f32x4 s_a[kBatch];  // mean_x, mean_y, conic_0, conic_1
+  __shared__ f32x4 s_b[kBatch];  // conic_2, opacity, depth, red
+  __shared__ f32x4 s_c[kBatch];  // green, blue, gaussian_id bits, camera bits
+
+  const int64_t block = blockIdx.x;
+  int active;
+  int64_t image_tile;
+  if (args.uniform_tiles > 0) {
+    active = static_cast<int>(block / args.uniform_tiles);
+    if (active >= args.active_count) return;
+    image_tile = block - static_cast<int64_t>(active) * args.uniform_tiles;
+  } else {
+    active = find_active_camera(args.table, args.active_count, block);
+    image_tile = block - args.table[active].active_tile_begin;
+    if (image_tile >= static_cast<int64_t>(args.table[active].tiles_x) * args.table[active].tiles_y) return;
+  }
+  const CameraParams& cam = args.table[active];
+  if (cam.camera < 0) return;
+  const int tile_x = static_cast<int>(image_tile % cam.tiles_x);
+  const int tile_y = static_cast<int>(image_tile / cam.tiles_x);
+  const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
+  const int lane = threadIdx.x;
+  const int pixel_x = tile_x * kTileSize + lane % kTileSize;
+  const int pixel_y = tile_y * kTileSize + lane / kTileSize;
+  const bool valid_pixel = pixel_x < cam.width && pixel_y < cam.height;
+  const float px = static_cast<float>(pixel_x) + 0.5f;
+  const float py = static_cast<float>(pixel_y) + 0.5f;
+
+  float transmittance = 1.0f;
+  float red = 0.0f, green = 0.0f, blue = 0.0f, depth_sum = 0.0f;
+  float best_weight = -1.0f;
+  int best_gaussian = -1;
+  bool done = !valid_pixel;
+
+  const int64_t range_start = args.tile_starts[global_tile];
+  const int64_t range_end = args.tile_ends[global_tile];
+  if (range_start >= 0 && range_end > range_start) {
+    f32x4 next_a[kPerThread] = {}, next_b[kPerThread] = {}, next_c[kPerThread] = {};
+    gather_batch<kThreads, kBatch, kPerThread>(args, range_start, range_end, lane, next_a, next_b, next_c);
+    for (int64_t batch_start = range_start; batch_start < range_end; batch_start += kBatch) {
+      const int count = static_cast<int>(range_end - batch_start < kBatch ? range_end - batch_start : kBatch);
+      __syncthreads();  // previous batch fully consumed
+#pragma unroll
+      for (int k = 0; k < kPerThread; ++k) {
+        const int slot = k * kThreads + lane;
+        if (slot < count) {
+          s_a[slot] = next_a[k];
+          s_b[slot] = next_b[k];
+          s_c[slot] = next_c[k];
+        }
+      }
+      __syncthreads();
+      if (batch_start + kBatch < range_end) {
+        gather_batch<kThreads, kBatch, kPerThread>(args, batch_start + kBatch, range_end, lane, next_a, next_b,
+                                                   next_c);
+      }
+      if (!done) {
+        for (int item = 0; item < count; ++item) {
+          const f32x4 a = s_a[item];
+          const float delta_x = px - a.x;
+          const float delta_y = py - a.y;
+          const f32x4 b = s_b[item];
+          const float power =
+              -0.5f * (a.z * delta_x * delta_x + 2.0f * a.w * delta_x * delta_y + b.x * delta_y * delta_y);
+          if (power > 0.0f || power < -20.0f) continue;
+          const float splat_alpha = fminf(VGR_MAX_ALPHA, b.y * vgr_expf(power));
+          if (splat_alpha < VGR_ALPHA_THRESHOLD) continue;
+          const float next_transmittance = transmittance * (1.0f - splat_alpha);
+          if (next_transmittance <= VGR_TRANSMITTANCE_THRESHOLD) {
+            done = true;
+            break;
+          }
+          const f32x4 c = s_c[item];
+          const float weight = transmittance * splat_alpha;
+          red = fmaf(weight, b.w, red);
+          green = fmaf(weight, c.x, green);
+          blue = fmaf(weight, c.y, blue);
+          if (kFull) {
+            depth_sum = fmaf(weight, b.z, depth_sum);
+            if (weight > best_weight) {
+              best_weight = weight;
+              best_gaussian = __float_as_int(c.z);
+            }
+          }
+          transmittance = next_transmittance;
+        }
+      }
+      if (!__syncthreads_or(done ? 0 : 1)) break;
+    }
+  }
+
+  if (!valid_pixel) return;
+  const int64_t pixel = cam.pixel_offset + static_cast<int64_t>(pixel_y) * cam.width + pixel_x;
+  const float accumulated_alpha = 1.0f - transmittance;
+  if (args.output_srgb) {
+    red = vgr_linear_to_srgb(red);
+    green = vgr_linear_to_srgb(green);
+    blue = vgr_linear_to_srgb(blue);
+  }
+  args.rgb[pixel * 3 + 0] = red;
+  args.rgb[pixel * 3 + 1] = green;
+  args.rgb[pixel * 3 + 2] = blue;
+  if (kFull) {
+    args.alpha[pixel] = accumulated_alpha;
+    args.depth[pixel] = accumulated_alpha > 1.0e-8f ? depth_sum / accumulated_alpha : __int_as_float(0x7f800000);
+    const bool labelled = best_weight >= 0.0f && accumulated_alpha >= args.semantic_min_alpha &&
+                          best_gaussian >= 0 && best_gaussian < args.gaussian_count;
+    args.semantic[pixel] = labelled ? args.semantic_ids[best_gaussian] : -1;
+  }
+}
+
+template <int kTileSize, int kBatch>
+void launch_full_or_rgb(bool full, const CompositeArgs& args, int64_t blocks, hipStream_t stream) {
+  if (full) {
+    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, true>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(kTileSize * kTileSize), 0, stream, args);
+  } else {
+    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, false>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(kTileSize * kTileSize), 0, stream, args);
+  }
+}
+
+template <int kTileSize>
+void launch_batch(int batch, bool full, const CompositeArgs& args, int64_t blocks, hipStream_t stream) {
+  switch (batch) {
+    case 64:
+      launch_full_or_rgb<kTileSize, 64>(full, args, blocks, stream);
+      break;
+    case 128:
+      launch_full_or_rgb<kTileSize, 128>(full, args, blocks, stream);
+      break;
+    case 512:
+      launch_full_or_rgb<kTileSize, 512>(full, args, blocks, stream);
+      break;
+    default:
+      launch_full_or_rgb<kTileSize, 256>(full, args, blocks, stream);
+      break;
+  }
+}
+
+}  // namespace
+
+int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_outputs& outputs,
+                     const vgr_options& options, const vgr_workspace& ws, const ScratchLayout& layout,
+                     const char* scratch, bool sorted_in_out, hipStream_t stream) {
+  if (cameras.active_count <= 0) return 0;
+  CompositeArgs args;
+  args.table = reinterpret_cast<const CameraParams*>(scratch + layout.camera_table);
+  args.tile_starts = ws.tile_starts;
+  args.tile_ends = ws.tile_ends;
+  args.sorted_values = sorted_in_out ? ws.values_out : ws.values_in;
+  args.records = ws.records;
+  args.semantic_ids = scene.semantic_ids;
+  args.rgb = outputs.rgb;
+  args.depth = outputs.depth;
+  args.alpha = outputs.alpha;
+  args.semantic = outputs.semantic;
+  args.visible_capacity = ws.visible_capacity;
+  args.gaussian_count = scene.gaussian_count;
+  args.active_count = cameras.active_count;
+  args.output_srgb = options.output_srgb;
+  args.semantic_min_alpha = options.semantic_min_alpha;
+  int64_t blocks;
+  if (cameras.widths == nullptr) {
+    const int tiles_x = (options.width + options.tile_size - 1) / options.tile_size;
+    const int tiles_y = (options.height + options.tile_size - 1) / options.tile_size;
+    args.uniform_tiles = tiles_x * tiles_y;
+    blocks = static_cast<int64_t>(cameras.active_count) * args.uniform_tiles;
+  } else {
+    args.uniform_tiles = 0;
+    blocks = cameras.max_global_tiles;
+  }
+  if (blocks <= 0) return 0;
+  if (blocks > 0x7fffffffLL) {
+    set_error("composite grid of %lld tiles exceeds the launch limit", static_cast<long long>(blocks));
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const bool full = options.full_sensor_output != 0;
+  if (options.tile_size == 8) {
+    launch_batch<8>(options.raster_batch, full, args, blocks, stream);
+  } else {
+    launch_batch<16>(options.raster_batch, full, args, blocks, stream);
+  }
+  const hipError_t error = hipGetLastError();
+  if (error != hipSuccess) {
+    set_error("composite launch failed: %s", hipGetErrorString(error));
+    return VGR_E_LAUNCH;
+  }
+  return 0;
+}
+
+}  // namespace vgr

@@ -1,0 +1,93 @@
+// Internal declarations shared by the HIP translation units of libvgr_hip.
+#ifndef VGR_INTERNAL_H
+#define VGR_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/vgr.h"
+#include "vgr_math.h"
+
+namespace vgr {
+
+constexpr int kBlockThreads = 256;
+constexpr int kWave = 64;
+constexpr uint64_t kSentinelKey = ~0ull;
+
+// One row of the device camera table (256 bytes; read with uniform indices, so
+// the compiler keeps it in scalar registers).  Built on the device by
+// build_camera_table_kernel from the caller's camera tensors.
+struct alignas(16) CameraParams {
+  float env[12];        // rows 0..2 of the scene->world affine transform
+  float view[12];       // rows 0..2 of the world->camera affine transform
+  float rotation[9];    // R_view * R_env (reference multiply_rotation_3x3 order)
+  float focal_x, focal_y, center_x, center_y;
+  float limit_x_pos, limit_x_neg, limit_y_pos, limit_y_neg;
+  float origin_x, origin_y, origin_z;  // camera centre in scene coordinates (SH view direction)
+  int32_t camera;        // index into the batch
+  int32_t scene_slot;    // -1 when the camera's scene is not registered
+  int32_t width, height;
+  int32_t tiles_x, tiles_y;
+  uint32_t tile_base;    // global tile index of this camera's tile (0, 0)
+  int32_t pad0;
+  int64_t pixel_offset;  // first output pixel of this camera
+  int64_t gaussian_begin, gaussian_end;  // packed Gaussian range of its scene
+  int64_t active_tile_begin;  // tiles of the active cameras before this one
+  int32_t pad1[2];
+};
+static_assert(sizeof(CameraParams) == 256, "camera table row must be 256 bytes");
+
+// Scratch layout inside workspace.scratch (all offsets 256-byte aligned).
+struct ScratchLayout {
+  size_t camera_table;     // CameraParams[active_count]
+  size_t block_counts;     // int64[2 * projection_blocks]
+  size_t block_bases;      // int64[2 * projection_blocks]
+  size_t sort_histogram;   // uint32[passes * 256]
+  size_t sort_digit_base;  // uint32[passes * 256]
+  size_t sort_tickets;     // uint32[passes]
+  size_t sort_status;      // uint32[4]: [0] lookback spin overflow
+  size_t sort_flags;       // uint32[passes * sort_tiles * 256]
+  size_t total;
+  int64_t projection_blocks;
+  int64_t projection_chunks;
+  int32_t projection_groups;
+  int32_t cameras_per_block;
+  int64_t sort_tiles;
+  int32_t sort_passes;
+  int32_t sort_digit_bits;
+  int32_t sort_key_bits;
+  int32_t tile_bits;
+  int32_t depth_bits;
+  uint32_t depth_base;     // float bits of the near plane
+};
+
+constexpr int kSortItems = 16;                           // keys per thread per sort tile
+constexpr int kSortTile = kBlockThreads * kSortItems;    // 4096 keys per sort tile
+
+// Host helpers implemented in vgr_capi.hip.
+void set_error(const char* fmt, ...);
+
+// Launch wrappers (each returns 0 or sets the error and returns nonzero).
+int launch_camera_table(const vgr_scene& scene, const vgr_cameras& cameras,
+                        const vgr_options& options, CameraParams* table,
+                        hipStream_t stream);
+int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras,
+                      const vgr_options& options, const vgr_workspace& ws,
+                      const ScratchLayout& layout, char* scratch,
+                      hipStream_t stream);
+int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scratch,
+                hipStream_t stream, int* result_in_out_buffer);
+int launch_ranges(const vgr_workspace& ws, int64_t global_tiles, bool sorted_in_out,
+                  hipStream_t stream);
+int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras,
+                     const vgr_outputs& outputs, const vgr_options& options,
+                     const vgr_workspace& ws, const ScratchLayout& layout,
+                     const char* scratch, bool sorted_in_out, hipStream_t stream);
+int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream);
+int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream);
+
+}  // namespace vgr
+
+#endif  // VGR_INTERNAL_H

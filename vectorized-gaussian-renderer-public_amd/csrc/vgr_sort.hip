@@ -1,0 +1,464 @@
+// Stable LSD radix ordering of tile intersections and tile-range marking.
+//
+// Replaces the reference's cub::DeviceRadixSort::SortPairs (DoubleBuffer) call
+// and mark_sorted_tile_ranges_kernel (renderer_cuda.cu:3215-3286, 1908-1942).
+//
+// Keys keep the reference layout (uint64 global_tile << 32 | float_bits(depth)).
+// The sort digits are taken from a compressed, order-preserving image of the
+// key: (tile << depth_bits) | (depth_bits_value - bits(near_plane)).  Every
+// emitted depth lies in [near, far], and positive floats order like their bit
+// patterns, so the image orders exactly like the key while needing
+// bit_length(GT-1) + bit_length(bits(far) - bits(near) + 1) bits instead of
+// 32 + bit_length(GT-1).  Sentinel keys (all ones) map to the all-ones image,
+// which is strictly larger than any valid image.
+//
+// Structure (onesweep):
+//   1. one histogram pass over the keys fills the digit histograms of every
+//      radix pass at once;
+//   2. per pass, each workgroup takes a ticket (its tile index), ranks its 4096
+//      keys stably in LDS (each wave ranks a contiguous 1024-key run with
+//      64-lane ballot matching), publishes its per-digit counts and resolves
+//      its global per-digit offsets by decoupled look-back over predecessor
+//      tickets, then writes the tile out digit-contiguously.
+// Ticket order guarantees every predecessor a workgroup waits on is already
+// resident; the look-back spin is additionally bounded and reports overflow in
+// a status word instead of hanging.
+// The element count is read from the device counter block, so the sequence
+// never synchronizes the host and can be captured in a graph.
+#include "vgr_internal.h"
+
+namespace vgr {
+namespace {
+
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagPrefix = 2u << 30;
+constexpr uint32_t kFlagValueMask = (1u << 30) - 1u;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+struct SortKeyParams {
+  int depth_bits;
+  uint32_t depth_base;
+  int key_bits;
+};
+
+__device__ __forceinline__ uint64_t compressed_key(uint64_t key, const SortKeyParams& p) {
+  if (key == kSentinelKey) return (1ull << p.key_bits) - 1ull;
+  const uint64_t tile = key >> 32;
+  const uint32_t depth = static_cast<uint32_t>(key) - p.depth_base;
+  return (tile << p.depth_bits) | static_cast<uint64_t>(depth);
+}
+
+__device__ __forceinline__ int64_t element_count(const int64_t* counters, int64_t capacity) {
+  int64_t n = counters[VGR_COUNTER_INTERSECTIONS];
+  n = n < 0 ? 0 : n;
+  return n < capacity ? n : capacity;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void sort_histogram_kernel(const uint64_t* __restrict__ keys,
+                                                                       const int64_t* __restrict__ counters,
+                                                                       int64_t capacity, SortKeyParams kp,
+                                                                       int digit_bits, int passes,
+                                                                       uint32_t* __restrict__ histogram) {
+  __shared__ uint32_t local[16 * 256];
+  const int radix = 1 << digit_bits;
+  for (int i = threadIdx.x; i < passes * radix; i += kBlockThreads) local[i] = 0;
+  __syncthreads();
+  const int64_t n = element_count(counters, capacity);
+  const uint32_t mask = static_cast<uint32_t>(radix - 1);
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kBlockThreads) {
+    const uint64_t ck = compressed_key(keys[i], kp);
+    for (int p = 0; p < passes; ++p) {
+      const uint32_t digit = static_cast<uint32_t>(ck >> (p * digit_bits)) & mask;
+      atomicAdd(&local[p * radix + digit], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < passes * radix; i += kBlockThreads) {
+    if (local[i] != 0) atomicAdd(&histogram[i], local[i]);
+  }
+}
+
+// Exclusive scan of every pass's histogram (one block, one digit per thread),
+// then clears the tickets and status words for the passes.
+__global__ __launch_bounds__(kBlockThreads) void sort_digit_scan_kernel(const uint32_t* __restrict__ histogram,
+                                                                        int digit_bits, int passes,
+                                                                        uint32_t* __restrict__ digit_base) {
+  __shared__ uint32_t wave_totals[kBlockThreads / kWave];
+  const int radix = 1 << digit_bits;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t value = threadIdx.x < radix ? histogram[p * radix + threadIdx.x] : 0u;
+    uint32_t inclusive = value;
+#pragma unroll
+    for (int offset = 1; offset < kWave; offset <<= 1) {
+      const uint32_t t = __shfl_up(inclusive, offset, kWave);
+      if (lane >= offset) inclusive += t;
+    }
+    if (lane == kWave - 1) wave_totals[wave] = inclusive;
+    __syncthreads();
+    uint32_t prefix = 0;
+    for (int w = 0; w < wave; ++w) prefix += wave_totals[w];
+    if (threadIdx.x < radix) digit_base[p * radix + threadIdx.x] = prefix + inclusive - value;
+    __syncthreads();
+  }
+}
+
+// Clears histogram, tickets, status and the look-back flags of the tiles the
+// device count needs (graph-safe replacement for hipMemsetAsync).
+__global__ __launch_bounds__(kBlockThreads) void sort_clear_kernel(const int64_t* __restrict__ counters,
+                                                                   int64_t capacity, int digit_bits, int passes,
+                                                                   int64_t max_tiles, uint32_t* __restrict__ histogram,
+                                                                   uint32_t* __restrict__ tickets,
+                                                                   uint32_t* __restrict__ status,
+                                                                   uint32_t* __restrict__ flags) {
+  const int radix = 1 << digit_bits;
+  const int64_t n = element_count(counters, capacity);
+  int64_t tiles = (n + kSortTile - 1) / kSortTile;
+  tiles = tiles < max_tiles ? tiles : max_tiles;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlockThreads;
+  const int64_t first = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x;
+  if (first < static_cast<int64_t>(passes) * radix) {
+    for (int64_t i = first; i < static_cast<int64_t>(passes) * radix; i += stride) histogram[i] = 0;
+  }
+  if (first < passes) tickets[first] = 0;
+  if (first < 4) status[first] = 0;
+  const int64_t per_pass = tiles * radix;
+  for (int p = 0; p < passes; ++p) {
+    uint32_t* pass_flags = flags + static_cast<int64_t>(p) * max_tiles * radix;
+    for (int64_t i = first; i < per_pass; i += stride) pass_flags[i] = 0;
+  }
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int lane = threadIdx.x & (kWave - 1);
+  return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+}
+
+template <int kDigitBits>
+__global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
+    const uint64_t* __restrict__ keys_in, const int32_t* __restrict__ values_in, uint64_t* __restrict__ keys_out,
+    int32_t* __restrict__ values_out, const int64_t* __restrict__ counters, int64_t capacity, SortKeyParams kp,
+    int shift, const uint32_t* __restrict__ digit_base, uint32_t* __restrict__ flags, uint32_t* __restrict__ ticket,
+    uint32_t* __restrict__ status) {
+  constexpr int kRadix = 1 << kDigitBits;
+  constexpr int kWaves = kBlockThreads / kWave;
+  constexpr int kWaveRun = kSortTile / kWaves;  // contiguous keys ranked by one wave
+  constexpr uint32_t kMask = kRadix - 1;
+  __shared__ uint64_t s_keys[kSortTile];
+  __shared__ int32_t s_values[kSortTile];
+  __shared__ uint32_t s_count[kWaves][kRadix];
+  __shared__ uint32_t s_local_start[kRadix];
+  __shared__ int64_t s_global_start[kRadix];
+  __shared__ uint32_t s_ticket;
+
+  if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1u);
+  for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlockThreads) (&s_count[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t tile = s_ticket;
+  const int64_t n = element_count(counters, capacity);
+  const int64_t tile_start = tile * kSortTile;
+  if (tile_start >= n) return;
+  const int valid_count = static_cast<int>(n - tile_start < kSortTile ? n - tile_start : kSortTile);
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+
+  uint64_t keys[kSortItems];
+  int32_t values[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int local = wave * kWaveRun + r * kWave + lane;
+    if (local < valid_count) {
+      keys[r] = keys_in[tile_start + local];
+      values[r] = values_in[tile_start + local];
+    } else {
+      keys[r] = 0;
+      values[r] = 0;
+    }
+  }
+  uint32_t digits[kSortItems];
+  uint32_t ranks[kSortItems];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int local = wave * kWaveRun + r * kWave + lane;
+    const bool valid = local < valid_count;
+    const uint32_t digit = valid ? static_cast<uint32_t>(compressed_key(keys[r], kp) >> shift) & kMask : 0u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kDigitBits; ++b) {
+      const bool bit = (digit >> b) & 1u;
+      const uint64_t vote = __ballot(bit);
+      peers &= bit ? vote : ~vote;
+    }
+    digits[r] = digit;
+    ranks[r] = 0;
+    if (valid) {
+      const uint32_t below = static_cast<uint32_t>(__popcll(peers & lt));
+      const uint32_t base = s_count[wave][digit];
+      ranks[r] = base + below;
+      if (below == 0) s_count[wave][digit] = base + static_cast<uint32_t>(__popcll(peers));
+    }
+  }
+  __syncthreads();
+
+  // Per-digit wave offsets (in place) and tile totals.
+  uint32_t total = 0;
+  if (threadIdx.x < kRadix) {
+    uint32_t running = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t c = s_count[w][threadIdx.x];
+      s_count[w][threadIdx.x] = running;
+      running += c;
+    }
+    total = running;
+    // Publish this tile's aggregate (tile 0 publishes its inclusive prefix).
+    const uint32_t flag = (tile == 0 ? kFlagPrefix : kFlagAggregate) | total;
+    __hip_atomic_store(&flags[tile * kRadix + threadIdx.x], flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // Exclusive scan of the digit totals (kRadix <= 256 = one per thread).
+  {
+    __shared__ uint32_t wave_totals[kWaves];
+    uint32_t inclusive = total;
+#pragma unroll
+    for (int offset = 1; offset < kWave; offset <<= 1) {
+      const uint32_t t = __shfl_up(inclusive, offset, kWave);
+      if (lane >= offset) inclusive += t;
+    }
+    if (lane == kWave - 1) wave_totals[wave] = inclusive;
+    __syncthreads();
+    uint32_t prefix = 0;
+    for (int w = 0; w < wave; ++w) prefix += wave_totals[w];
+    if (threadIdx.x < kRadix) s_local_start[threadIdx.x] = prefix + inclusive - total;
+  }
+  __syncthreads();
+
+  // Scatter into LDS in block-local sorted order.
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int local = wave * kWaveRun + r * kWave + lane;
+    if (local < valid_count) {
+      const uint32_t position = s_local_start[digits[r]] + s_count[wave][digits[r]] + ranks[r];
+      s_keys[position] = keys[r];
+      s_values[position] = values[r];
+    }
+  }
+
+  // Decoupled look-back for the global offset of every digit.
+  if (threadIdx.x < kRadix) {
+    uint32_t exclusive = 0;
+    if (tile > 0) {
+      int64_t predecessor = tile - 1;
+      uint32_t spins = 0;
+      while (true) {
+        const uint32_t flag = __hip_atomic_load(&flags[predecessor * kRadix + threadIdx.x], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t state = flag & ~kFlagValueMask;
+        if (state == 0u) {
+          if (++spins > kSpinLimit) {
+            atomicAdd(&status[0], 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        exclusive += flag & kFlagValueMask;
+        if (state == kFlagPrefix || predecessor == 0) break;
+        --predecessor;
+      }
+      __hip_atomic_store(&flags[tile * kRadix + threadIdx.x], kFlagPrefix | (exclusive + total), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_global_start[threadIdx.x] = static_cast<int64_t>(digit_base[threadIdx.x]) + exclusive -
+                                  static_cast<int64_t>(s_local_start[threadIdx.x]);
+  }
+  __syncthreads();
+
+  for (int j = threadIdx.x; j < valid_count; j += kBlockThreads) {
+    const uint64_t key = s_keys[j];
+    const uint32_t digit = static_cast<uint32_t>(compressed_key(key, kp) >> shift) & kMask;
+    const int64_t out = s_global_start[digit] + j;
+    // Always true for a consistent histogram; keeps a corrupted input in bounds.
+    if (out >= 0 && out < capacity) {
+      keys_out[out] = key;
+      values_out[out] = s_values[j];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlockThreads) void ranges_kernel(const uint64_t* __restrict__ keys,
+                                                               const int64_t* __restrict__ counters_in,
+                                                               int64_t capacity, int64_t global_tiles,
+                                                               int32_t* __restrict__ tile_starts,
+                                                               int32_t* __restrict__ tile_ends,
+                                                               int64_t* __restrict__ counters) {
+  const int64_t n = element_count(counters_in, capacity);
+  int64_t ends = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kBlockThreads) {
+    const uint64_t tile = keys[i] >> 32;
+    if (tile >= static_cast<uint64_t>(global_tiles)) continue;
+    const bool begins = i == 0 || (keys[i - 1] >> 32) != tile;
+    const bool finishes = i + 1 == n || (keys[i + 1] >> 32) != tile;
+    if (begins) tile_starts[tile] = static_cast<int32_t>(i);
+    if (finishes) {
+      tile_ends[tile] = static_cast<int32_t>(i + 1);
+      ++ends;
+    }
+  }
+  for (int offset = 32; offset > 0; offset >>= 1) ends += __shfl_xor(ends, offset, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && ends != 0) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(counters + VGR_COUNTER_ACTIVE_TILES),
+              static_cast<unsigned long long>(ends));
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ data, int64_t n, uint32_t value) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    data[i] = value;
+  }
+}
+
+__global__ void fill_i64_kernel(int64_t* __restrict__ data, int64_t n, int64_t value) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    data[i] = value;
+  }
+}
+
+__global__ void fill_f32_kernel(float* __restrict__ data, int64_t n, float value) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    data[i] = value;
+  }
+}
+
+unsigned fill_grid(int64_t n) {
+  int64_t blocks = (n + kBlockThreads - 1) / kBlockThreads;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  return static_cast<unsigned>(blocks);
+}
+
+template <int kDigitBits>
+void launch_pass(const uint64_t* keys_in, const int32_t* values_in, uint64_t* keys_out, int32_t* values_out,
+                 const int64_t* counters, int64_t capacity, SortKeyParams kp, int shift, const uint32_t* digit_base,
+                 uint32_t* flags, uint32_t* ticket, uint32_t* status, int64_t tiles, hipStream_t stream) {
+  hipLaunchKernelGGL(sort_onesweep_pass_kernel<kDigitBits>, dim3(static_cast<unsigned>(tiles)), dim3(kBlockThreads),
+                     0, stream, keys_in, values_in, keys_out, values_out, counters, capacity, kp, shift, digit_base,
+                     flags, ticket, status);
+}
+
+}  // namespace
+
+int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(fill_grid(n)), dim3(kBlockThreads), 0, stream, data, n, value);
+  return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
+}
+
+int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(fill_i64_kernel, dim3(fill_grid(n)), dim3(kBlockThreads), 0, stream, data, n, value);
+  return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
+}
+
+int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(fill_f32_kernel, dim3(fill_grid(n)), dim3(kBlockThreads), 0, stream, data, n, value);
+  return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
+}
+
+int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, hipStream_t stream,
+                int* result_in_out_buffer) {
+  const int digit_bits = layout.sort_digit_bits;
+  const int passes = layout.sort_passes;
+  const int radix = 1 << digit_bits;
+  uint32_t* histogram = reinterpret_cast<uint32_t*>(scratch + layout.sort_histogram);
+  uint32_t* digit_base = reinterpret_cast<uint32_t*>(scratch + layout.sort_digit_base);
+  uint32_t* tickets = reinterpret_cast<uint32_t*>(scratch + layout.sort_tickets);
+  uint32_t* status = reinterpret_cast<uint32_t*>(scratch + layout.sort_status);
+  uint32_t* flags = reinterpret_cast<uint32_t*>(scratch + layout.sort_flags);
+  SortKeyParams kp;
+  kp.depth_bits = layout.depth_bits;
+  kp.depth_base = layout.depth_base;
+  kp.key_bits = layout.sort_key_bits;
+  const int64_t capacity = ws.intersection_capacity;
+  const int64_t tiles = layout.sort_tiles;
+
+  hipLaunchKernelGGL(sort_clear_kernel, dim3(1024), dim3(kBlockThreads), 0, stream, ws.counters, capacity,
+                     digit_bits, passes, tiles, histogram, tickets, status, flags);
+  int64_t histogram_blocks = (capacity + kBlockThreads * 16 - 1) / (kBlockThreads * 16);
+  histogram_blocks = histogram_blocks > 2048 ? 2048 : (histogram_blocks < 1 ? 1 : histogram_blocks);
+  hipLaunchKernelGGL(sort_histogram_kernel, dim3(static_cast<unsigned>(histogram_blocks)), dim3(kBlockThreads), 0,
+                     stream, ws.keys_in, ws.counters, capacity, kp, digit_bits, passes, histogram);
+  hipLaunchKernelGGL(sort_digit_scan_kernel, dim3(1), dim3(kBlockThreads), 0, stream, histogram, digit_bits, passes,
+                     digit_base);
+  const uint64_t* source_keys = ws.keys_in;
+  const int32_t* source_values = ws.values_in;
+  uint64_t* target_keys = ws.keys_out;
+  int32_t* target_values = ws.values_out;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * digit_bits;
+    uint32_t* pass_flags = flags + static_cast<int64_t>(p) * tiles * radix;
+    const uint32_t* pass_base = digit_base + p * radix;
+    switch (digit_bits) {
+      case 4:
+        launch_pass<4>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+        break;
+      case 5:
+        launch_pass<5>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+        break;
+      case 6:
+        launch_pass<6>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+        break;
+      case 7:
+        launch_pass<7>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+        break;
+      default:
+        launch_pass<8>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+        break;
+    }
+    const uint64_t* next_keys = target_keys;
+    const int32_t* next_values = target_values;
+    target_keys = const_cast<uint64_t*>(source_keys);
+    target_values = const_cast<int32_t*>(source_values);
+    source_keys = next_keys;
+    source_values = next_values;
+  }
+  *result_in_out_buffer = (passes % 2) == 1 ? 1 : 0;
+  const hipError_t error = hipGetLastError();
+  if (error != hipSuccess) {
+    set_error("sort launch failed: %s", hipGetErrorString(error));
+    return VGR_E_LAUNCH;
+  }
+  return 0;
+}
+
+int launch_ranges(const vgr_workspace& ws, int64_t global_tiles, bool sorted_in_out, hipStream_t stream) {
+  if (global_tiles > 0) {
+    launch_fill_u32(reinterpret_cast<uint32_t*>(ws.tile_starts), global_tiles, 0u, stream);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(ws.tile_ends), global_tiles, 0u, stream);
+  }
+  const uint64_t* keys = sorted_in_out ? ws.keys_out : ws.keys_in;
+  int64_t blocks = (ws.intersection_capacity + kBlockThreads * 8 - 1) / (kBlockThreads * 8);
+  blocks = blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks);
+  hipLaunchKernelGGL(ranges_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0, stream, keys,
+                     ws.counters, ws.intersection_capacity, global_tiles, ws.tile_starts, ws.tile_ends, ws.counters);
+  const hipError_t error = hipGetLastError();
+  if (error != hipSuccess) {
+    set_error("tile range launch failed: %s", hipGetErrorString(error));
+    return VGR_E_LAUNCH;
+  }
+  return 0;
+}
+
+}  // namespace vgr

@@ -1,0 +1,424 @@
+"""ctypes binding of ``libvgr_hip.so`` (the gfx950 kernels behind a C ABI).
+
+The module object returned by :func:`load_native_module` exposes the same
+method names and positional argument order as the reference's pybind module
+(``src/isaacsim_gaussian_renderer/native/renderer.cpp:312-337``):
+``render``, ``sort_temp_bytes``, ``radix_sort_double_buffer_temp_bytes``,
+``precompute_covariances``, ``prepare_chunked_active_ids`` and
+``aggregate_chunk_counters``, so the backend's ``native_module`` seam is
+unchanged.  Tensors are handed over as raw device pointers; work is enqueued on
+the current torch stream of the tensors' device and nothing synchronizes the
+host.
+
+There is no fallback: if the library is missing, or a call fails, this module
+raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import struct
+from pathlib import Path
+from typing import Sequence
+
+import torch
+
+LIBRARY_NAME = "libvgr_hip.so"
+LIBRARY_PATH = Path(__file__).resolve().with_name(LIBRARY_NAME)
+
+# Workspace tensors passed positionally to render() (this build's layout; the
+# reference used 24 tensors, renderer_cuda.cu:73-98).
+WORKSPACE_TENSORS = (
+    "visible_records",
+    "keys_in",
+    "keys_out",
+    "values_in",
+    "values_out",
+    "tile_starts",
+    "tile_ends",
+    "counters",
+    "scratch",
+)
+RECORD_FLOATS = 12  # 48-byte visible record, see vgr_record in include/vgr.h
+
+
+class _Scene(ctypes.Structure):
+    _fields_ = [
+        ("means", ctypes.c_void_p),
+        ("covariances", ctypes.c_void_p),
+        ("opacities", ctypes.c_void_p),
+        ("features", ctypes.c_void_p),
+        ("semantic_ids", ctypes.c_void_p),
+        ("registered_scene_ids", ctypes.c_void_p),
+        ("scene_offsets", ctypes.c_void_p),
+        ("gaussian_count", ctypes.c_int64),
+        ("max_scene_gaussians", ctypes.c_int64),
+        ("feature_width", ctypes.c_int32),
+        ("scene_count", ctypes.c_int32),
+        ("sh_degree", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class _Cameras(ctypes.Structure):
+    _fields_ = [
+        ("viewmats", ctypes.c_void_p),
+        ("intrinsics", ctypes.c_void_p),
+        ("env_xforms", ctypes.c_void_p),
+        ("scene_ids", ctypes.c_void_p),
+        ("active_ids", ctypes.c_void_p),
+        ("batch", ctypes.c_int32),
+        ("active_count", ctypes.c_int32),
+        ("widths", ctypes.c_void_p),
+        ("heights", ctypes.c_void_p),
+        ("pixel_offsets", ctypes.c_void_p),
+        ("max_global_tiles", ctypes.c_int64),
+    ]
+
+
+class _Outputs(ctypes.Structure):
+    _fields_ = [
+        ("rgb", ctypes.c_void_p),
+        ("depth", ctypes.c_void_p),
+        ("alpha", ctypes.c_void_p),
+        ("semantic", ctypes.c_void_p),
+    ]
+
+
+class _Workspace(ctypes.Structure):
+    _fields_ = [
+        ("records", ctypes.c_void_p),
+        ("radii", ctypes.c_void_p),
+        ("keys_in", ctypes.c_void_p),
+        ("keys_out", ctypes.c_void_p),
+        ("values_in", ctypes.c_void_p),
+        ("values_out", ctypes.c_void_p),
+        ("tile_starts", ctypes.c_void_p),
+        ("tile_ends", ctypes.c_void_p),
+        ("counters", ctypes.c_void_p),
+        ("scratch", ctypes.c_void_p),
+        ("visible_capacity", ctypes.c_int64),
+        ("intersection_capacity", ctypes.c_int64),
+        ("tile_capacity", ctypes.c_int64),
+        ("scratch_bytes", ctypes.c_int64),
+    ]
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("near_plane", ctypes.c_float),
+        ("far_plane", ctypes.c_float),
+        ("gaussian_support_sigma", ctypes.c_float),
+        ("covariance_epsilon", ctypes.c_float),
+        ("semantic_min_alpha", ctypes.c_float),
+        ("antialiased", ctypes.c_int32),
+        ("tile_size", ctypes.c_int32),
+        ("output_srgb", ctypes.c_int32),
+        ("deterministic", ctypes.c_int32),
+        ("full_sensor_output", ctypes.c_int32),
+        ("fixed_capacity_sort", ctypes.c_int32),
+        ("sort_digit_bits", ctypes.c_int32),
+        ("raster_batch", ctypes.c_int32),
+        ("cameras_per_block", ctypes.c_int32),
+        ("reuse_projection", ctypes.c_int32),
+    ]
+
+
+class _Sizes(ctypes.Structure):
+    _fields_ = [
+        ("scratch_bytes", ctypes.c_int64),
+        ("tile_capacity", ctypes.c_int64),
+        ("sort_key_bits", ctypes.c_int32),
+        ("sort_passes", ctypes.c_int32),
+    ]
+
+
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_P = ctypes.c_void_p
+
+EXPORTED_SYMBOLS: dict[str, tuple[type, tuple]] = {
+    "vgr_abi_version": (_I32, ()),
+    "vgr_last_error": (ctypes.c_char_p, ()),
+    "vgr_emitted_sort_key_end_bit": (_I32, (_I64,)),
+    "vgr_workspace_sizes_query": (
+        _I32, (ctypes.POINTER(_Cameras), ctypes.POINTER(_Options), _I64, _I64, ctypes.POINTER(_Sizes))),
+    "vgr_sort_temp_bytes": (_I64, (_I64, _I64)),
+    "vgr_precompute_covariances": (_I32, (_P, _P, _I64, _P, _P)),
+    "vgr_render": (
+        _I32,
+        (ctypes.POINTER(_Scene), ctypes.POINTER(_Cameras), ctypes.POINTER(_Outputs), ctypes.POINTER(_Workspace),
+         ctypes.POINTER(_Options), _P, ctypes.POINTER(_I64)),
+    ),
+    "vgr_prepare_chunked_active_ids": (_I32, (_P, _I64, _P, _P, _I64, _I64, _P)),
+    "vgr_aggregate_chunk_counters": (_I32, (_P, _I64, _I64, _P, _P, _P)),
+    "vgr_fill_i64": (_I32, (_P, _I64, _I64, _P)),
+    "vgr_fill_f32": (_I32, (_P, _I64, ctypes.c_float, _P)),
+}
+
+
+def load_library(path: Path | str | None = None) -> ctypes.CDLL:
+    """Load the C-ABI library and declare every exported signature.
+
+    ``import torch`` has already run, so the HIP runtime torch ships is the
+    one this library binds to (both carry the soname libamdhip64.so.7).
+    """
+    target = Path(path) if path is not None else LIBRARY_PATH
+    if not target.is_file():
+        raise RuntimeError(
+            f"{target} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(or make -C vectorized-gaussian-renderer-public_amd/csrc) first.")
+    library = ctypes.CDLL(str(target), mode=os.RTLD_NOW | os.RTLD_LOCAL)
+    for name, (restype, argtypes) in EXPORTED_SYMBOLS.items():
+        function = getattr(library, name)
+        function.restype = restype
+        function.argtypes = list(argtypes)
+    return library
+
+
+def _ptr(tensor: torch.Tensor | None) -> int | None:
+    if tensor is None or tensor.numel() == 0:
+        return None
+    return tensor.data_ptr()
+
+
+def _stream(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def infer_sh_degree(feature_width: int) -> int:
+    """Largest SH degree whose bands fit in ``feature_width`` as 3*(d+1)^2 channels."""
+    bands = feature_width // 3
+    if bands * 3 != feature_width:
+        return 0
+    degree = int(math.isqrt(bands)) - 1
+    if (degree + 1) ** 2 != bands or degree > 3:
+        return 0
+    return degree
+
+
+class NativeRenderer:
+    """Python face of libvgr_hip with the reference native module's method names."""
+
+    # Intersection counter includes intersections of records that did not fit
+    # the visible capacity (the reference counts only stored records).
+    counts_full_intersection_demand = True
+
+    def __init__(self, library: ctypes.CDLL | None = None) -> None:
+        self.lib = library if library is not None else load_library()
+        version = int(self.lib.vgr_abi_version())
+        if version != 1:
+            raise RuntimeError(f"libvgr_hip ABI version {version} is not supported by this binding (expected 1).")
+
+    # -- helpers --------------------------------------------------------------------------
+    def _check(self, status: int, what: str) -> None:
+        if status != 0:
+            message = self.lib.vgr_last_error().decode("utf-8", "replace")
+            raise RuntimeError(f"{what} failed (status {status}): {message}")
+
+    @staticmethod
+    def _options(**values) -> _Options:
+        options = _Options()
+        for name, value in values.items():
+            setattr(options, name, value)
+        return options
+
+    # -- reference-compatible surface -----------------------------------------------------
+    def emitted_sort_key_end_bit(self, global_tile_capacity: int) -> int:
+        bit = int(self.lib.vgr_emitted_sort_key_end_bit(int(global_tile_capacity)))
+        if bit < 0:
+            raise ValueError("Global tile capacity must be in [1, INT32_MAX].")
+        return bit
+
+    def sort_temp_bytes(self, n_items: int, n_segments: int) -> int:
+        value = int(self.lib.vgr_sort_temp_bytes(int(n_items), int(n_segments)))
+        if value < 0:
+            raise ValueError("sort_temp_bytes: invalid arguments")
+        return value
+
+    radix_sort_double_buffer_temp_bytes = sort_temp_bytes
+
+    def scratch_bytes(self, *, max_views: int, height: int, width: int, gaussians: int, intersection_capacity: int,
+                      tile_size: int, near_plane: float, far_plane: float, sort_digit_bits: int = 0,
+                      cameras_per_block: int = 0, resolutions: Sequence[tuple[int, int]] | None = None) -> int:
+        """Scratch bytes one render of up to ``max_views`` cameras needs."""
+        cameras = _Cameras()
+        cameras.batch = max_views
+        cameras.active_count = max_views
+        if resolutions is not None:
+            cameras.widths = 1  # marks per-camera resolutions; only the tile total is read
+            cameras.heights = 1
+            cameras.max_global_tiles = sum(((w + tile_size - 1) // tile_size) * ((h + tile_size - 1) // tile_size)
+                                           for w, h in resolutions)
+        options = self._options(height=height, width=width, near_plane=near_plane, far_plane=far_plane,
+                                gaussian_support_sigma=3.0, covariance_epsilon=0.0, tile_size=tile_size,
+                                sort_digit_bits=sort_digit_bits, cameras_per_block=cameras_per_block)
+        sizes = _Sizes()
+        self._check(self.lib.vgr_workspace_sizes_query(ctypes.byref(cameras), ctypes.byref(options), int(gaussians),
+                                                        int(intersection_capacity), ctypes.byref(sizes)),
+                    "vgr_workspace_sizes_query")
+        return int(sizes.scratch_bytes)
+
+    def precompute_covariances(self, scales: torch.Tensor, rotations: torch.Tensor, out: torch.Tensor) -> None:
+        count = int(scales.shape[0])
+        if count == 0:
+            return
+        with torch.cuda.device(scales.device):
+            self._check(self.lib.vgr_precompute_covariances(_ptr(scales), _ptr(rotations), count, _ptr(out),
+                                                            _stream(scales.device)),
+                        "vgr_precompute_covariances")
+
+    def prepare_chunked_active_ids(self, active_ids: torch.Tensor, active_mask: torch.Tensor,
+                                   chunk_active_ids: torch.Tensor, logical_batch: int, physical_batch: int) -> None:
+        with torch.cuda.device(active_mask.device):
+            self._check(self.lib.vgr_prepare_chunked_active_ids(
+                _ptr(active_ids), int(active_ids.numel()), _ptr(active_mask), _ptr(chunk_active_ids),
+                int(logical_batch), int(physical_batch), _stream(active_mask.device)),
+                "vgr_prepare_chunked_active_ids")
+
+    def aggregate_chunk_counters(self, chunk_counters: torch.Tensor, counters: torch.Tensor,
+                                 physical_capacity_counters: torch.Tensor, chunk_count: int) -> None:
+        with torch.cuda.device(counters.device):
+            self._check(self.lib.vgr_aggregate_chunk_counters(
+                _ptr(chunk_counters), int(chunk_counters.stride(0)), int(chunk_count), _ptr(counters),
+                _ptr(physical_capacity_counters), _stream(counters.device)), "vgr_aggregate_chunk_counters")
+
+    def render(
+        self,
+        scene: Sequence[torch.Tensor],
+        cameras: Sequence[torch.Tensor],
+        outputs: Sequence[torch.Tensor],
+        workspace: Sequence[torch.Tensor],
+        height: int,
+        width: int,
+        max_scene_gaussians: int,
+        near_plane: float,
+        far_plane: float,
+        gaussian_support_sigma: float,
+        covariance_epsilon: float,
+        antialiased: bool,
+        semantic_min_alpha: float,
+        ray_gaussian_evaluation: bool,
+        tile_size: int,
+        depth_bucket_count: int,
+        depth_bucket_group_size: int,
+        compact_projection_cache: bool,
+        materialize_projected_records: bool,
+        reuse_projection: bool,
+        output_srgb: bool,
+        deterministic: bool,
+        full_sensor_output: bool,
+        fixed_capacity_sort: bool,
+        *,
+        sh_degree: int = 0,
+        sort_digit_bits: int = 0,
+        raster_batch: int = 0,
+        cameras_per_block: int = 0,
+        resolutions: tuple[torch.Tensor, torch.Tensor, torch.Tensor, int] | None = None,
+    ) -> int:
+        """One render submission; returns the sorted-buffer selector (1: *_out holds the result)."""
+        if ray_gaussian_evaluation:
+            raise ValueError("ray_gaussian_evaluation is not available on the gfx950 tile path.")
+        if compact_projection_cache or materialize_projected_records:
+            raise ValueError("The dense 1x1 compact projection path is not available on the gfx950 build.")
+        if len(scene) != 7 or len(cameras) != 5 or len(outputs) != 4 or len(workspace) != len(WORKSPACE_TENSORS):
+            raise ValueError("render expects 7 scene, 5 camera, 4 output and "
+                             f"{len(WORKSPACE_TENSORS)} workspace tensors.")
+        means, covariances, opacities, features, semantic_ids, registered, offsets = scene
+        viewmats, intrinsics, env_xforms, camera_scene_ids, active_ids = cameras
+        rgb, depth, alpha, semantic = outputs
+        records, keys_in, keys_out, values_in, values_out, tile_starts, tile_ends, counters, scratch = workspace
+        _require_dtypes(
+            (means, torch.float32, "means"), (covariances, torch.float32, "covariances"),
+            (opacities, torch.float32, "opacities"), (features, torch.float32, "features"),
+            (semantic_ids, torch.int64, "semantic_ids"), (registered, torch.int64, "registered_scene_ids"),
+            (offsets, torch.int64, "scene_offsets"), (viewmats, torch.float32, "viewmats"),
+            (intrinsics, torch.float32, "intrinsics"), (env_xforms, torch.float32, "env_xforms"),
+            (camera_scene_ids, torch.int64, "camera_scene_ids"), (active_ids, torch.int64, "active_camera_ids"),
+            (rgb, torch.float32, "rgb"), (depth, torch.float32, "depth"), (alpha, torch.float32, "alpha"),
+            (semantic, torch.int64, "semantic"), (records, torch.float32, "visible_records"),
+            (keys_in, torch.uint64, "keys_in"), (keys_out, torch.uint64, "keys_out"),
+            (values_in, torch.int32, "values_in"), (values_out, torch.int32, "values_out"),
+            (tile_starts, torch.int32, "tile_starts"), (tile_ends, torch.int32, "tile_ends"),
+            (counters, torch.int64, "counters"), (scratch, torch.uint8, "scratch"))
+        tensors = [t for t in (*scene, *cameras, *outputs, *workspace) if t.numel() > 0]
+        device = means.device
+        for tensor in tensors:
+            if tensor.device != device:
+                raise ValueError(f"all render tensors must be on {device}; found {tensor.device}.")
+            if not tensor.is_contiguous():
+                raise ValueError("all render tensors must be contiguous.")
+        if device.type != "cuda":
+            raise ValueError(f"render tensors must be on a GPU device; got {device}.")
+        if int(counters.numel()) < 5:
+            raise ValueError("counters workspace must contain at least five int64 values")
+        if env_xforms.shape[0] < viewmats.shape[0]:
+            raise ValueError("env_xforms must provide one transform per camera")
+
+        c_scene = _Scene(
+            means=_ptr(means), covariances=_ptr(covariances), opacities=_ptr(opacities), features=_ptr(features),
+            semantic_ids=_ptr(semantic_ids), registered_scene_ids=_ptr(registered), scene_offsets=_ptr(offsets),
+            gaussian_count=int(means.shape[0]), max_scene_gaussians=int(max_scene_gaussians),
+            feature_width=int(features.shape[1]) if features.ndim == 2 else 0,
+            scene_count=int(registered.numel()), sh_degree=int(sh_degree), reserved=0)
+        batch = int(viewmats.shape[0])
+        c_cameras = _Cameras(
+            viewmats=_ptr(viewmats), intrinsics=_ptr(intrinsics), env_xforms=_ptr(env_xforms),
+            scene_ids=_ptr(camera_scene_ids), active_ids=_ptr(active_ids), batch=batch,
+            active_count=int(active_ids.numel()) if active_ids.numel() else batch,
+            widths=None, heights=None, pixel_offsets=None, max_global_tiles=0)
+        if resolutions is not None:
+            widths, heights, pixel_offsets, total_tiles = resolutions
+            c_cameras.widths = _ptr(widths)
+            c_cameras.heights = _ptr(heights)
+            c_cameras.pixel_offsets = _ptr(pixel_offsets)
+            c_cameras.max_global_tiles = int(total_tiles)
+        c_outputs = _Outputs(rgb=_ptr(rgb), depth=_ptr(depth) if full_sensor_output else None,
+                             alpha=_ptr(alpha) if full_sensor_output else None,
+                             semantic=_ptr(semantic) if full_sensor_output else None)
+        c_workspace = _Workspace(
+            records=_ptr(records), radii=None, keys_in=_ptr(keys_in), keys_out=_ptr(keys_out),
+            values_in=_ptr(values_in), values_out=_ptr(values_out), tile_starts=_ptr(tile_starts),
+            tile_ends=_ptr(tile_ends), counters=_ptr(counters), scratch=_ptr(scratch),
+            visible_capacity=int(records.shape[0]), intersection_capacity=int(keys_in.numel()),
+            tile_capacity=int(tile_starts.numel()), scratch_bytes=int(scratch.numel()))
+        c_options = self._options(
+            height=int(height), width=int(width), near_plane=float(near_plane), far_plane=float(far_plane),
+            gaussian_support_sigma=float(gaussian_support_sigma), covariance_epsilon=float(covariance_epsilon),
+            semantic_min_alpha=float(semantic_min_alpha), antialiased=int(bool(antialiased)),
+            tile_size=int(tile_size), output_srgb=int(bool(output_srgb)), deterministic=int(bool(deterministic)),
+            full_sensor_output=int(bool(full_sensor_output)), fixed_capacity_sort=int(bool(fixed_capacity_sort)),
+            sort_digit_bits=int(sort_digit_bits), raster_batch=int(raster_batch),
+            cameras_per_block=int(cameras_per_block), reuse_projection=int(bool(reuse_projection)))
+        selector = _I64(1)
+        with torch.cuda.device(device):
+            self._check(self.lib.vgr_render(ctypes.byref(c_scene), ctypes.byref(c_cameras), ctypes.byref(c_outputs),
+                                            ctypes.byref(c_workspace), ctypes.byref(c_options), _stream(device),
+                                            ctypes.byref(selector)), "vgr_render")
+        return int(selector.value)
+
+
+def _require_dtypes(*items: tuple[torch.Tensor, torch.dtype, str]) -> None:
+    for tensor, dtype, name in items:
+        if tensor.dtype != dtype:
+            raise ValueError(f"{name} has dtype {tensor.dtype}, expected {dtype}")
+
+
+_MODULE: NativeRenderer | None = None
+
+
+def load_native_module() -> NativeRenderer:
+    """Load (once) and return the native renderer; raises if it is unavailable."""
+    global _MODULE
+    if _MODULE is None:
+        _MODULE = NativeRenderer()
+    return _MODULE
+
+
+def float_bits(value: float) -> int:
+    """IEEE-754 binary32 bit pattern of ``value``."""
+    return struct.unpack("<I", struct.pack("<f", value))[0]
