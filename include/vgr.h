@@ -145,6 +145,13 @@ typedef struct vgr_options {
   int32_t reuse_projection;         /* 1: composite only, reusing the sorted intersections and
                                        tile ranges of the previous render into this workspace
                                        (reference projection cache, cuda_backend.py:607-619) */
+  /* Optional instrumentation (NULL / 0 to disable). */
+  void* const* stage_events;        /* 5 hipEvent_t recorded on the stream: start, after projection +
+                                       emission, after sort, after tile ranges, after compositing */
+  int32_t stage_event_count;        /* 0 or 5 */
+  int32_t reserved0;
+  int64_t* statistics;              /* int64[4], zeroed then filled: [0] compositor (pixel, Gaussian)
+                                       evaluations; [1] sort look-back spin overflows */
 } vgr_options;
 
 typedef struct vgr_workspace_sizes {

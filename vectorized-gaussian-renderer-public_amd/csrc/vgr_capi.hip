@@ -276,17 +276,44 @@ extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras
   resolved.cameras_per_block = layout.cameras_per_block;
   if (resolved.raster_batch == 0) resolved.raster_batch = 256;
 
+  if (opt.stage_event_count != 0 && (opt.stage_event_count != 5 || opt.stage_events == nullptr)) {
+    set_error("stage_events must hold exactly 5 events");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  auto mark = [&](int stage) -> int {
+    if (opt.stage_event_count == 0) return 0;
+    const hipError_t error = hipEventRecord(static_cast<hipEvent_t>(opt.stage_events[stage]), stream);
+    if (error != hipSuccess) {
+      set_error("hipEventRecord(stage %d) failed: %s", stage, hipGetErrorString(error));
+      return VGR_E_LAUNCH;
+    }
+    return 0;
+  };
+
   char* scratch = static_cast<char*>(ws.scratch);
   CameraParams* table = reinterpret_cast<CameraParams*>(scratch + layout.camera_table);
+  if ((status = mark(0)) != 0) return status;
+  if (opt.statistics != nullptr && (status = launch_fill_i64(opt.statistics, 4, 0, stream)) != 0) return status;
   if ((status = launch_camera_table(sc, cams, resolved, table, stream)) != 0) return status;
   int sorted_in_out = 1;
   if (!opt.reuse_projection) {
     if ((status = launch_projection(sc, cams, resolved, ws, layout, scratch, stream)) != 0) return status;
+    if ((status = mark(1)) != 0) return status;
     if ((status = launch_sort(ws, layout, scratch, stream, &sorted_in_out)) != 0) return status;
+    if ((status = mark(2)) != 0) return status;
     if ((status = launch_ranges(ws, tiles, sorted_in_out != 0, stream)) != 0) return status;
+    if ((status = mark(3)) != 0) return status;
+  } else {
+    if ((status = mark(1)) != 0 || (status = mark(2)) != 0 || (status = mark(3)) != 0) return status;
   }
   if ((status = launch_composite(sc, cams, out, resolved, ws, layout, scratch, sorted_in_out != 0, stream)) != 0)
     return status;
+  if ((status = mark(4)) != 0) return status;
+  if (opt.statistics != nullptr && !opt.reuse_projection) {
+    if ((status = launch_copy_u32_to_i64(reinterpret_cast<const uint32_t*>(scratch + layout.sort_status),
+                                         opt.statistics + 1, stream)) != 0)
+      return status;
+  }
   if (selector_out != nullptr) *selector_out = sorted_in_out;
   return 0;
 }
@@ -384,10 +411,6 @@ extern "C" int32_t vgr_fill_i64(int64_t* data, int64_t n, int64_t value, void* s
     return VGR_E_INVALID_ARGUMENT;
   }
   return launch_fill_i64(data, n, value, static_cast<hipStream_t>(stream));
-}
-
-namespace vgr {
-int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream);
 }
 
 extern "C" int32_t vgr_fill_f32(float* data, int64_t n, float value, void* stream) {

@@ -32,6 +32,7 @@ struct CompositeArgs {
   float* depth;
   float* alpha;
   int64_t* semantic;
+  int64_t* statistics;   // optional: [0] += evaluated (pixel, Gaussian) pairs
   int64_t visible_capacity;
   int64_t gaussian_count;
   int active_count;
@@ -78,7 +79,7 @@ __device__ __forceinline__ void gather_batch(const CompositeArgs& args, int64_t 
   }
 }
 
-template <int kTileSize, int kBatch, bool kFull>
+template <int kTileSize, int kBatch, bool kFull, bool kCount>
 __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(CompositeArgs args) {
   constexpr int kThreads = kTileSize * kTileSize;
   constexpr int kPerThread = (kBatch + kThreads - 1) / kThreads;
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(Composi
   float best_weight = -1.0f;
   int best_gaussian = -1;
   bool done = !valid_pixel;
+  int evaluations = 0;
 
   const int64_t range_start = args.tile_starts[global_tile];
   const int64_t range_end = args.tile_ends[global_tile];
@@ -140,6 +142,7 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(Composi
       }
       if (!done) {
         for (int item = 0; item < count; ++item) {
+          if (kCount) ++evaluations;
           const f32x4 a = s_a[item];
           const float delta_x = px - a.x;
           const float delta_y = py - a.y;
@@ -173,6 +176,13 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(Composi
     }
   }
 
+  if (kCount) {
+    int64_t total = evaluations;
+    for (int offset = 32; offset > 0; offset >>= 1) total += __shfl_xor(total, offset, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && total != 0) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(args.statistics), static_cast<unsigned long long>(total));
+    }
+  }
   if (!valid_pixel) return;
   const int64_t pixel = cam.pixel_offset + static_cast<int64_t>(pixel_y) * cam.width + pixel_x;
   const float accumulated_alpha = 1.0f - transmittance;
@@ -195,12 +205,18 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(Composi
 
 template <int kTileSize, int kBatch>
 void launch_full_or_rgb(bool full, const CompositeArgs& args, int64_t blocks, hipStream_t stream) {
-  if (full) {
-    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, true>), dim3(static_cast<unsigned>(blocks)),
-                       dim3(kTileSize * kTileSize), 0, stream, args);
+  const dim3 grid(static_cast<unsigned>(blocks));
+  const dim3 block(kTileSize * kTileSize);
+  if (args.statistics != nullptr) {
+    if (full) {
+      hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, true, true>), grid, block, 0, stream, args);
+    } else {
+      hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, false, true>), grid, block, 0, stream, args);
+    }
+  } else if (full) {
+    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, true, false>), grid, block, 0, stream, args);
   } else {
-    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, false>), dim3(static_cast<unsigned>(blocks)),
-                       dim3(kTileSize * kTileSize), 0, stream, args);
+    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, false, false>), grid, block, 0, stream, args);
   }
 }
 
@@ -239,6 +255,7 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const v
   args.depth = outputs.depth;
   args.alpha = outputs.alpha;
   args.semantic = outputs.semantic;
+  args.statistics = options.statistics;
   args.visible_capacity = ws.visible_capacity;
   args.gaussian_count = scene.gaussian_count;
   args.active_count = cameras.active_count;

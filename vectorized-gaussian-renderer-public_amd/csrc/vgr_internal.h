@@ -87,6 +87,8 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras,
                      const char* scratch, bool sorted_in_out, hipStream_t stream);
 int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream);
 int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream);
+int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream);
+int launch_copy_u32_to_i64(const uint32_t* source, int64_t* target, hipStream_t stream);
 
 }  // namespace vgr
 

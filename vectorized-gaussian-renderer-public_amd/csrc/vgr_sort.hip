@@ -336,6 +336,10 @@ __global__ void fill_f32_kernel(float* __restrict__ data, int64_t n, float value
   }
 }
 
+__global__ void copy_u32_to_i64_kernel(const uint32_t* __restrict__ source, int64_t* __restrict__ target) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *target = static_cast<int64_t>(*source);
+}
+
 unsigned fill_grid(int64_t n) {
   int64_t blocks = (n + kBlockThreads - 1) / kBlockThreads;
   if (blocks > 4096) blocks = 4096;
@@ -357,6 +361,11 @@ void launch_pass(const uint64_t* keys_in, const int32_t* values_in, uint64_t* ke
 int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(fill_grid(n)), dim3(kBlockThreads), 0, stream, data, n, value);
+  return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
+}
+
+int launch_copy_u32_to_i64(const uint32_t* source, int64_t* target, hipStream_t stream) {
+  hipLaunchKernelGGL(copy_u32_to_i64_kernel, dim3(1), dim3(64), 0, stream, source, target);
   return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
 }
 

@@ -205,6 +205,10 @@ class CustomHipBackend:
         self.cameras_per_block = cameras_per_block
         self._native = native_module
         self._native_loader = native_loader
+        # Optional instrumentation forwarded to the native render: five recorded
+        # torch events (stage boundaries) and an int64[4] statistics tensor.
+        self.stage_events: list[torch.cuda.Event] | None = None
+        self.statistics: torch.Tensor | None = None
         self.allow_cpu_for_tests = allow_cpu_for_tests
 
         self.device: torch.device | None = None
@@ -462,6 +466,11 @@ class CustomHipBackend:
     def _native_render(self, scene, cameras, outputs, workspace, *, reuse_projection: bool,
                        full_sensor_output: bool) -> int:
         assert self._native is not None
+        instrumentation: dict[str, Any] = {}
+        if self.stage_events is not None:
+            instrumentation["stage_events"] = self.stage_events
+        if self.statistics is not None:
+            instrumentation["statistics"] = self.statistics
         selector = int(self._native.render(
             scene, cameras, outputs, workspace, self._height, self._width, self._max_scene_gaussians,
             self.near_plane, self.far_plane, self.gaussian_support_sigma, self.covariance_epsilon,
@@ -469,7 +478,7 @@ class CustomHipBackend:
             self.depth_bucket_count, self.depth_bucket_group_size, False, False, reuse_projection,
             self.output_srgb, self.deterministic, full_sensor_output, self.fixed_capacity_sort,
             sh_degree=self._resolved_sh_degree, sort_digit_bits=self.sort_digit_bits,
-            raster_batch=self.raster_batch, cameras_per_block=self.cameras_per_block))
+            raster_batch=self.raster_batch, cameras_per_block=self.cameras_per_block, **instrumentation))
         if selector not in (0, 1):
             raise RuntimeError(f"Native renderer returned an invalid sorted-buffer selector: {selector}.")
         if selector == 0:

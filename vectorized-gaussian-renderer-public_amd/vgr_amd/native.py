@@ -125,6 +125,10 @@ class _Options(ctypes.Structure):
         ("raster_batch", ctypes.c_int32),
         ("cameras_per_block", ctypes.c_int32),
         ("reuse_projection", ctypes.c_int32),
+        ("stage_events", ctypes.POINTER(ctypes.c_void_p)),
+        ("stage_event_count", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("statistics", ctypes.c_void_p),
     ]
 
 
@@ -319,8 +323,15 @@ class NativeRenderer:
         raster_batch: int = 0,
         cameras_per_block: int = 0,
         resolutions: tuple[torch.Tensor, torch.Tensor, torch.Tensor, int] | None = None,
+        stage_events: Sequence[torch.cuda.Event] | None = None,
+        statistics: torch.Tensor | None = None,
     ) -> int:
-        """One render submission; returns the sorted-buffer selector (1: *_out holds the result)."""
+        """One render submission; returns the sorted-buffer selector (1: *_out holds the result).
+
+        ``stage_events`` (5 recorded torch events) are re-recorded at the stage
+        boundaries on the render stream; ``statistics`` (int64[4] on the device)
+        receives the compositor's evaluation count.
+        """
         if ray_gaussian_evaluation:
             raise ValueError("ray_gaussian_evaluation is not available on the gfx950 tile path.")
         if compact_projection_cache or materialize_projected_records:
@@ -394,6 +405,17 @@ class NativeRenderer:
             full_sensor_output=int(bool(full_sensor_output)), fixed_capacity_sort=int(bool(fixed_capacity_sort)),
             sort_digit_bits=int(sort_digit_bits), raster_batch=int(raster_batch),
             cameras_per_block=int(cameras_per_block), reuse_projection=int(bool(reuse_projection)))
+        event_array = None
+        if stage_events is not None:
+            if len(stage_events) != 5:
+                raise ValueError("stage_events must hold 5 events")
+            event_array = (ctypes.c_void_p * 5)(*[int(e.cuda_event) for e in stage_events])
+            c_options.stage_events = ctypes.cast(event_array, ctypes.POINTER(ctypes.c_void_p))
+            c_options.stage_event_count = 5
+        if statistics is not None:
+            if statistics.dtype != torch.int64 or statistics.numel() < 4 or statistics.device != device:
+                raise ValueError("statistics must be an int64[4] tensor on the render device")
+            c_options.statistics = statistics.data_ptr()
         selector = _I64(1)
         with torch.cuda.device(device):
             self._check(self.lib.vgr_render(ctypes.byref(c_scene), ctypes.byref(c_cameras), ctypes.byref(c_outputs),
