@@ -45,6 +45,8 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--sort-digit-bits", type=int, default=0)
     parser.add_argument("--raster-batch", type=int, default=0)
     parser.add_argument("--cameras-per-block", type=int, default=0)
+    parser.add_argument("--copy-calibration-mib", type=int, default=0,
+                        help="also run one device-to-device copy of this size (PMC byte-counter calibration)")
     return parser.parse_args()
 
 
@@ -77,6 +79,14 @@ def main() -> int:
     scene_ids = torch.zeros(args.cameras, dtype=torch.int64, device=device)
     outputs = service.prepare_outputs(args.cameras)
     print(f"[bench] setup done: {args.cameras} cameras, {args.gaussians} Gaussians", file=sys.stderr, flush=True)
+    if args.copy_calibration_mib > 0:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give the
+        # scale between those counters and bytes moved.
+        source = torch.ones(args.copy_calibration_mib * (1 << 20) // 4, dtype=torch.float32, device=device)
+        target = torch.empty_like(source)
+        target.copy_(source)
+        torch.cuda.synchronize()
+        del source, target
 
     for _ in range(args.warmup):
         service.render(viewmats, intrinsics, scene_ids, outputs=outputs)
