@@ -45,6 +45,7 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--sort-digit-bits", type=int, default=0)
     parser.add_argument("--raster-batch", type=int, default=0)
     parser.add_argument("--cameras-per-block", type=int, default=0)
+    parser.add_argument("--sort-tile-items", type=int, default=0)
     parser.add_argument("--copy-calibration-mib", type=int, default=0,
                         help="also run one device-to-device copy of this size (PMC byte-counter calibration)")
     return parser.parse_args()
@@ -70,7 +71,8 @@ def main() -> int:
     rig = synthetic.orbit_rig(args.cameras, width, height, focal, seed=seed, radius=(10.0, 15.0))
     backend = CustomHipBackend(output_srgb=False, deterministic=True, visible_capacity_per_view=1_000_000,
                                intersections_per_visible=2.6, sort_digit_bits=args.sort_digit_bits,
-                               raster_batch=args.raster_batch, cameras_per_block=args.cameras_per_block)
+                               raster_batch=args.raster_batch, cameras_per_block=args.cameras_per_block,
+                               sort_tile_items=args.sort_tile_items)
     service = RendererService(backend, height=height, width=width, max_views=args.cameras)
     service.initialize(stage=None, device=device)
     service.load_scene(0, **scene.to(device).as_kwargs())

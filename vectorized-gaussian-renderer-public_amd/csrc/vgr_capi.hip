@@ -61,7 +61,8 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   layout.sort_key_bits = layout.tile_bits + layout.depth_bits;
   layout.sort_digit_bits = resolve_digit_bits(options);
   layout.sort_passes = (layout.sort_key_bits + layout.sort_digit_bits - 1) / layout.sort_digit_bits;
-  layout.sort_tiles = (intersection_capacity + kSortTile - 1) / kSortTile;
+  layout.sort_tile = kBlockThreads * (options.sort_tile_items == 0 ? kDefaultSortItems : options.sort_tile_items);
+  layout.sort_tiles = (intersection_capacity + layout.sort_tile - 1) / layout.sort_tile;
   const int64_t radix = 1 << layout.sort_digit_bits;
 
   size_t offset = 0;
@@ -116,6 +117,10 @@ int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
   const int batch = options.raster_batch == 0 ? 256 : options.raster_batch;
   if (batch != 64 && batch != 128 && batch != 256 && batch != 512) {
     set_error("raster_batch must be 64, 128, 256 or 512; got %d", options.raster_batch);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (options.sort_tile_items != 0 && options.sort_tile_items != 8 && options.sort_tile_items != 16) {
+    set_error("sort_tile_items must be 0 (default), 8 or 16; got %d", options.sort_tile_items);
     return VGR_E_INVALID_ARGUMENT;
   }
   const int group = resolve_cameras_per_block(options);

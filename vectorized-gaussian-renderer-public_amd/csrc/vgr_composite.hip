@@ -79,13 +79,66 @@ __device__ __forceinline__ void gather_batch(const CompositeArgs& args, int64_t 
   }
 }
 
+// Margin subtracted from the per-Gaussian log-threshold: far larger than the
+// error of vgr_logf/vgr_expf and of the alpha product, so "power below the
+// threshold" implies the exact alpha test fails.
+constexpr float kThresholdMargin = 1.0e-4f;
+
+// Staged form of one record.  The power of the reference,
+//   -0.5 * (c0*dx*dx + 2*c1*dx*dy + c2*dy*dy),
+// is evaluated as (A*dx*dx + B*dx*dy) + C*dy*dy with A = -0.5*c0, B = -c1,
+// C = -0.5*c2: scaling by powers of two commutes with rounding, so the value is
+// bit-identical and two multiplies cheaper.
+struct StagedSplat {
+  f32x4 geometry;  // mean_x, mean_y, A, B
+  f32x4 shading;   // C, opacity, power threshold, depth
+  f32x4 colour;    // red, green, blue, gaussian id bits
+};
+
+__device__ __forceinline__ StagedSplat stage_record(const f32x4& a, const f32x4& b, const f32x4& c) {
+  StagedSplat s;
+  const float opacity = b.y;
+  // power < threshold  =>  min(0.99, opacity * exp(power)) < 1/255 exactly.
+  const float threshold = opacity > 0.0f ? vgr_logf(VGR_ALPHA_THRESHOLD / opacity) - kThresholdMargin
+                                         : __int_as_float(0x7f800000);
+  s.geometry = f32x4{a.x, a.y, -0.5f * a.z, -a.w};
+  s.shading = f32x4{-0.5f * b.x, opacity, threshold, b.z};
+  s.colour = f32x4{b.w, c.x, c.y, c.z};
+  return s;
+}
+
+// Half extents of a box containing every pixel offset whose float-evaluated
+// power can reach the threshold: the ellipse d^T C d <= R^2 (R^2 = -2 threshold)
+// inflated by a bound on the rounding error of the power expression, which
+// grows with the conic's condition.  Returns false when no finite box is
+// certain (non-positive-definite or extremely anisotropic conic).
+__device__ __forceinline__ bool splat_extent(const f32x4& a, const f32x4& b, float threshold, float& half_x,
+                                             float& half_y) {
+  const double c0 = a.z, c1 = a.w, c2 = b.x;
+  const double det = c0 * c2 - c1 * c1;
+  if (!(det > 0.0) || !(threshold < 0.0f)) return false;
+  const double sxx = c2 / det;
+  const double syy = c0 / det;
+  const double condition = (fabs(c0) + 2.0 * fabs(c1) + fabs(c2)) * (sxx + syy);
+  if (!(condition < 1.0e5)) return false;
+  const double r2 = -2.0 * static_cast<double>(threshold) * (1.0 + 1.0e-6 * condition) * 1.001 + 1.0e-6;
+  half_x = static_cast<float>(sqrt(r2 * sxx)) + 1.0e-3f;
+  half_y = static_cast<float>(sqrt(r2 * syy)) + 1.0e-3f;
+  return true;
+}
+
 template <int kTileSize, int kBatch, bool kFull, bool kCount>
 __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(CompositeArgs args) {
   constexpr int kThreads = kTileSize * kTileSize;
+  constexpr int kWaves = kThreads / kWave;
+  constexpr int kRowsPerWave = kWave / kTileSize;
   constexpr int kPerThread = (kBatch + kThreads - 1) / kThreads;
-  __shared__ f32x4 s_a[kBatch];  // mean_x, mean_y, conic_0, conic_1
-  __shared__ f32x4 s_b[kBatch];  // conic_2, opacity, depth, red
-  __shared__ f32x4 s_c[kBatch];  // green, blue, gaussian_id bits, camera bits
+  static_assert(kThreads % kWave == 0, "tile must be a whole number of waves");
+  __shared__ f32x4 s_geometry[kBatch];
+  __shared__ f32x4 s_shading[kBatch];
+  __shared__ f32x4 s_colour[kBatch];
+  __shared__ uint16_t s_list[kWaves][kBatch];  // per-wave ordered sub-lists of batch slots
+  __shared__ uint32_t s_totals[kPerThread][kWaves][kWaves];  // [slice][source wave][list]
 
   const int64_t block = blockIdx.x;
   int active;
@@ -105,11 +158,17 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(Composi
   const int tile_y = static_cast<int>(image_tile / cam.tiles_x);
   const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
   const int lane = threadIdx.x;
+  const int wave = lane / kWave;
+  const int wave_lane = lane & (kWave - 1);
   const int pixel_x = tile_x * kTileSize + lane % kTileSize;
   const int pixel_y = tile_y * kTileSize + lane / kTileSize;
   const bool valid_pixel = pixel_x < cam.width && pixel_y < cam.height;
   const float px = static_cast<float>(pixel_x) + 0.5f;
   const float py = static_cast<float>(pixel_y) + 0.5f;
+  // Pixel-centre rectangle of tile column span and of each wave's rows.
+  const float rect_x0 = static_cast<float>(tile_x * kTileSize) + 0.5f;
+  const float rect_x1 = rect_x0 + static_cast<float>(kTileSize - 1);
+  const float rect_y_base = static_cast<float>(tile_y * kTileSize) + 0.5f;
 
   float transmittance = 1.0f;
   float red = 0.0f, green = 0.0f, blue = 0.0f, depth_sum = 0.0f;
@@ -126,13 +185,59 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(Composi
     for (int64_t batch_start = range_start; batch_start < range_end; batch_start += kBatch) {
       const int count = static_cast<int>(range_end - batch_start < kBatch ? range_end - batch_start : kBatch);
       __syncthreads();  // previous batch fully consumed
+      uint32_t wave_bits[kPerThread];
 #pragma unroll
       for (int k = 0; k < kPerThread; ++k) {
         const int slot = k * kThreads + lane;
+        wave_bits[k] = 0u;
         if (slot < count) {
-          s_a[slot] = next_a[k];
-          s_b[slot] = next_b[k];
-          s_c[slot] = next_c[k];
+          const StagedSplat staged = stage_record(next_a[k], next_b[k], next_c[k]);
+          s_geometry[slot] = staged.geometry;
+          s_shading[slot] = staged.shading;
+          s_colour[slot] = staged.colour;
+          float half_x, half_y;
+          if (splat_extent(next_a[k], next_b[k], staged.shading.z, half_x, half_y)) {
+            const float mx = next_a[k].x, my = next_a[k].y;
+            const float gap_x = fabsf(mx - fminf(fmaxf(mx, rect_x0), rect_x1));
+            if (gap_x <= half_x) {
+#pragma unroll
+              for (int w = 0; w < kWaves; ++w) {
+                const float y0 = rect_y_base + static_cast<float>(w * kRowsPerWave);
+                const float y1 = y0 + static_cast<float>(kRowsPerWave - 1);
+                const float gap_y = fabsf(my - fminf(fmaxf(my, y0), y1));
+                if (gap_y <= half_y) wave_bits[k] |= 1u << w;
+              }
+            }
+          } else {
+            wave_bits[k] = (1u << kWaves) - 1u;  // no certain box: every wave evaluates it
+          }
+        }
+      }
+      // Ordered compaction of each wave's sub-list.  Slots are ordered by
+      // (slice k, source wave, lane), which is batch-slot order = depth order.
+      const uint64_t below = wave_lane == 0 ? 0ull : (~0ull >> (kWave - wave_lane));
+      uint32_t prefix[kPerThread][kWaves];
+#pragma unroll
+      for (int k = 0; k < kPerThread; ++k) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+          const uint64_t vote = __ballot((wave_bits[k] >> w) & 1u);
+          prefix[k][w] = static_cast<uint32_t>(__popcll(vote & below));
+          if (wave_lane == 0) s_totals[k][wave][w] = static_cast<uint32_t>(__popcll(vote));
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kPerThread; ++k) {
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+          if ((wave_bits[k] >> w) & 1u) {
+            uint32_t position = prefix[k][w];
+            for (int kk = 0; kk < k; ++kk)
+              for (int q = 0; q < kWaves; ++q) position += s_totals[kk][q][w];
+            for (int q = 0; q < wave; ++q) position += s_totals[k][q][w];
+            s_list[w][position] = static_cast<uint16_t>(k * kThreads + lane);
+          }
         }
       }
       __syncthreads();
@@ -141,32 +246,37 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(Composi
                                                    next_c);
       }
       if (!done) {
-        for (int item = 0; item < count; ++item) {
+        uint32_t list_count = 0;
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k)
+#pragma unroll
+          for (int q = 0; q < kWaves; ++q) list_count += s_totals[k][q][wave];
+        for (uint32_t i = 0; i < list_count; ++i) {
+          const int item = s_list[wave][i];
           if (kCount) ++evaluations;
-          const f32x4 a = s_a[item];
-          const float delta_x = px - a.x;
-          const float delta_y = py - a.y;
-          const f32x4 b = s_b[item];
-          const float power =
-              -0.5f * (a.z * delta_x * delta_x + 2.0f * a.w * delta_x * delta_y + b.x * delta_y * delta_y);
-          if (power > 0.0f || power < -20.0f) continue;
-          const float splat_alpha = fminf(VGR_MAX_ALPHA, b.y * vgr_expf(power));
+          const f32x4 g = s_geometry[item];
+          const float delta_x = px - g.x;
+          const float delta_y = py - g.y;
+          const f32x4 sh = s_shading[item];
+          const float power = g.z * delta_x * delta_x + g.w * delta_x * delta_y + sh.x * delta_y * delta_y;
+          if (power > 0.0f || power < sh.z) continue;
+          const float splat_alpha = fminf(VGR_MAX_ALPHA, sh.y * vgr_expf(power));
           if (splat_alpha < VGR_ALPHA_THRESHOLD) continue;
           const float next_transmittance = transmittance * (1.0f - splat_alpha);
           if (next_transmittance <= VGR_TRANSMITTANCE_THRESHOLD) {
             done = true;
             break;
           }
-          const f32x4 c = s_c[item];
+          const f32x4 col = s_colour[item];
           const float weight = transmittance * splat_alpha;
-          red = fmaf(weight, b.w, red);
-          green = fmaf(weight, c.x, green);
-          blue = fmaf(weight, c.y, blue);
+          red = fmaf(weight, col.x, red);
+          green = fmaf(weight, col.y, green);
+          blue = fmaf(weight, col.z, blue);
           if (kFull) {
-            depth_sum = fmaf(weight, b.z, depth_sum);
+            depth_sum = fmaf(weight, sh.w, depth_sum);
             if (weight > best_weight) {
               best_weight = weight;
-              best_gaussian = __float_as_int(c.z);
+              best_gaussian = __float_as_int(col.w);
             }
           }
           transmittance = next_transmittance;

@@ -55,6 +55,7 @@ struct ScratchLayout {
   int32_t projection_groups;
   int32_t cameras_per_block;
   int64_t sort_tiles;
+  int32_t sort_tile;        // keys per sort tile
   int32_t sort_passes;
   int32_t sort_digit_bits;
   int32_t sort_key_bits;
@@ -63,8 +64,7 @@ struct ScratchLayout {
   uint32_t depth_base;     // float bits of the near plane
 };
 
-constexpr int kSortItems = 16;                           // keys per thread per sort tile
-constexpr int kSortTile = kBlockThreads * kSortItems;    // 4096 keys per sort tile
+constexpr int kDefaultSortItems = 16;  // keys per thread per sort tile (tile = 256 * items keys)
 
 // Host helpers implemented in vgr_capi.hip.
 void set_error(const char* fmt, ...);

@@ -161,12 +161,15 @@ __device__ __forceinline__ int rect_tiles(const TileRect& rect) {
 
 // View-dependent colour: base RGB plus real-SH bands 1..degree evaluated at the
 // unit direction from the camera centre to the Gaussian (3DGS basis/constants),
-// clamped at zero.  features[g, 3k + c] holds band k, channel c.
-__device__ __forceinline__ void sh_colour(const float* f, int degree, float dx, float dy, float dz, float* rgb) {
+// clamped at zero.  f[3k + c] holds band k, channel c (held in registers).
+template <int kDegree>
+__device__ __forceinline__ void sh_colour(const float (&f)[3 * (kDegree + 1) * (kDegree + 1)], float dx, float dy,
+                                          float dz, float* rgb) {
+  constexpr int kBands = (kDegree + 1) * (kDegree + 1);
   rgb[0] = f[0];
   rgb[1] = f[1];
   rgb[2] = f[2];
-  if (degree <= 0) return;
+  if (kDegree <= 0) return;
   const float norm2 = dx * dx + dy * dy + dz * dz;
   float x = 0.0f, y = 0.0f, z = 0.0f;
   if (norm2 > 0.0f) {
@@ -179,8 +182,7 @@ __device__ __forceinline__ void sh_colour(const float* f, int degree, float dx, 
   basis[1] = -kShC1 * y;
   basis[2] = kShC1 * z;
   basis[3] = -kShC1 * x;
-  int count = 4;
-  if (degree >= 2) {
+  if (kDegree >= 2) {
     const float xx = x * x, yy = y * y, zz = z * z;
     const float xy = x * y, yz = y * z, xz = x * z;
     basis[4] = kShC2_0 * xy;
@@ -188,8 +190,7 @@ __device__ __forceinline__ void sh_colour(const float* f, int degree, float dx, 
     basis[6] = kShC2_2 * (2.0f * zz - xx - yy);
     basis[7] = kShC2_3 * xz;
     basis[8] = kShC2_4 * (xx - yy);
-    count = 9;
-    if (degree >= 3) {
+    if (kDegree >= 3) {
       basis[9] = kShC3_0 * y * (3.0f * xx - yy);
       basis[10] = kShC3_1 * xy * z;
       basis[11] = kShC3_2 * y * (4.0f * zz - xx - yy);
@@ -197,14 +198,37 @@ __device__ __forceinline__ void sh_colour(const float* f, int degree, float dx, 
       basis[13] = kShC3_4 * x * (4.0f * zz - xx - yy);
       basis[14] = kShC3_5 * z * (xx - yy);
       basis[15] = kShC3_6 * x * (xx - 3.0f * yy);
-      count = 16;
     }
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float acc = 0.0f;
-    for (int k = 1; k < count; ++k) acc = fmaf(basis[k], f[3 * k + c], acc);
+#pragma unroll
+    for (int k = 1; k < kBands; ++k) acc = fmaf(basis[k], f[3 * k + c], acc);
     rgb[c] = fmaxf(f[c] + acc, 0.0f);
+  }
+}
+
+// Loads the 3*(degree+1)^2 colour channels of Gaussian g into registers, with
+// 16-byte loads when the row stride allows it.
+template <int kDegree>
+__device__ __forceinline__ void load_features(const float* __restrict__ features, int64_t g, int width,
+                                              float (&f)[3 * (kDegree + 1) * (kDegree + 1)]) {
+  constexpr int kChannels = 3 * (kDegree + 1) * (kDegree + 1);
+  const float* row = features + g * width;
+  if (kChannels % 4 == 0 && (width & 3) == 0 && (reinterpret_cast<uintptr_t>(features) & 15u) == 0) {
+    const float4* row4 = reinterpret_cast<const float4*>(row);
+#pragma unroll
+    for (int i = 0; i < kChannels / 4; ++i) {
+      const float4 v = row4[i];
+      f[4 * i + 0] = v.x;
+      f[4 * i + 1] = v.y;
+      f[4 * i + 2] = v.z;
+      f[4 * i + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kChannels; ++i) f[i] = row[i];
   }
 }
 
@@ -438,6 +462,7 @@ __device__ __forceinline__ void wave_inclusive_scan2(int& a, int& b) {
   }
 }
 
+template <int kDegree>
 __global__ __launch_bounds__(kBlockThreads) void projection_write_kernel(
     ProjectionArgs args, const int64_t* __restrict__ block_bases, vgr_record* __restrict__ records,
     int32_t* __restrict__ radii, uint64_t* __restrict__ keys, int32_t* __restrict__ values,
@@ -449,6 +474,11 @@ __global__ __launch_bounds__(kBlockThreads) void projection_write_kernel(
   bool loaded;
   float m0, m1, m2, opacity, cov[6];
   load_gaussian(args, g, loaded, m0, m1, m2, opacity, cov);
+  constexpr int kChannels = 3 * (kDegree + 1) * (kDegree + 1);
+  float f[kChannels];
+#pragma unroll
+  for (int i = 0; i < kChannels; ++i) f[i] = 0.0f;
+  if (loaded && opacity >= VGR_ALPHA_THRESHOLD) load_features<kDegree>(args.features, g, args.feature_width, f);
   int64_t visible_base = block_bases[block * 2 + 0];
   int64_t intersection_base = block_bases[block * 2 + 1];
   const int first = group * args.cameras_per_block;
@@ -486,8 +516,7 @@ __global__ __launch_bounds__(kBlockThreads) void projection_write_kernel(
       const bool stored = record < visible_capacity;
       if (stored) {
         float rgb[3];
-        const float* f = args.features + g * args.feature_width;
-        sh_colour(f, args.sh_degree, m0 - cam.origin_x, m1 - cam.origin_y, m2 - cam.origin_z, rgb);
+        sh_colour<kDegree>(f, m0 - cam.origin_x, m1 - cam.origin_y, m2 - cam.origin_z, rgb);
         float4* out = reinterpret_cast<float4*>(records + record);
         out[0] = make_float4(p.mean_x, p.mean_y, p.conic_0, p.conic_1);
         out[1] = make_float4(p.conic_2, p.opacity, p.depth, rgb[0]);
@@ -742,9 +771,25 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
   if (blocks > 0) {
     hipLaunchKernelGGL(scan_downsweep_kernel, dim3(static_cast<unsigned>(scan_blocks)), dim3(kBlockThreads), 0,
                        stream, block_counts, blocks, partials, block_bases);
-    hipLaunchKernelGGL(projection_write_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0,
-                       stream, args, block_bases, ws.records, ws.radii, ws.keys_in, ws.values_in,
-                       ws.visible_capacity, ws.intersection_capacity);
+    const dim3 grid(static_cast<unsigned>(blocks)), threads(kBlockThreads);
+    switch (args.sh_degree) {
+      case 1:
+        hipLaunchKernelGGL(projection_write_kernel<1>, grid, threads, 0, stream, args, block_bases, ws.records,
+                           ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
+        break;
+      case 2:
+        hipLaunchKernelGGL(projection_write_kernel<2>, grid, threads, 0, stream, args, block_bases, ws.records,
+                           ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
+        break;
+      case 3:
+        hipLaunchKernelGGL(projection_write_kernel<3>, grid, threads, 0, stream, args, block_bases, ws.records,
+                           ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
+        break;
+      default:
+        hipLaunchKernelGGL(projection_write_kernel<0>, grid, threads, 0, stream, args, block_bases, ws.records,
+                           ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
+        break;
+    }
   }
   const hipError_t status = hipGetLastError();
   if (status != hipSuccess) {

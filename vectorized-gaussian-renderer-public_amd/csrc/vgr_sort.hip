@@ -109,13 +109,14 @@ __global__ __launch_bounds__(kBlockThreads) void sort_digit_scan_kernel(const ui
 // device count needs (graph-safe replacement for hipMemsetAsync).
 __global__ __launch_bounds__(kBlockThreads) void sort_clear_kernel(const int64_t* __restrict__ counters,
                                                                    int64_t capacity, int digit_bits, int passes,
-                                                                   int64_t max_tiles, uint32_t* __restrict__ histogram,
+                                                                   int64_t max_tiles, int tile_keys,
+                                                                   uint32_t* __restrict__ histogram,
                                                                    uint32_t* __restrict__ tickets,
                                                                    uint32_t* __restrict__ status,
                                                                    uint32_t* __restrict__ flags) {
   const int radix = 1 << digit_bits;
   const int64_t n = element_count(counters, capacity);
-  int64_t tiles = (n + kSortTile - 1) / kSortTile;
+  int64_t tiles = (n + tile_keys - 1) / tile_keys;
   tiles = tiles < max_tiles ? tiles : max_tiles;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlockThreads;
   const int64_t first = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x;
@@ -136,13 +137,14 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
 }
 
-template <int kDigitBits>
+template <int kDigitBits, int kSortItems>
 __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
     const uint64_t* __restrict__ keys_in, const int32_t* __restrict__ values_in, uint64_t* __restrict__ keys_out,
     int32_t* __restrict__ values_out, const int64_t* __restrict__ counters, int64_t capacity, SortKeyParams kp,
     int shift, const uint32_t* __restrict__ digit_base, uint32_t* __restrict__ flags, uint32_t* __restrict__ ticket,
     uint32_t* __restrict__ status) {
   constexpr int kRadix = 1 << kDigitBits;
+  constexpr int kSortTile = kBlockThreads * kSortItems;
   constexpr int kWaves = kBlockThreads / kWave;
   constexpr int kWaveRun = kSortTile / kWaves;  // contiguous keys ranked by one wave
   constexpr uint32_t kMask = kRadix - 1;
@@ -348,12 +350,19 @@ unsigned fill_grid(int64_t n) {
 }
 
 template <int kDigitBits>
-void launch_pass(const uint64_t* keys_in, const int32_t* values_in, uint64_t* keys_out, int32_t* values_out,
-                 const int64_t* counters, int64_t capacity, SortKeyParams kp, int shift, const uint32_t* digit_base,
-                 uint32_t* flags, uint32_t* ticket, uint32_t* status, int64_t tiles, hipStream_t stream) {
-  hipLaunchKernelGGL(sort_onesweep_pass_kernel<kDigitBits>, dim3(static_cast<unsigned>(tiles)), dim3(kBlockThreads),
-                     0, stream, keys_in, values_in, keys_out, values_out, counters, capacity, kp, shift, digit_base,
-                     flags, ticket, status);
+void launch_pass(int items, const uint64_t* keys_in, const int32_t* values_in, uint64_t* keys_out,
+                 int32_t* values_out, const int64_t* counters, int64_t capacity, SortKeyParams kp, int shift,
+                 const uint32_t* digit_base, uint32_t* flags, uint32_t* ticket, uint32_t* status, int64_t tiles,
+                 hipStream_t stream) {
+  if (items == 8) {
+    hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 8>), dim3(static_cast<unsigned>(tiles)),
+                       dim3(kBlockThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
+                       kp, shift, digit_base, flags, ticket, status);
+  } else {
+    hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 16>), dim3(static_cast<unsigned>(tiles)),
+                       dim3(kBlockThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
+                       kp, shift, digit_base, flags, ticket, status);
+  }
 }
 
 }  // namespace
@@ -399,7 +408,8 @@ int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scra
   const int64_t tiles = layout.sort_tiles;
 
   hipLaunchKernelGGL(sort_clear_kernel, dim3(1024), dim3(kBlockThreads), 0, stream, ws.counters, capacity,
-                     digit_bits, passes, tiles, histogram, tickets, status, flags);
+                     digit_bits, passes, tiles, layout.sort_tile, histogram, tickets, status, flags);
+  const int items = layout.sort_tile / kBlockThreads;
   int64_t histogram_blocks = (capacity + kBlockThreads * 16 - 1) / (kBlockThreads * 16);
   histogram_blocks = histogram_blocks > 2048 ? 2048 : (histogram_blocks < 1 ? 1 : histogram_blocks);
   hipLaunchKernelGGL(sort_histogram_kernel, dim3(static_cast<unsigned>(histogram_blocks)), dim3(kBlockThreads), 0,
@@ -416,23 +426,23 @@ int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scra
     const uint32_t* pass_base = digit_base + p * radix;
     switch (digit_bits) {
       case 4:
-        launch_pass<4>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+        launch_pass<4>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
                        pass_base, pass_flags, tickets + p, status, tiles, stream);
         break;
       case 5:
-        launch_pass<5>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+        launch_pass<5>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
                        pass_base, pass_flags, tickets + p, status, tiles, stream);
         break;
       case 6:
-        launch_pass<6>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+        launch_pass<6>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
                        pass_base, pass_flags, tickets + p, status, tiles, stream);
         break;
       case 7:
-        launch_pass<7>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+        launch_pass<7>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
                        pass_base, pass_flags, tickets + p, status, tiles, stream);
         break;
       default:
-        launch_pass<8>(source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+        launch_pass<8>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
                        pass_base, pass_flags, tickets + p, status, tiles, stream);
         break;
     }

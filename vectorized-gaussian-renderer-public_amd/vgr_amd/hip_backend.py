@@ -108,6 +108,7 @@ class CustomHipBackend:
         sort_digit_bits: int = 0,
         raster_batch: int = 0,
         cameras_per_block: int = 0,
+        sort_tile_items: int = 0,
         native_module: Any | None = None,
         native_loader: Callable[[], Any] = load_native_module,
         allow_cpu_for_tests: bool = False,
@@ -172,6 +173,8 @@ class CustomHipBackend:
             raise ValueError("raster_batch must be 0 (default), 64, 128, 256 or 512.")
         if not 0 <= cameras_per_block <= 64:
             raise ValueError("cameras_per_block must be 0 (default) or in [1, 64].")
+        if sort_tile_items not in (0, 8, 16):
+            raise ValueError("sort_tile_items must be 0 (default), 8 or 16.")
 
         self.max_visible_records = max_visible_records
         self.max_intersections = max_intersections
@@ -203,6 +206,7 @@ class CustomHipBackend:
         self.sort_digit_bits = sort_digit_bits
         self.raster_batch = raster_batch
         self.cameras_per_block = cameras_per_block
+        self.sort_tile_items = sort_tile_items
         self._native = native_module
         self._native_loader = native_loader
         # Optional instrumentation forwarded to the native render: five recorded
@@ -478,7 +482,8 @@ class CustomHipBackend:
             self.depth_bucket_count, self.depth_bucket_group_size, False, False, reuse_projection,
             self.output_srgb, self.deterministic, full_sensor_output, self.fixed_capacity_sort,
             sh_degree=self._resolved_sh_degree, sort_digit_bits=self.sort_digit_bits,
-            raster_batch=self.raster_batch, cameras_per_block=self.cameras_per_block, **instrumentation))
+            raster_batch=self.raster_batch, cameras_per_block=self.cameras_per_block,
+            sort_tile_items=self.sort_tile_items, **instrumentation))
         if selector not in (0, 1):
             raise RuntimeError(f"Native renderer returned an invalid sorted-buffer selector: {selector}.")
         if selector == 0:
@@ -644,7 +649,7 @@ class CustomHipBackend:
         return int(query(max_views=views, height=height, width=width, gaussians=gaussians,
                          intersection_capacity=capacity, tile_size=self.tile_size, near_plane=self.near_plane,
                          far_plane=self.far_plane, sort_digit_bits=self.sort_digit_bits,
-                         cameras_per_block=self.cameras_per_block))
+                         cameras_per_block=self.cameras_per_block, sort_tile_items=self.sort_tile_items))
 
     def _workspace_plan(self, *, visible_capacity: int, intersection_capacity: int) -> dict[str, Any]:
         if self._workspace_config is None:

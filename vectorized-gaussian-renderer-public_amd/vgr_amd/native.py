@@ -127,7 +127,7 @@ class _Options(ctypes.Structure):
         ("reuse_projection", ctypes.c_int32),
         ("stage_events", ctypes.POINTER(ctypes.c_void_p)),
         ("stage_event_count", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("sort_tile_items", ctypes.c_int32),
         ("statistics", ctypes.c_void_p),
     ]
 
@@ -248,7 +248,8 @@ class NativeRenderer:
 
     def scratch_bytes(self, *, max_views: int, height: int, width: int, gaussians: int, intersection_capacity: int,
                       tile_size: int, near_plane: float, far_plane: float, sort_digit_bits: int = 0,
-                      cameras_per_block: int = 0, resolutions: Sequence[tuple[int, int]] | None = None) -> int:
+                      cameras_per_block: int = 0, sort_tile_items: int = 0,
+                      resolutions: Sequence[tuple[int, int]] | None = None) -> int:
         """Scratch bytes one render of up to ``max_views`` cameras needs."""
         cameras = _Cameras()
         cameras.batch = max_views
@@ -260,7 +261,8 @@ class NativeRenderer:
                                            for w, h in resolutions)
         options = self._options(height=height, width=width, near_plane=near_plane, far_plane=far_plane,
                                 gaussian_support_sigma=3.0, covariance_epsilon=0.0, tile_size=tile_size,
-                                sort_digit_bits=sort_digit_bits, cameras_per_block=cameras_per_block)
+                                sort_digit_bits=sort_digit_bits, cameras_per_block=cameras_per_block,
+                                sort_tile_items=sort_tile_items)
         sizes = _Sizes()
         self._check(self.lib.vgr_workspace_sizes_query(ctypes.byref(cameras), ctypes.byref(options), int(gaussians),
                                                         int(intersection_capacity), ctypes.byref(sizes)),
@@ -322,6 +324,7 @@ class NativeRenderer:
         sort_digit_bits: int = 0,
         raster_batch: int = 0,
         cameras_per_block: int = 0,
+        sort_tile_items: int = 0,
         resolutions: tuple[torch.Tensor, torch.Tensor, torch.Tensor, int] | None = None,
         stage_events: Sequence[torch.cuda.Event] | None = None,
         statistics: torch.Tensor | None = None,
@@ -404,7 +407,8 @@ class NativeRenderer:
             tile_size=int(tile_size), output_srgb=int(bool(output_srgb)), deterministic=int(bool(deterministic)),
             full_sensor_output=int(bool(full_sensor_output)), fixed_capacity_sort=int(bool(fixed_capacity_sort)),
             sort_digit_bits=int(sort_digit_bits), raster_batch=int(raster_batch),
-            cameras_per_block=int(cameras_per_block), reuse_projection=int(bool(reuse_projection)))
+            cameras_per_block=int(cameras_per_block), reuse_projection=int(bool(reuse_projection)),
+            sort_tile_items=int(sort_tile_items))
         event_array = None
         if stage_events is not None:
             if len(stage_events) != 5:
