@@ -63,6 +63,14 @@ typedef struct vgr_scene {
   int32_t scene_count;              /* S */
   int32_t sh_degree;                /* 0..3; bands beyond F/3 are absent */
   int32_t reserved;
+  /* Optional spatial ordering.  When the packed arrays above are stored in a
+   * permuted (e.g. Morton) order, gaussian_ids[p] is the registration-order id
+   * of packed entry p; records, semantic lookups and tie order use those ids,
+   * and semantic_ids stays in registration order.  NULL = identity. */
+  const int32_t* gaussian_ids;      /* [G] or NULL */
+  /* Optional per-chunk bounds from vgr_chunk_bounds (256 Gaussians per chunk):
+   * enables frustum culling of (camera group x chunk) work items.  NULL = off. */
+  const float* chunk_bounds;        /* [ceil(G/256), 8] */
 } vgr_scene;
 
 /* Camera batch (reference camera tensor list, renderer_cuda.cu:58-64). */
@@ -151,7 +159,12 @@ typedef struct vgr_options {
   int32_t stage_event_count;        /* 0 or 5 */
   int32_t sort_tile_items;          /* keys per thread per sort tile: 0 = default (16); 8 or 16 */
   int64_t* statistics;              /* int64[4], zeroed then filled: [0] compositor (pixel, Gaussian)
-                                       evaluations; [1] sort look-back spin overflows */
+                                       evaluations; [1] sort look-back spin overflows; [2] projection
+                                       work items kept by chunk culling */
+  int32_t persistent_composite;     /* 0: one workgroup per (camera, tile); N > 0: N workgroups pull
+                                       (camera, tile) items from a device queue */
+  int32_t chunk_culling;            /* 1: cull (camera group x chunk) items against camera frusta using
+                                       scene.chunk_bounds (scratch sized for it) */
 } vgr_options;
 
 typedef struct vgr_workspace_sizes {
@@ -181,6 +194,13 @@ int32_t vgr_workspace_sizes_query(const vgr_cameras* cameras,
 /* Bytes of sort scratch for n_items keys over n_segments tiles (the caller may
  * size workspace.scratch with vgr_workspace_sizes_query instead). */
 int64_t vgr_sort_temp_bytes(int64_t n_items, int64_t n_segments);
+
+/* Bounds of each chunk of 256 consecutive packed Gaussians: min xyz, max xyz,
+ * an upper bound of the largest standard deviation (sqrt of the covariance
+ * trace) and 0.  No reference counterpart (the reference projects every
+ * camera x Gaussian pair, renderer_cuda.cu:581-591); used for frustum culling. */
+int32_t vgr_chunk_bounds(const float* means, const float* covariances, int64_t count, float* bounds,
+                         void* stream);
 
 /* covariances[g] = R(q_g) diag(s_g^2) R(q_g)^T, upper triangle (renderer_cuda.cu:492-540). */
 int32_t vgr_precompute_covariances(const float* scales, const float* rotations,

@@ -83,6 +83,23 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   layout.sort_flags = offset;
   offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(layout.sort_passes) *
                                  static_cast<size_t>(layout.sort_tiles) * static_cast<size_t>(radix));
+  layout.work_counter = offset;
+  offset = align_up(offset + sizeof(uint32_t) * 4);
+  layout.culling = options.chunk_culling != 0;
+  const size_t blocks = static_cast<size_t>(layout.projection_blocks);
+  const size_t cull_blocks = layout.culling ? blocks : 0;
+  layout.cull_counts = offset;
+  offset = align_up(offset + sizeof(int64_t) * 2 * (cull_blocks + 1));
+  layout.cull_bases = offset;
+  offset = align_up(offset + sizeof(int64_t) * 2 * (cull_blocks + (layout.culling ? scan_blocks : 0) + 1));
+  layout.cull_masks = offset;
+  offset = align_up(offset + sizeof(uint64_t) * (cull_blocks + 1));
+  layout.item_blocks = offset;
+  offset = align_up(offset + sizeof(int32_t) * (cull_blocks + 1));
+  layout.item_masks = offset;
+  offset = align_up(offset + sizeof(uint64_t) * (cull_blocks + 1));
+  layout.item_totals = offset;
+  offset = align_up(offset + sizeof(int64_t) * 2);
   layout.total = offset;
   return layout;
 }
@@ -117,6 +134,10 @@ int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
   const int batch = options.raster_batch == 0 ? 256 : options.raster_batch;
   if (batch != 64 && batch != 128 && batch != 256 && batch != 512) {
     set_error("raster_batch must be 64, 128, 256 or 512; got %d", options.raster_batch);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (options.persistent_composite < 0 || options.persistent_composite > (1 << 20)) {
+    set_error("persistent_composite must be in [0, 2^20]; got %d", options.persistent_composite);
     return VGR_E_INVALID_ARGUMENT;
   }
   if (options.sort_tile_items != 0 && options.sort_tile_items != 8 && options.sort_tile_items != 16) {
@@ -304,7 +325,8 @@ extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras
   if (!opt.reuse_projection) {
     if ((status = launch_projection(sc, cams, resolved, ws, layout, scratch, stream)) != 0) return status;
     if ((status = mark(1)) != 0) return status;
-    if ((status = launch_sort(ws, layout, scratch, stream, &sorted_in_out)) != 0) return status;
+    if ((status = launch_sort(ws, layout, scratch, stream, &sorted_in_out, sc.gaussian_ids != nullptr)) != 0)
+      return status;
     if ((status = mark(2)) != 0) return status;
     if ((status = launch_ranges(ws, tiles, sorted_in_out != 0, stream)) != 0) return status;
     if ((status = mark(3)) != 0) return status;

@@ -35,6 +35,8 @@ struct CompositeArgs {
   int64_t* statistics;   // optional: [0] += evaluated (pixel, Gaussian) pairs
   int64_t visible_capacity;
   int64_t gaussian_count;
+  uint32_t* work_counter;  // persistent queue counter (zeroed before the launch)
+  int64_t total_items;     // (camera, tile) items in the launch
   int active_count;
   int uniform_tiles;     // tiles per camera when every camera has the same resolution, else 0
   int output_srgb;
@@ -127,8 +129,9 @@ __device__ __forceinline__ bool splat_extent(const f32x4& a, const f32x4& b, flo
   return true;
 }
 
+// Renders one (active camera, tile) work item with the whole workgroup.
 template <int kTileSize, int kBatch, bool kFull, bool kCount>
-__global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(CompositeArgs args) {
+__device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_t block) {
   constexpr int kThreads = kTileSize * kTileSize;
   constexpr int kWaves = kThreads / kWave;
   constexpr int kRowsPerWave = kWave / kTileSize;
@@ -140,7 +143,6 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(Composi
   __shared__ uint16_t s_list[kWaves][kBatch];  // per-wave ordered sub-lists of batch slots
   __shared__ uint32_t s_totals[kPerThread][kWaves][kWaves];  // [slice][source wave][list]
 
-  const int64_t block = blockIdx.x;
   int active;
   int64_t image_tile;
   if (args.uniform_tiles > 0) {
@@ -313,37 +315,70 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(Composi
   }
 }
 
-template <int kTileSize, int kBatch>
-void launch_full_or_rgb(bool full, const CompositeArgs& args, int64_t blocks, hipStream_t stream) {
-  const dim3 grid(static_cast<unsigned>(blocks));
+template <int kTileSize, int kBatch, bool kFull, bool kCount>
+__global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(CompositeArgs args) {
+  composite_tile<kTileSize, kBatch, kFull, kCount>(args, blockIdx.x);
+}
+
+// Persistent form: a fixed grid whose workgroups pull (camera, tile) items from
+// a device counter until the queue is empty; every workgroup exits once the
+// counter passes the item count.
+template <int kTileSize, int kBatch, bool kFull, bool kCount>
+__global__ __launch_bounds__(kTileSize* kTileSize) void composite_persistent_kernel(CompositeArgs args) {
+  __shared__ int64_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = static_cast<int64_t>(atomicAdd(args.work_counter, 1u));
+    __syncthreads();
+    const int64_t item = s_item;
+    __syncthreads();
+    if (item >= args.total_items) break;
+    composite_tile<kTileSize, kBatch, kFull, kCount>(args, item);
+  }
+}
+
+template <int kTileSize, int kBatch, bool kFull, bool kCount>
+void launch_variant(const CompositeArgs& args, int64_t blocks, int persistent, hipStream_t stream) {
   const dim3 block(kTileSize * kTileSize);
+  if (persistent > 0) {
+    const int64_t grid = blocks < persistent ? blocks : persistent;
+    hipLaunchKernelGGL((composite_persistent_kernel<kTileSize, kBatch, kFull, kCount>),
+                       dim3(static_cast<unsigned>(grid)), block, 0, stream, args);
+  } else {
+    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, kFull, kCount>), dim3(static_cast<unsigned>(blocks)),
+                       block, 0, stream, args);
+  }
+}
+
+template <int kTileSize, int kBatch>
+void launch_full_or_rgb(bool full, const CompositeArgs& args, int64_t blocks, int persistent, hipStream_t stream) {
   if (args.statistics != nullptr) {
     if (full) {
-      hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, true, true>), grid, block, 0, stream, args);
+      launch_variant<kTileSize, kBatch, true, true>(args, blocks, persistent, stream);
     } else {
-      hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, false, true>), grid, block, 0, stream, args);
+      launch_variant<kTileSize, kBatch, false, true>(args, blocks, persistent, stream);
     }
   } else if (full) {
-    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, true, false>), grid, block, 0, stream, args);
+    launch_variant<kTileSize, kBatch, true, false>(args, blocks, persistent, stream);
   } else {
-    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, false, false>), grid, block, 0, stream, args);
+    launch_variant<kTileSize, kBatch, false, false>(args, blocks, persistent, stream);
   }
 }
 
 template <int kTileSize>
-void launch_batch(int batch, bool full, const CompositeArgs& args, int64_t blocks, hipStream_t stream) {
+void launch_batch(int batch, bool full, const CompositeArgs& args, int64_t blocks, int persistent,
+                  hipStream_t stream) {
   switch (batch) {
     case 64:
-      launch_full_or_rgb<kTileSize, 64>(full, args, blocks, stream);
+      launch_full_or_rgb<kTileSize, 64>(full, args, blocks, persistent, stream);
       break;
     case 128:
-      launch_full_or_rgb<kTileSize, 128>(full, args, blocks, stream);
+      launch_full_or_rgb<kTileSize, 128>(full, args, blocks, persistent, stream);
       break;
     case 512:
-      launch_full_or_rgb<kTileSize, 512>(full, args, blocks, stream);
+      launch_full_or_rgb<kTileSize, 512>(full, args, blocks, persistent, stream);
       break;
     default:
-      launch_full_or_rgb<kTileSize, 256>(full, args, blocks, stream);
+      launch_full_or_rgb<kTileSize, 256>(full, args, blocks, persistent, stream);
       break;
   }
 }
@@ -387,10 +422,14 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const v
     return VGR_E_INVALID_ARGUMENT;
   }
   const bool full = options.full_sensor_output != 0;
+  args.total_items = blocks;
+  args.work_counter = reinterpret_cast<uint32_t*>(const_cast<char*>(scratch) + layout.work_counter);
+  const int persistent = options.persistent_composite;
+  if (persistent > 0) launch_fill_u32(args.work_counter, 1, 0u, stream);
   if (options.tile_size == 8) {
-    launch_batch<8>(options.raster_batch, full, args, blocks, stream);
+    launch_batch<8>(options.raster_batch, full, args, blocks, persistent, stream);
   } else {
-    launch_batch<16>(options.raster_batch, full, args, blocks, stream);
+    launch_batch<16>(options.raster_batch, full, args, blocks, persistent, stream);
   }
   const hipError_t error = hipGetLastError();
   if (error != hipSuccess) {

@@ -371,7 +371,23 @@ struct ProjectionArgs {
   int tile_size;
   int antialiased;
   float near_plane, far_plane, support_sigma, covariance_epsilon;
+  const int32_t* gaussian_ids;   // registration-order id per packed entry, or null (identity)
+  const int32_t* item_blocks;    // kept (group, chunk) blocks in order, or null (every block)
+  const uint64_t* item_masks;    // cameras of the group that may see the chunk
+  const int64_t* item_count;     // device count of kept blocks
 };
+
+// Resolves the (group, chunk) block and camera mask processed by this workgroup.
+// Returns false for workgroups beyond the kept item count.
+__device__ __forceinline__ bool resolve_block(const ProjectionArgs& args, int64_t& block, uint64_t& mask) {
+  block = blockIdx.x;
+  mask = ~0ull;
+  if (args.item_blocks == nullptr) return true;
+  if (block >= *args.item_count) return false;
+  mask = args.item_masks[block];
+  block = args.item_blocks[block];
+  return true;
+}
 
 // Pair results for one camera of the group; identical in both passes.
 __device__ __forceinline__ bool evaluate_pair(const ProjectionArgs& args, const CameraParams& cam, int64_t g,
@@ -408,7 +424,15 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t value) {
 
 __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(ProjectionArgs args,
                                                                          int64_t* __restrict__ block_counts) {
-  const int64_t block = blockIdx.x;
+  int64_t block;
+  uint64_t mask;
+  if (!resolve_block(args, block, mask)) {
+    if (threadIdx.x == 0) {
+      block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 0] = 0;
+      block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 1] = 0;
+    }
+    return;
+  }
   const int group = static_cast<int>(block / args.chunks);
   const int64_t chunk = block - static_cast<int64_t>(group) * args.chunks;
   const int64_t g = chunk * kBlockThreads + threadIdx.x;
@@ -419,6 +443,7 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
   const int first = group * args.cameras_per_block;
   const int last = min(args.active_count, first + args.cameras_per_block);
   for (int a = first; a < last; ++a) {
+    if (((mask >> (a - first)) & 1ull) == 0) continue;
     const CameraParams& cam = args.table[a];
     Projected p;
     TileRect rect;
@@ -443,8 +468,8 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
       v += partial[0][w];
       t += partial[1][w];
     }
-    block_counts[block * 2 + 0] = v;
-    block_counts[block * 2 + 1] = t;
+    block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 0] = v;
+    block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 1] = t;
   }
 }
 
@@ -467,7 +492,9 @@ __global__ __launch_bounds__(kBlockThreads) void projection_write_kernel(
     ProjectionArgs args, const int64_t* __restrict__ block_bases, vgr_record* __restrict__ records,
     int32_t* __restrict__ radii, uint64_t* __restrict__ keys, int32_t* __restrict__ values,
     int64_t visible_capacity, int64_t intersection_capacity) {
-  const int64_t block = blockIdx.x;
+  int64_t block;
+  uint64_t mask;
+  if (!resolve_block(args, block, mask)) return;
   const int group = static_cast<int>(block / args.chunks);
   const int64_t chunk = block - static_cast<int64_t>(group) * args.chunks;
   const int64_t g = chunk * kBlockThreads + threadIdx.x;
@@ -479,14 +506,16 @@ __global__ __launch_bounds__(kBlockThreads) void projection_write_kernel(
 #pragma unroll
   for (int i = 0; i < kChannels; ++i) f[i] = 0.0f;
   if (loaded && opacity >= VGR_ALPHA_THRESHOLD) load_features<kDegree>(args.features, g, args.feature_width, f);
-  int64_t visible_base = block_bases[block * 2 + 0];
-  int64_t intersection_base = block_bases[block * 2 + 1];
+  int64_t visible_base = block_bases[static_cast<int64_t>(blockIdx.x) * 2 + 0];
+  int64_t intersection_base = block_bases[static_cast<int64_t>(blockIdx.x) * 2 + 1];
+  const int32_t record_id = args.gaussian_ids != nullptr && loaded ? args.gaussian_ids[g] : static_cast<int32_t>(g);
   const int first = group * args.cameras_per_block;
   const int last = min(args.active_count, first + args.cameras_per_block);
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   __shared__ int wave_totals[2][kBlockThreads / kWave];
   for (int a = first; a < last; ++a) {
+    if (((mask >> (a - first)) & 1ull) == 0) continue;  // uniform: whole block skips the camera
     const CameraParams& cam = args.table[a];
     Projected p;
     TileRect rect;
@@ -520,8 +549,7 @@ __global__ __launch_bounds__(kBlockThreads) void projection_write_kernel(
         float4* out = reinterpret_cast<float4*>(records + record);
         out[0] = make_float4(p.mean_x, p.mean_y, p.conic_0, p.conic_1);
         out[1] = make_float4(p.conic_2, p.opacity, p.depth, rgb[0]);
-        out[2] = make_float4(rgb[1], rgb[2], __int_as_float(static_cast<int32_t>(g)),
-                             __int_as_float(cam.camera));
+        out[2] = make_float4(rgb[1], rgb[2], __int_as_float(record_id), __int_as_float(cam.camera));
         if (radii != nullptr) {
           radii[record * 2 + 0] = p.radius_x;
           radii[record * 2 + 1] = p.radius_y;
@@ -624,11 +652,13 @@ __device__ __forceinline__ void block_exclusive_scan2_i64(int64_t a, int64_t b, 
   eb = pb + ib - b;
 }
 
-// Single block: exclusive scan of the partials and the counter block.
+// Single block: exclusive scan of the partials.  Writes the totals to
+// `totals` when given, and the counter block when `counters` is given.
 __global__ __launch_bounds__(kBlockThreads) void scan_partials_kernel(int64_t* __restrict__ partials, int64_t n,
                                                                       int64_t visible_capacity,
                                                                       int64_t intersection_capacity,
-                                                                      int64_t* __restrict__ counters) {
+                                                                      int64_t* __restrict__ counters,
+                                                                      int64_t* __restrict__ totals) {
   int64_t carry_a = 0, carry_b = 0;
   for (int64_t start = 0; start < n; start += kBlockThreads) {
     const int64_t index = start + threadIdx.x;
@@ -644,12 +674,18 @@ __global__ __launch_bounds__(kBlockThreads) void scan_partials_kernel(int64_t* _
     carry_b += tb;
   }
   if (threadIdx.x == 0) {
-    counters[VGR_COUNTER_VISIBLE] = carry_a;
-    counters[VGR_COUNTER_INTERSECTIONS] = carry_b;
-    counters[VGR_COUNTER_VISIBLE_OVERFLOW] = carry_a > visible_capacity ? carry_a - visible_capacity : 0;
-    counters[VGR_COUNTER_INTERSECTION_OVERFLOW] =
-        carry_b > intersection_capacity ? carry_b - intersection_capacity : 0;
-    counters[VGR_COUNTER_ACTIVE_TILES] = 0;
+    if (totals != nullptr) {
+      totals[0] = carry_a;
+      totals[1] = carry_b;
+    }
+    if (counters != nullptr) {
+      counters[VGR_COUNTER_VISIBLE] = carry_a;
+      counters[VGR_COUNTER_INTERSECTIONS] = carry_b;
+      counters[VGR_COUNTER_VISIBLE_OVERFLOW] = carry_a > visible_capacity ? carry_a - visible_capacity : 0;
+      counters[VGR_COUNTER_INTERSECTION_OVERFLOW] =
+          carry_b > intersection_capacity ? carry_b - intersection_capacity : 0;
+      counters[VGR_COUNTER_ACTIVE_TILES] = 0;
+    }
   }
 }
 
@@ -672,6 +708,146 @@ __global__ __launch_bounds__(kBlockThreads) void scan_downsweep_kernel(const int
     carry_a += ta;
     carry_b += tb;
   }
+}
+
+// Per-chunk bounds: min/max of the means and the largest sqrt(trace(cov)),
+// which bounds every standard deviation of every Gaussian in the chunk.
+__global__ __launch_bounds__(kBlockThreads) void chunk_bounds_kernel(const float* __restrict__ means,
+                                                                     const float* __restrict__ covariances,
+                                                                     int64_t count, float* __restrict__ bounds) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x;
+  float lo[3], hi[3], extent = 0.0f;
+  if (g < count) {
+    for (int i = 0; i < 3; ++i) lo[i] = hi[i] = means[g * 3 + i];
+    const float* c = covariances + g * 6;
+    extent = sqrtf(fmaxf(c[0] + c[3] + c[5], 0.0f));
+  } else {
+    for (int i = 0; i < 3; ++i) {
+      lo[i] = __int_as_float(0x7f800000);
+      hi[i] = -__int_as_float(0x7f800000);
+    }
+  }
+  for (int offset = 32; offset > 0; offset >>= 1) {
+    for (int i = 0; i < 3; ++i) {
+      lo[i] = fminf(lo[i], __shfl_xor(lo[i], offset, kWave));
+      hi[i] = fmaxf(hi[i], __shfl_xor(hi[i], offset, kWave));
+    }
+    extent = fmaxf(extent, __shfl_xor(extent, offset, kWave));
+  }
+  __shared__ float partial[kBlockThreads / kWave][7];
+  const int wave = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    for (int i = 0; i < 3; ++i) {
+      partial[wave][i] = lo[i];
+      partial[wave][3 + i] = hi[i];
+    }
+    partial[wave][6] = extent;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    float value = 0.0f;
+    if (threadIdx.x < 7) {
+      value = partial[0][threadIdx.x];
+      for (int w = 1; w < kBlockThreads / kWave; ++w) {
+        const float other = partial[w][threadIdx.x];
+        value = threadIdx.x < 3 ? fminf(value, other) : fmaxf(value, other);
+      }
+    }
+    bounds[static_cast<int64_t>(blockIdx.x) * 8 + threadIdx.x] = value;
+  }
+}
+
+// Conservative visibility of a chunk for one camera.  A Gaussian can only be
+// emitted if its centre c (camera space) satisfies z in [near, far] and, for
+// each image side, f*x/z + cx + r_x > 0 etc. with the splat radius bounded by
+// r_x <= sigma*(s*(f/z)*sqrt(1 + lim^2)*|W| + sqrt(eps)) + 1, where s bounds the
+// standard deviation, lim the Jacobian clamp and |W| the norm of the camera
+// rotation.  Multiplying by z/f > 0 turns every side condition into a half
+// space x + a z + b > 0, tested against a ball around the chunk.
+__device__ __forceinline__ bool chunk_may_be_visible(const CameraParams& cam, const float* bounds,
+                                                     float near_plane, float far_plane, float support_sigma,
+                                                     float covariance_epsilon) {
+  if (!(bounds[0] <= bounds[3])) return false;  // empty chunk
+  const double cx_s = 0.5 * (static_cast<double>(bounds[0]) + bounds[3]);
+  const double cy_s = 0.5 * (static_cast<double>(bounds[1]) + bounds[4]);
+  const double cz_s = 0.5 * (static_cast<double>(bounds[2]) + bounds[5]);
+  const double hx = 0.5 * (static_cast<double>(bounds[3]) - bounds[0]);
+  const double hy = 0.5 * (static_cast<double>(bounds[4]) - bounds[1]);
+  const double hz = 0.5 * (static_cast<double>(bounds[5]) - bounds[2]);
+  const double half_diagonal = sqrt(hx * hx + hy * hy + hz * hz);
+  // Scene -> camera: env then view affine; Frobenius norm bounds the scaling.
+  const float* e = cam.env;
+  const float* v = cam.view;
+  const double wx = e[0] * cx_s + e[1] * cy_s + e[2] * cz_s + e[3];
+  const double wy = e[4] * cx_s + e[5] * cy_s + e[6] * cz_s + e[7];
+  const double wz = e[8] * cx_s + e[9] * cy_s + e[10] * cz_s + e[11];
+  const double x = v[0] * wx + v[1] * wy + v[2] * wz + v[3];
+  const double y = v[4] * wx + v[5] * wy + v[6] * wz + v[7];
+  const double z = v[8] * wx + v[9] * wy + v[10] * wz + v[11];
+  double norm2 = 0.0;
+  for (int i = 0; i < 9; ++i) norm2 += static_cast<double>(cam.rotation[i]) * cam.rotation[i];
+  const double scale = sqrt(norm2);
+  const double radius = half_diagonal * scale * 1.001 + 1.0e-5;
+  if (z + radius < near_plane || z - radius > far_plane) return false;
+  const double sigma = support_sigma;
+  const double margin = sigma * sqrt(static_cast<double>(covariance_epsilon)) + 2.0;
+  const double extent = bounds[6];
+  const double lim_x = fmax(cam.limit_x_pos, cam.limit_x_neg);
+  const double lim_y = fmax(cam.limit_y_pos, cam.limit_y_neg);
+  const double bx = sigma * extent * sqrt(1.0 + lim_x * lim_x) * scale * 1.001 + 1.0e-5;
+  const double by = sigma * extent * sqrt(1.0 + lim_y * lim_y) * scale * 1.001 + 1.0e-5;
+  const double fx = cam.focal_x, fy = cam.focal_y;
+  // Half spaces: n.(x,y,z) + b > 0 with n = (+-1, 0, a) or (0, +-1, a).
+  const double a_left = (cam.center_x + margin) / fx;
+  const double a_right = (cam.width - cam.center_x + margin) / fx;
+  const double a_top = (cam.center_y + margin) / fy;
+  const double a_bottom = (cam.height - cam.center_y + margin) / fy;
+  if (x + a_left * z + radius * sqrt(1.0 + a_left * a_left) + bx <= 0.0) return false;
+  if (-x + a_right * z + radius * sqrt(1.0 + a_right * a_right) + bx <= 0.0) return false;
+  if (y + a_top * z + radius * sqrt(1.0 + a_top * a_top) + by <= 0.0) return false;
+  if (-y + a_bottom * z + radius * sqrt(1.0 + a_bottom * a_bottom) + by <= 0.0) return false;
+  return true;
+}
+
+// One thread per (camera group, chunk) candidate: camera mask of the group's
+// cameras that may see the chunk, and a 0/1 keep flag for the ordered compaction.
+__global__ __launch_bounds__(kBlockThreads) void cull_chunks_kernel(ProjectionArgs args,
+                                                                    const float* __restrict__ chunk_bounds,
+                                                                    int64_t blocks, int64_t* __restrict__ flags,
+                                                                    uint64_t* __restrict__ masks) {
+  const int64_t block = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x;
+  if (block >= blocks) return;
+  const int group = static_cast<int>(block / args.chunks);
+  const int64_t chunk = block - static_cast<int64_t>(group) * args.chunks;
+  const float* bounds = chunk_bounds + chunk * 8;
+  const int64_t first_gaussian = chunk * kBlockThreads;
+  const int64_t last_gaussian = first_gaussian + kBlockThreads;
+  const int first = group * args.cameras_per_block;
+  const int last = min(args.active_count, first + args.cameras_per_block);
+  uint64_t mask = 0;
+  for (int a = first; a < last; ++a) {
+    const CameraParams& cam = args.table[a];
+    if (cam.scene_slot < 0 || cam.gaussian_end <= first_gaussian || cam.gaussian_begin >= last_gaussian) continue;
+    if (chunk_may_be_visible(cam, bounds, args.near_plane, args.far_plane, args.support_sigma,
+                             args.covariance_epsilon)) {
+      mask |= 1ull << (a - first);
+    }
+  }
+  masks[block] = mask;
+  flags[block * 2 + 0] = mask != 0 ? 1 : 0;
+  flags[block * 2 + 1] = 0;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void compact_items_kernel(const int64_t* __restrict__ flags,
+                                                                      const int64_t* __restrict__ positions,
+                                                                      const uint64_t* __restrict__ masks,
+                                                                      int64_t blocks, int32_t* __restrict__ items,
+                                                                      uint64_t* __restrict__ item_masks) {
+  const int64_t block = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x;
+  if (block >= blocks || flags[block * 2] == 0) return;
+  const int64_t position = positions[block * 2];
+  items[position] = static_cast<int32_t>(block);
+  item_masks[position] = masks[block];
 }
 
 __global__ void precompute_covariances_kernel(const float* __restrict__ scales, const float* __restrict__ rotations,
@@ -732,6 +908,23 @@ int launch_camera_table(const vgr_scene& scene, const vgr_cameras& cameras, cons
   return 0;
 }
 
+// Device-wide exclusive scan of n int64 pairs (3 launches); partials hold
+// 2 * ceil(n / 2048) entries.  Totals go to `totals` and/or the counter block.
+void scan_pairs(const int64_t* counts, int64_t n, int64_t* bases, int64_t* partials, int64_t visible_capacity,
+                int64_t intersection_capacity, int64_t* counters, int64_t* totals, hipStream_t stream) {
+  const int64_t scan_blocks = (n + kScanTile - 1) / kScanTile;
+  if (n > 0) {
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(scan_blocks)), dim3(kBlockThreads), 0, stream,
+                       counts, n, partials);
+  }
+  hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kBlockThreads), 0, stream, partials, scan_blocks,
+                     visible_capacity, intersection_capacity, counters, totals);
+  if (n > 0) {
+    hipLaunchKernelGGL(scan_downsweep_kernel, dim3(static_cast<unsigned>(scan_blocks)), dim3(kBlockThreads), 0,
+                       stream, counts, n, partials, bases);
+  }
+}
+
 int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_options& options,
                       const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, hipStream_t stream) {
   ProjectionArgs args;
@@ -752,25 +945,43 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
   args.far_plane = options.far_plane;
   args.support_sigma = options.gaussian_support_sigma;
   args.covariance_epsilon = options.covariance_epsilon;
+  args.gaussian_ids = scene.gaussian_ids;
+  args.item_blocks = nullptr;
+  args.item_masks = nullptr;
+  args.item_count = nullptr;
 
   int64_t* block_counts = reinterpret_cast<int64_t*>(scratch + layout.block_counts);
   int64_t* block_bases = reinterpret_cast<int64_t*>(scratch + layout.block_bases);
   const int64_t blocks = layout.projection_blocks;
-  const int64_t scan_blocks = (blocks + kScanTile - 1) / kScanTile;
   // Partials live in the tail of block_bases' region, which holds 2*(blocks + scan_blocks) entries.
   int64_t* partials = block_bases + 2 * blocks;
+
+  if (blocks > 0 && layout.culling && scene.chunk_bounds != nullptr) {
+    int64_t* cull_flags = reinterpret_cast<int64_t*>(scratch + layout.cull_counts);
+    int64_t* cull_positions = reinterpret_cast<int64_t*>(scratch + layout.cull_bases);
+    uint64_t* cull_masks = reinterpret_cast<uint64_t*>(scratch + layout.cull_masks);
+    int32_t* items = reinterpret_cast<int32_t*>(scratch + layout.item_blocks);
+    uint64_t* item_masks = reinterpret_cast<uint64_t*>(scratch + layout.item_masks);
+    int64_t* item_totals = reinterpret_cast<int64_t*>(scratch + layout.item_totals);
+    const unsigned candidate_grid = static_cast<unsigned>((blocks + kBlockThreads - 1) / kBlockThreads);
+    hipLaunchKernelGGL(cull_chunks_kernel, dim3(candidate_grid), dim3(kBlockThreads), 0, stream, args,
+                       scene.chunk_bounds, blocks, cull_flags, cull_masks);
+    scan_pairs(cull_flags, blocks, cull_positions, cull_positions + 2 * blocks, 0, 0, nullptr, item_totals, stream);
+    hipLaunchKernelGGL(compact_items_kernel, dim3(candidate_grid), dim3(kBlockThreads), 0, stream, cull_flags,
+                       cull_positions, cull_masks, blocks, items, item_masks);
+    args.item_blocks = items;
+    args.item_masks = item_masks;
+    args.item_count = item_totals;
+    if (options.statistics != nullptr) launch_copy_i64(item_totals, options.statistics + 2, stream);
+  }
 
   if (blocks > 0) {
     hipLaunchKernelGGL(projection_count_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0,
                        stream, args, block_counts);
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(scan_blocks)), dim3(kBlockThreads), 0,
-                       stream, block_counts, blocks, partials);
   }
-  hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kBlockThreads), 0, stream, partials, scan_blocks,
-                     ws.visible_capacity, ws.intersection_capacity, ws.counters);
+  scan_pairs(block_counts, blocks, block_bases, partials, ws.visible_capacity, ws.intersection_capacity,
+             ws.counters, nullptr, stream);
   if (blocks > 0) {
-    hipLaunchKernelGGL(scan_downsweep_kernel, dim3(static_cast<unsigned>(scan_blocks)), dim3(kBlockThreads), 0,
-                       stream, block_counts, blocks, partials, block_bases);
     const dim3 grid(static_cast<unsigned>(blocks)), threads(kBlockThreads);
     switch (args.sh_degree) {
       case 1:
@@ -819,6 +1030,24 @@ extern "C" int32_t vgr_precompute_covariances(const float* scales, const float* 
   const hipError_t status = hipGetLastError();
   if (status != hipSuccess) {
     vgr::set_error("covariance launch failed: %s", hipGetErrorString(status));
+    return VGR_E_LAUNCH;
+  }
+  return 0;
+}
+
+extern "C" int32_t vgr_chunk_bounds(const float* means, const float* covariances, int64_t count, float* bounds,
+                                    void* stream) {
+  if (count < 0 || (count > 0 && (means == nullptr || covariances == nullptr || bounds == nullptr))) {
+    vgr::set_error("vgr_chunk_bounds: invalid arguments (count=%lld)", static_cast<long long>(count));
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (count == 0) return 0;
+  const int64_t chunks = (count + vgr::kBlockThreads - 1) / vgr::kBlockThreads;
+  hipLaunchKernelGGL(vgr::chunk_bounds_kernel, dim3(static_cast<unsigned>(chunks)), dim3(vgr::kBlockThreads), 0,
+                     static_cast<hipStream_t>(stream), means, covariances, count, bounds);
+  const hipError_t status = hipGetLastError();
+  if (status != hipSuccess) {
+    vgr::set_error("chunk bounds launch failed: %s", hipGetErrorString(status));
     return VGR_E_LAUNCH;
   }
   return 0;

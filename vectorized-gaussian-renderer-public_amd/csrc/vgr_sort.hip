@@ -317,6 +317,38 @@ __global__ __launch_bounds__(kBlockThreads) void ranges_kernel(const uint64_t* _
   }
 }
 
+// Restores Gaussian-ID order inside runs of equal keys (same tile, same depth
+// bits).  Needed only when the packed scene is stored in a spatial order, in
+// which case records of a camera are not in registration-id order.  Every
+// run is owned by the thread that sees its first element.
+__global__ __launch_bounds__(kBlockThreads) void tie_order_kernel(const uint64_t* __restrict__ keys,
+                                                                  int32_t* __restrict__ values,
+                                                                  const vgr_record* __restrict__ records,
+                                                                  const int64_t* __restrict__ counters,
+                                                                  int64_t capacity, int64_t visible_capacity) {
+  const int64_t n = element_count(counters, capacity);
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x; i + 1 < n;
+       i += static_cast<int64_t>(gridDim.x) * kBlockThreads) {
+    const uint64_t key = keys[i];
+    if (key == kSentinelKey || keys[i + 1] != key || (i > 0 && keys[i - 1] == key)) continue;
+    int64_t end = i + 1;
+    while (end < n && keys[end] == key) ++end;
+    for (int64_t j = i + 1; j < end; ++j) {
+      const int32_t moving = values[j];
+      const int32_t moving_id = (moving >= 0 && moving < visible_capacity) ? records[moving].gaussian_id : 0;
+      int64_t k = j - 1;
+      while (k >= i) {
+        const int32_t other = values[k];
+        const int32_t other_id = (other >= 0 && other < visible_capacity) ? records[other].gaussian_id : 0;
+        if (other_id <= moving_id) break;
+        values[k + 1] = other;
+        --k;
+      }
+      values[k + 1] = moving;
+    }
+  }
+}
+
 __global__ void fill_u32_kernel(uint32_t* __restrict__ data, int64_t n, uint32_t value) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -340,6 +372,10 @@ __global__ void fill_f32_kernel(float* __restrict__ data, int64_t n, float value
 
 __global__ void copy_u32_to_i64_kernel(const uint32_t* __restrict__ source, int64_t* __restrict__ target) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *target = static_cast<int64_t>(*source);
+}
+
+__global__ void copy_i64_kernel(const int64_t* __restrict__ source, int64_t* __restrict__ target) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *target = *source;
 }
 
 unsigned fill_grid(int64_t n) {
@@ -378,6 +414,11 @@ int launch_copy_u32_to_i64(const uint32_t* source, int64_t* target, hipStream_t 
   return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
 }
 
+int launch_copy_i64(const int64_t* source, int64_t* target, hipStream_t stream) {
+  hipLaunchKernelGGL(copy_i64_kernel, dim3(1), dim3(64), 0, stream, source, target);
+  return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
+}
+
 int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(fill_i64_kernel, dim3(fill_grid(n)), dim3(kBlockThreads), 0, stream, data, n, value);
@@ -391,7 +432,7 @@ int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream) {
 }
 
 int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, hipStream_t stream,
-                int* result_in_out_buffer) {
+                int* result_in_out_buffer, bool restore_id_order) {
   const int digit_bits = layout.sort_digit_bits;
   const int passes = layout.sort_passes;
   const int radix = 1 << digit_bits;
@@ -454,6 +495,14 @@ int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scra
     source_values = next_values;
   }
   *result_in_out_buffer = (passes % 2) == 1 ? 1 : 0;
+  if (restore_id_order) {
+    int64_t tie_blocks = (capacity + kBlockThreads * 8 - 1) / (kBlockThreads * 8);
+    tie_blocks = tie_blocks > 4096 ? 4096 : (tie_blocks < 1 ? 1 : tie_blocks);
+    hipLaunchKernelGGL(tie_order_kernel, dim3(static_cast<unsigned>(tie_blocks)), dim3(kBlockThreads), 0, stream,
+                       *result_in_out_buffer ? ws.keys_out : ws.keys_in,
+                       *result_in_out_buffer ? ws.values_out : ws.values_in, ws.records, ws.counters, capacity,
+                       ws.visible_capacity);
+  }
   const hipError_t error = hipGetLastError();
   if (error != hipSuccess) {
     set_error("sort launch failed: %s", hipGetErrorString(error));

@@ -1,0 +1,401 @@
+"""Batched-persistent rendering: many cameras of uneven resolution over several
+scenes in one device-only submission, optionally captured in a hipGraph.
+
+The reference renders one resolution per native submission and answers "many
+cameras" with sequential fixed-capacity chunks (``cuda_backend.py:759-967``).
+Here one :class:`BatchedRenderer` plan holds every camera's resolution, scene
+and output offset on the device, and one submission runs:
+
+  camera table -> frustum cull of (camera group x 256-Gaussian chunk) items
+  against conservative per-chunk bounds -> projection + emission over the kept
+  items -> onesweep radix ordering -> tile-ID tie order -> tile ranges ->
+  persistent compositing over a (camera, tile) work queue.
+
+Nothing in the submission reads a count on the host, so after one eager
+calibration render (which sizes the capacities) the whole submission is
+captured once with ``torch.cuda.CUDAGraph`` and replayed per frame.
+
+Scenes are stored in Morton order of their means (per scene) so chunk bounds
+are tight; records carry registration-order Gaussian IDs, so outputs, tie
+order and semantic IDs are those of the plain path.  ``render_plain`` runs the
+same batch as one submission per (scene, resolution) group through the plain
+path for comparison.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Any
+
+import torch
+
+from .hip_backend import COUNTER_NAMES, RendererCapacityError
+from .native import RECORD_FLOATS, infer_sh_degree, load_native_module
+
+
+@dataclass(frozen=True)
+class CameraSpec:
+    """Scene and resolution of one camera of a batched plan."""
+
+    scene_id: int
+    width: int
+    height: int
+
+
+def morton_order(means: torch.Tensor) -> torch.Tensor:
+    """Permutation sorting points by a 30-bit Morton code over their bounding box."""
+    lo = means.min(dim=0).values
+    hi = means.max(dim=0).values
+    span = torch.clamp(hi - lo, min=1e-12)
+    q = ((means - lo) / span * 1023.0).round().clamp(0, 1023).to(torch.int64)
+
+    def spread(v: torch.Tensor) -> torch.Tensor:
+        v = (v | (v << 16)) & 0x030000FF
+        v = (v | (v << 8)) & 0x0300F00F
+        v = (v | (v << 4)) & 0x030C30C3
+        v = (v | (v << 2)) & 0x09249249
+        return v
+
+    code = spread(q[:, 0]) | (spread(q[:, 1]) << 1) | (spread(q[:, 2]) << 2)
+    return torch.argsort(code, stable=True)
+
+
+class BatchedRenderer:
+    """Planned, graph-capturable renderer for a fixed camera layout."""
+
+    def __init__(
+        self,
+        *,
+        device: torch.device | str,
+        tile_size: int = 16,
+        near_plane: float = 0.01,
+        far_plane: float = 100.0,
+        gaussian_support_sigma: float = 3.33,
+        covariance_epsilon: float = 0.3,
+        antialiased: bool = False,
+        semantic_min_alpha: float = 0.01,
+        output_srgb: bool = False,
+        full_sensor_output: bool = True,
+        sh_degree: int | None = None,
+        spatial_order: bool = True,
+        chunk_culling: bool = True,
+        persistent_composite: int | None = None,
+        sort_digit_bits: int = 0,
+        sort_tile_items: int = 0,
+        raster_batch: int = 0,
+        cameras_per_block: int = 0,
+        capacity_headroom: float = 1.15,
+        native_module: Any | None = None,
+    ) -> None:
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError(f"BatchedRenderer requires a GPU device; got {self.device}.")
+        if tile_size not in (8, 16):
+            raise ValueError("tile_size must be 8 or 16.")
+        self.native = native_module if native_module is not None else load_native_module()
+        self.options = dict(
+            near_plane=near_plane, far_plane=far_plane, gaussian_support_sigma=gaussian_support_sigma,
+            covariance_epsilon=covariance_epsilon, antialiased=antialiased, semantic_min_alpha=semantic_min_alpha,
+            output_srgb=output_srgb, tile_size=tile_size)
+        self.full_sensor_output = full_sensor_output
+        self.requested_sh_degree = sh_degree
+        self.spatial_order = spatial_order
+        self.chunk_culling = chunk_culling
+        if persistent_composite is None:
+            persistent_composite = torch.cuda.get_device_properties(self.device).multi_processor_count * 5
+        self.persistent_composite = persistent_composite
+        self.tuning = dict(sort_digit_bits=sort_digit_bits, sort_tile_items=sort_tile_items,
+                           raster_batch=raster_batch, cameras_per_block=cameras_per_block)
+        self.capacity_headroom = capacity_headroom
+        self._scenes: dict[int, dict[str, torch.Tensor]] = {}
+        self._packed: dict[str, torch.Tensor] | None = None
+        self.cameras: list[CameraSpec] = []
+        self.graph: torch.cuda.CUDAGraph | None = None
+        self.workspace: dict[str, torch.Tensor] = {}
+
+    # ----------------------------------------------------------------------- scenes
+    def register_scene(self, scene_id: int, *, means, scales, rotations, opacities, features, semantic_ids) -> None:
+        if scene_id in self._scenes:
+            raise ValueError(f"Scene {scene_id} is already registered.")
+        tensors = dict(means=means, scales=scales, rotations=rotations, opacities=opacities, features=features,
+                       semantic_ids=semantic_ids)
+        width = None
+        for other in self._scenes.values():
+            width = other["features"].shape[1]
+        if width is not None and features.shape[1] != width:
+            raise ValueError("All packed scenes must use the same feature width.")
+        for name, tensor in tensors.items():
+            if tensor.device != self.device or not tensor.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous tensor on {self.device}.")
+        self._scenes[scene_id] = tensors
+        self._packed = None
+        self.graph = None
+
+    def _pack(self) -> dict[str, torch.Tensor]:
+        if self._packed is not None:
+            return self._packed
+        if not self._scenes:
+            raise RuntimeError("register_scene() before planning.")
+        order: list[torch.Tensor] = []
+        offset = 0
+        offsets = [0]
+        for tensors in self._scenes.values():
+            count = tensors["means"].shape[0]
+            local = morton_order(tensors["means"]) if self.spatial_order and count > 1 else torch.arange(
+                count, device=self.device)
+            order.append(local + offset)
+            offset += count
+            offsets.append(offset)
+        permutation = torch.cat(order)
+        plain = {name: torch.cat([t[name] for t in self._scenes.values()]).contiguous()
+                 for name in ("means", "scales", "rotations", "opacities", "features", "semantic_ids")}
+        packed = {name: plain[name][permutation].contiguous()
+                  for name in ("means", "scales", "rotations", "opacities", "features")}
+        packed["semantic_ids"] = plain["semantic_ids"]  # registration order: indexed by Gaussian ID
+        packed["gaussian_ids"] = permutation.to(torch.int32).contiguous()
+        g = packed["means"].shape[0]
+        packed["covariances"] = torch.empty((g, 6), device=self.device, dtype=torch.float32)
+        self.native.precompute_covariances(packed["scales"], packed["rotations"], packed["covariances"])
+        packed["chunk_bounds"] = torch.empty((math.ceil(g / 256), 8), device=self.device, dtype=torch.float32)
+        self.native.chunk_bounds(packed["means"], packed["covariances"], packed["chunk_bounds"])
+        packed["registered_scene_ids"] = torch.tensor(list(self._scenes), device=self.device, dtype=torch.int64)
+        packed["scene_offsets"] = torch.tensor(offsets, device=self.device, dtype=torch.int64)
+        plain["covariances"] = torch.empty((g, 6), device=self.device, dtype=torch.float32)
+        self.native.precompute_covariances(plain["scales"], plain["rotations"], plain["covariances"])
+        packed["plain"] = plain
+        width = packed["features"].shape[1]
+        self.sh_degree = self.requested_sh_degree if self.requested_sh_degree is not None else infer_sh_degree(width)
+        self._packed = packed
+        return packed
+
+    # ------------------------------------------------------------------------- plan
+    def plan(self, cameras: list[CameraSpec], *, visible_capacity: int | None = None,
+             intersection_capacity: int | None = None) -> None:
+        """Fix the camera layout and allocate static inputs, outputs and workspace."""
+        packed = self._pack()
+        if not cameras:
+            raise ValueError("plan() needs at least one camera.")
+        for spec in cameras:
+            if spec.scene_id not in self._scenes:
+                raise ValueError(f"Camera refers to unregistered scene {spec.scene_id}.")
+            if spec.width <= 0 or spec.height <= 0:
+                raise ValueError("camera resolutions must be positive.")
+        self.cameras = list(cameras)
+        b = len(cameras)
+        ts = self.options["tile_size"]
+        pixels = [c.width * c.height for c in cameras]
+        self.pixel_offsets_host = [0]
+        for count in pixels[:-1]:
+            self.pixel_offsets_host.append(self.pixel_offsets_host[-1] + count)
+        self.total_pixels = sum(pixels)
+        self.total_tiles = sum(math.ceil(c.width / ts) * math.ceil(c.height / ts) for c in cameras)
+        dev = self.device
+        self.widths = torch.tensor([c.width for c in cameras], dtype=torch.int32, device=dev)
+        self.heights = torch.tensor([c.height for c in cameras], dtype=torch.int32, device=dev)
+        self.pixel_offsets = torch.tensor(self.pixel_offsets_host, dtype=torch.int64, device=dev)
+        self.scene_ids = torch.tensor([c.scene_id for c in cameras], dtype=torch.int64, device=dev)
+        self.viewmats = torch.eye(4, device=dev).repeat(b, 1, 1).contiguous()
+        self.intrinsics = torch.eye(3, device=dev).repeat(b, 1, 1).contiguous()
+        self.env_xforms = torch.eye(4, device=dev).repeat(b, 1, 1).contiguous()
+        self.empty_active = torch.empty(0, dtype=torch.int64, device=dev)
+        self.outputs = {
+            "rgb": torch.zeros((self.total_pixels, 3), device=dev, dtype=torch.float32),
+            "depth": torch.zeros((self.total_pixels, 1), device=dev, dtype=torch.float32),
+            "alpha": torch.zeros((self.total_pixels, 1), device=dev, dtype=torch.float32),
+            "semantic_id": torch.full((self.total_pixels, 1), -1, device=dev, dtype=torch.int64),
+        }
+        g = packed["means"].shape[0]
+        if visible_capacity is None:
+            visible_capacity = min(b * g, max(1, b * 200_000))
+        if intersection_capacity is None:
+            intersection_capacity = min((1 << 30) - 1, 3 * visible_capacity)
+        self._allocate(visible_capacity, intersection_capacity)
+        # Plain-path groups: one submission per (scene, resolution).
+        groups: dict[tuple[int, int, int], list[int]] = {}
+        for index, spec in enumerate(cameras):
+            groups.setdefault((spec.scene_id, spec.width, spec.height), []).append(index)
+        self.plain_groups = []
+        for (scene_id, width, height), indices in groups.items():
+            idx = torch.tensor(indices, dtype=torch.int64, device=dev)
+            outputs = {
+                "rgb": torch.zeros((len(indices), height, width, 3), device=dev),
+                "depth": torch.zeros((len(indices), height, width, 1), device=dev),
+                "alpha": torch.zeros((len(indices), height, width, 1), device=dev),
+                "semantic_id": torch.full((len(indices), height, width, 1), -1, device=dev, dtype=torch.int64),
+            }
+            self.plain_groups.append(dict(scene_id=scene_id, width=width, height=height, indices=idx,
+                                          index_list=indices, outputs=outputs))
+        self.graph = None
+
+    def _scratch_bytes(self, intersection_capacity: int) -> int:
+        packed = self._pack()
+        g = packed["means"].shape[0]
+        batched = self.native.scratch_bytes(
+            max_views=len(self.cameras), height=1, width=1, gaussians=g, intersection_capacity=intersection_capacity,
+            tile_size=self.options["tile_size"], near_plane=self.options["near_plane"],
+            far_plane=self.options["far_plane"], sort_digit_bits=self.tuning["sort_digit_bits"],
+            cameras_per_block=self.tuning["cameras_per_block"], sort_tile_items=self.tuning["sort_tile_items"],
+            chunk_culling=self.chunk_culling, resolutions=[(c.width, c.height) for c in self.cameras])
+        plain = self.native.scratch_bytes(
+            max_views=len(self.cameras), height=max(c.height for c in self.cameras),
+            width=max(c.width for c in self.cameras), gaussians=g, intersection_capacity=intersection_capacity,
+            tile_size=self.options["tile_size"], near_plane=self.options["near_plane"],
+            far_plane=self.options["far_plane"], sort_digit_bits=self.tuning["sort_digit_bits"],
+            cameras_per_block=self.tuning["cameras_per_block"], sort_tile_items=self.tuning["sort_tile_items"])
+        return max(batched, plain)
+
+    def _allocate(self, visible_capacity: int, intersection_capacity: int) -> None:
+        dev = self.device
+        self.workspace = {}
+        torch.cuda.empty_cache()
+        ts = self.options["tile_size"]
+        plain_tiles = len(self.cameras) * math.ceil(max(c.width for c in self.cameras) / ts) * \
+            math.ceil(max(c.height for c in self.cameras) / ts)
+        tiles = max(self.total_tiles, plain_tiles)
+        self.workspace = {
+            "visible_records": torch.empty((visible_capacity, RECORD_FLOATS), device=dev, dtype=torch.float32),
+            "keys_in": torch.empty(intersection_capacity, device=dev, dtype=torch.uint64),
+            "keys_out": torch.empty(intersection_capacity, device=dev, dtype=torch.uint64),
+            "values_in": torch.empty(intersection_capacity, device=dev, dtype=torch.int32),
+            "values_out": torch.empty(intersection_capacity, device=dev, dtype=torch.int32),
+            "tile_starts": torch.empty(tiles, device=dev, dtype=torch.int32),
+            "tile_ends": torch.empty(tiles, device=dev, dtype=torch.int32),
+            "counters": torch.zeros(5, device=dev, dtype=torch.int64),
+            "scratch": torch.empty(self._scratch_bytes(intersection_capacity), device=dev, dtype=torch.uint8),
+        }
+        self.visible_capacity = visible_capacity
+        self.intersection_capacity = intersection_capacity
+        self.graph = None
+
+    # ---------------------------------------------------------------------- submit
+    def _workspace_list(self) -> list[torch.Tensor]:
+        ws = self.workspace
+        return [ws["visible_records"], ws["keys_in"], ws["keys_out"], ws["values_in"], ws["values_out"],
+                ws["tile_starts"], ws["tile_ends"], ws["counters"], ws["scratch"]]
+
+    def _submit(self, *, stage_events=None, statistics=None) -> None:
+        packed = self._pack()
+        o = self.outputs
+        scene = [packed["means"], packed["covariances"], packed["opacities"], packed["features"],
+                 packed["semantic_ids"], packed["registered_scene_ids"], packed["scene_offsets"]]
+        cameras = [self.viewmats, self.intrinsics, self.env_xforms, self.scene_ids, self.empty_active]
+        outputs = [o["rgb"], o["depth"], o["alpha"], o["semantic_id"]]
+        extra: dict[str, Any] = {}
+        if stage_events is not None:
+            extra["stage_events"] = stage_events
+        if statistics is not None:
+            extra["statistics"] = statistics
+        selector = self.native.render(
+            scene, cameras, outputs, self._workspace_list(), 1, 1, 0, self.options["near_plane"],
+            self.options["far_plane"], self.options["gaussian_support_sigma"], self.options["covariance_epsilon"],
+            self.options["antialiased"], self.options["semantic_min_alpha"], False, self.options["tile_size"],
+            4096, 64, False, False, False, self.options["output_srgb"], True, self.full_sensor_output, False,
+            sh_degree=self.sh_degree, persistent_composite=self.persistent_composite,
+            gaussian_ids=packed["gaussian_ids"] if self.spatial_order else None,
+            chunk_bounds=packed["chunk_bounds"] if self.chunk_culling else None,
+            resolutions=(self.widths, self.heights, self.pixel_offsets, self.total_tiles), **self.tuning,
+            **extra)
+        self._selector = selector
+
+    def read_counters(self) -> dict[str, int]:
+        torch.cuda.synchronize(self.device)
+        return dict(zip(COUNTER_NAMES, (int(v) for v in self.workspace["counters"].cpu().tolist())))
+
+    def check_capacity(self) -> dict[str, int]:
+        counters = self.read_counters()
+        if counters["visible_overflow"] or counters["intersection_overflow"]:
+            raise RendererCapacityError(
+                "Batched renderer workspace overflow: "
+                f"visible={counters['visible_overflow']} intersections={counters['intersection_overflow']}.",
+                telemetry={"schema_version": "renderer-capacity-error/v1", "reason": "batched-overflow",
+                           "counters": counters,
+                           "capacity": {"visible_records": self.visible_capacity,
+                                        "tile_intersections": self.intersection_capacity}})
+        return counters
+
+    def set_cameras(self, viewmats: torch.Tensor, intrinsics: torch.Tensor,
+                    env_xforms: torch.Tensor | None = None) -> None:
+        """Copy a frame's camera tensors into the plan's static inputs."""
+        b = len(self.cameras)
+        if viewmats.shape != (b, 4, 4) or intrinsics.shape != (b, 3, 3):
+            raise ValueError(f"expected viewmats [{b},4,4] and intrinsics [{b},3,3].")
+        self.viewmats.copy_(viewmats)
+        self.intrinsics.copy_(intrinsics)
+        if env_xforms is not None:
+            self.env_xforms.copy_(env_xforms)
+        else:
+            self.env_xforms.copy_(torch.eye(4, device=self.device).expand(b, 4, 4))
+
+    def calibrate(self, viewmats: torch.Tensor, intrinsics: torch.Tensor, env_xforms: torch.Tensor | None = None,
+                  max_retries: int = 4) -> dict[str, int]:
+        """Eager renders that size the capacities to the demand (x headroom)."""
+        self.set_cameras(viewmats, intrinsics, env_xforms)
+        for _ in range(max_retries + 1):
+            self._submit()
+            counters = self.read_counters()
+            if not (counters["visible_overflow"] or counters["intersection_overflow"]):
+                return counters
+            visible = max(self.visible_capacity, math.ceil(counters["visible_gaussians"] * self.capacity_headroom))
+            intersections = max(self.intersection_capacity,
+                                math.ceil(counters["tile_intersections"] * self.capacity_headroom))
+            if intersections > (1 << 30) - 1:
+                raise RendererCapacityError("intersection demand exceeds the 2^30 sort limit",
+                                            telemetry={"counters": counters})
+            self._allocate(visible, intersections)
+        raise RendererCapacityError("capacity calibration did not converge", telemetry={"counters": counters})
+
+    def capture(self) -> torch.cuda.CUDAGraph:
+        """Capture one whole submission into a graph (after calibrate())."""
+        stream = torch.cuda.Stream(device=self.device)
+        stream.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(stream):
+            self._submit()  # warm-up on the capture stream
+        torch.cuda.current_stream(self.device).wait_stream(stream)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            self._submit()
+        self.graph = graph
+        return graph
+
+    def render(self, viewmats: torch.Tensor | None = None, intrinsics: torch.Tensor | None = None,
+               env_xforms: torch.Tensor | None = None) -> dict[str, torch.Tensor]:
+        """Render the planned batch (graph replay when captured); returns flat [P, C] outputs."""
+        if viewmats is not None or intrinsics is not None:
+            self.set_cameras(viewmats if viewmats is not None else self.viewmats,
+                             intrinsics if intrinsics is not None else self.intrinsics, env_xforms)
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self._submit()
+        return self.outputs
+
+    def camera_outputs(self, index: int, outputs: dict[str, torch.Tensor] | None = None) -> dict[str, torch.Tensor]:
+        """[H, W, C] views of camera ``index`` in the flat outputs."""
+        outputs = outputs if outputs is not None else self.outputs
+        spec = self.cameras[index]
+        start = self.pixel_offsets_host[index]
+        stop = start + spec.width * spec.height
+        return {name: tensor[start:stop].view(spec.height, spec.width, tensor.shape[1])
+                for name, tensor in outputs.items()}
+
+    # ------------------------------------------------------------------- plain path
+    def render_plain(self) -> list[dict[str, Any]]:
+        """Same batch as one plain submission per (scene, resolution) group."""
+        packed = self._pack()
+        plain = packed["plain"]
+        scene = [plain["means"], plain["covariances"], plain["opacities"], plain["features"],
+                 plain["semantic_ids"], packed["registered_scene_ids"], packed["scene_offsets"]]
+        for group in self.plain_groups:
+            idx = group["indices"]
+            cameras = [self.viewmats.index_select(0, idx), self.intrinsics.index_select(0, idx),
+                       self.env_xforms.index_select(0, idx), self.scene_ids.index_select(0, idx), self.empty_active]
+            o = group["outputs"]
+            self.native.render(
+                scene, cameras, [o["rgb"], o["depth"], o["alpha"], o["semantic_id"]], self._workspace_list(),
+                group["height"], group["width"], 0, self.options["near_plane"], self.options["far_plane"],
+                self.options["gaussian_support_sigma"], self.options["covariance_epsilon"],
+                self.options["antialiased"], self.options["semantic_min_alpha"], False, self.options["tile_size"],
+                4096, 64, False, False, False, self.options["output_srgb"], True, self.full_sensor_output, False,
+                sh_degree=self.sh_degree, **self.tuning)
+        return self.plain_groups

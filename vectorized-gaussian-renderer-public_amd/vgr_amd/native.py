@@ -59,6 +59,8 @@ class _Scene(ctypes.Structure):
         ("scene_count", ctypes.c_int32),
         ("sh_degree", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("gaussian_ids", ctypes.c_void_p),
+        ("chunk_bounds", ctypes.c_void_p),
     ]
 
 
@@ -129,6 +131,8 @@ class _Options(ctypes.Structure):
         ("stage_event_count", ctypes.c_int32),
         ("sort_tile_items", ctypes.c_int32),
         ("statistics", ctypes.c_void_p),
+        ("persistent_composite", ctypes.c_int32),
+        ("chunk_culling", ctypes.c_int32),
     ]
 
 
@@ -153,6 +157,7 @@ EXPORTED_SYMBOLS: dict[str, tuple[type, tuple]] = {
         _I32, (ctypes.POINTER(_Cameras), ctypes.POINTER(_Options), _I64, _I64, ctypes.POINTER(_Sizes))),
     "vgr_sort_temp_bytes": (_I64, (_I64, _I64)),
     "vgr_precompute_covariances": (_I32, (_P, _P, _I64, _P, _P)),
+    "vgr_chunk_bounds": (_I32, (_P, _P, _I64, _P, _P)),
     "vgr_render": (
         _I32,
         (ctypes.POINTER(_Scene), ctypes.POINTER(_Cameras), ctypes.POINTER(_Outputs), ctypes.POINTER(_Workspace),
@@ -248,7 +253,7 @@ class NativeRenderer:
 
     def scratch_bytes(self, *, max_views: int, height: int, width: int, gaussians: int, intersection_capacity: int,
                       tile_size: int, near_plane: float, far_plane: float, sort_digit_bits: int = 0,
-                      cameras_per_block: int = 0, sort_tile_items: int = 0,
+                      cameras_per_block: int = 0, sort_tile_items: int = 0, chunk_culling: bool = False,
                       resolutions: Sequence[tuple[int, int]] | None = None) -> int:
         """Scratch bytes one render of up to ``max_views`` cameras needs."""
         cameras = _Cameras()
@@ -262,7 +267,7 @@ class NativeRenderer:
         options = self._options(height=height, width=width, near_plane=near_plane, far_plane=far_plane,
                                 gaussian_support_sigma=3.0, covariance_epsilon=0.0, tile_size=tile_size,
                                 sort_digit_bits=sort_digit_bits, cameras_per_block=cameras_per_block,
-                                sort_tile_items=sort_tile_items)
+                                sort_tile_items=sort_tile_items, chunk_culling=int(bool(chunk_culling)))
         sizes = _Sizes()
         self._check(self.lib.vgr_workspace_sizes_query(ctypes.byref(cameras), ctypes.byref(options), int(gaussians),
                                                         int(intersection_capacity), ctypes.byref(sizes)),
@@ -277,6 +282,17 @@ class NativeRenderer:
             self._check(self.lib.vgr_precompute_covariances(_ptr(scales), _ptr(rotations), count, _ptr(out),
                                                             _stream(scales.device)),
                         "vgr_precompute_covariances")
+
+    def chunk_bounds(self, means: torch.Tensor, covariances: torch.Tensor, out: torch.Tensor) -> None:
+        """Per-chunk (256 Gaussians) bounds: min xyz, max xyz, sqrt(max trace(cov)), 0."""
+        count = int(means.shape[0])
+        if count == 0:
+            return
+        if out.dtype != torch.float32 or out.numel() < 8 * ((count + 255) // 256):
+            raise ValueError("chunk bounds output must be float32 [ceil(G/256), 8]")
+        with torch.cuda.device(means.device):
+            self._check(self.lib.vgr_chunk_bounds(_ptr(means), _ptr(covariances), count, _ptr(out),
+                                                  _stream(means.device)), "vgr_chunk_bounds")
 
     def prepare_chunked_active_ids(self, active_ids: torch.Tensor, active_mask: torch.Tensor,
                                    chunk_active_ids: torch.Tensor, logical_batch: int, physical_batch: int) -> None:
@@ -325,6 +341,9 @@ class NativeRenderer:
         raster_batch: int = 0,
         cameras_per_block: int = 0,
         sort_tile_items: int = 0,
+        persistent_composite: int = 0,
+        gaussian_ids: torch.Tensor | None = None,
+        chunk_bounds: torch.Tensor | None = None,
         resolutions: tuple[torch.Tensor, torch.Tensor, torch.Tensor, int] | None = None,
         stage_events: Sequence[torch.cuda.Event] | None = None,
         statistics: torch.Tensor | None = None,
@@ -378,7 +397,13 @@ class NativeRenderer:
             semantic_ids=_ptr(semantic_ids), registered_scene_ids=_ptr(registered), scene_offsets=_ptr(offsets),
             gaussian_count=int(means.shape[0]), max_scene_gaussians=int(max_scene_gaussians),
             feature_width=int(features.shape[1]) if features.ndim == 2 else 0,
-            scene_count=int(registered.numel()), sh_degree=int(sh_degree), reserved=0)
+            scene_count=int(registered.numel()), sh_degree=int(sh_degree), reserved=0,
+            gaussian_ids=_ptr(gaussian_ids), chunk_bounds=_ptr(chunk_bounds))
+        if gaussian_ids is not None and (gaussian_ids.dtype != torch.int32 or gaussian_ids.numel() != means.shape[0]):
+            raise ValueError("gaussian_ids must be int32 with one entry per packed Gaussian")
+        if chunk_bounds is not None and (chunk_bounds.dtype != torch.float32
+                                         or chunk_bounds.numel() < 8 * ((int(means.shape[0]) + 255) // 256)):
+            raise ValueError("chunk_bounds must be float32 [ceil(G/256), 8]")
         batch = int(viewmats.shape[0])
         c_cameras = _Cameras(
             viewmats=_ptr(viewmats), intrinsics=_ptr(intrinsics), env_xforms=_ptr(env_xforms),
@@ -408,7 +433,8 @@ class NativeRenderer:
             full_sensor_output=int(bool(full_sensor_output)), fixed_capacity_sort=int(bool(fixed_capacity_sort)),
             sort_digit_bits=int(sort_digit_bits), raster_batch=int(raster_batch),
             cameras_per_block=int(cameras_per_block), reuse_projection=int(bool(reuse_projection)),
-            sort_tile_items=int(sort_tile_items))
+            sort_tile_items=int(sort_tile_items), persistent_composite=int(persistent_composite),
+            chunk_culling=int(chunk_bounds is not None))
         event_array = None
         if stage_events is not None:
             if len(stage_events) != 5:
