@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define VGR_ABI_VERSION 1
+#define VGR_ABI_VERSION 2
 
 enum vgr_status {
   VGR_OK = 0,
@@ -100,6 +100,13 @@ typedef struct vgr_outputs {
   float* depth;                     /* [.,1] */
   float* alpha;                     /* [.,1] */
   int64_t* semantic;                /* [.,1] */
+  /* Optional training checkpoints (both or neither; need full_sensor_output
+   * and output_srgb == 0).  Per pixel: the final transmittance T and the
+   * position, within the pixel's tile list, one past the last blended splat
+   * (0 when none).  No reference counterpart: the reference has no backward
+   * pass (KERNEL_ARCHITECTURE.md:454-460). */
+  float* final_transmittance;       /* [.] */
+  int32_t* last_blended;            /* [.] */
 } vgr_outputs;
 
 /* Visible-record layout (48 bytes, 16-byte aligned).  One record per visible
@@ -214,6 +221,45 @@ int32_t vgr_precompute_covariances(const float* scales, const float* rotations,
 int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras,
                    const vgr_outputs* outputs, const vgr_workspace* workspace,
                    const vgr_options* options, void* stream, int64_t* selector_out);
+
+/* Gradients of one checkpointed forward pass.  Pixel-indexed inputs use the
+ * forward's output layout; grad_depth / grad_alpha may be NULL (zero).  Scene
+ * gradients are indexed like the packed scene arrays and are ADDED to (the
+ * caller zero-fills them); grad_viewmats [B,4,4] may be NULL (no pose
+ * gradient).  record_grads is workspace of 10 floats per visible record
+ * (dL/d mean_x, mean_y, conic_0..2, opacity, depth, r, g, b). */
+typedef struct vgr_gradients {
+  const float* grad_rgb;            /* [.,3] dL/d rgb */
+  const float* grad_depth;          /* [.]   dL/d depth (ignored where depth is +inf) */
+  const float* grad_alpha;          /* [.]   dL/d alpha */
+  float* record_grads;              /* [visible_capacity, 10] workspace */
+  float* grad_means;                /* [G,3] */
+  float* grad_covariances;          /* [G,6] upper triangle, as scene.covariances */
+  float* grad_opacities;            /* [G] */
+  float* grad_features;             /* [G,feature_width]; the first 3*(d+1)^2 columns receive gradients */
+  float* grad_viewmats;             /* [B,16] or NULL */
+  float* camera_grads;              /* [active_count, 24] workspace when grad_viewmats != NULL */
+  int64_t sorted_selector;          /* the forward's *selector_out */
+} vgr_gradients;
+
+/* Backward pass of vgr_render.  Must follow the checkpointed vgr_render with
+ * the same scene, cameras, options and workspace, untouched in between: it
+ * re-uses the visible records, the sorted tile lists and the projection block
+ * layout left there.  Compositing runs back to front from the checkpointed
+ * final transmittance (no per-intersection state is stored); projection,
+ * EWA, SH and camera-pose gradients are recomputed per visible pair.  Float
+ * atomics merge contributions, so results are order-dependent to rounding.
+ * No reference counterpart (the reference is forward-only,
+ * KERNEL_ARCHITECTURE.md:454-460). */
+int32_t vgr_render_backward(const vgr_scene* scene, const vgr_cameras* cameras,
+                            const vgr_outputs* outputs, const vgr_workspace* workspace,
+                            const vgr_options* options, const vgr_gradients* gradients, void* stream);
+
+/* Chain rule of vgr_precompute_covariances: from dL/d covariances to dL/d
+ * scales [G,3] and dL/d rotations [G,4] (unnormalized WXYZ quaternions; the
+ * normalization is differentiated).  Outputs are overwritten. */
+int32_t vgr_covariance_backward(const float* scales, const float* rotations, const float* grad_covariances,
+                                int64_t count, float* grad_scales, float* grad_rotations, void* stream);
 
 /* Logical-batch chunking helpers (renderer_cuda.cu:2307-2360). */
 int32_t vgr_prepare_chunked_active_ids(const int64_t* active_ids, int64_t active_count,

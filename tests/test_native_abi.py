@@ -26,7 +26,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     library = native.load_library()
     for name in declared_functions():
         assert hasattr(library, name), name
-    assert library.vgr_abi_version() == 1
+    assert library.vgr_abi_version() == native.ABI_VERSION
 
 
 @pytest.mark.parametrize(("tiles", "end_bit"), [(1, 32), (2, 33), (3, 34), (4, 34), (1 << 20, 52),

@@ -204,19 +204,12 @@ extern "C" int64_t vgr_sort_temp_bytes(int64_t n_items, int64_t n_segments) {
   return static_cast<int64_t>(layout.total - layout.sort_histogram);
 }
 
-extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras, const vgr_outputs* outputs,
-                              const vgr_workspace* workspace, const vgr_options* options, void* stream_handle,
-                              int64_t* selector_out) {
-  if (scene == nullptr || cameras == nullptr || outputs == nullptr || workspace == nullptr || options == nullptr) {
-    set_error("vgr_render: null argument struct");
-    return VGR_E_INVALID_ARGUMENT;
-  }
-  const vgr_scene& sc = *scene;
-  const vgr_cameras& cams = *cameras;
-  const vgr_outputs& out = *outputs;
-  const vgr_workspace& ws = *workspace;
-  const vgr_options& opt = *options;
-  hipStream_t stream = static_cast<hipStream_t>(stream_handle);
+namespace vgr {
+namespace {
+
+// Validates one render's arguments and resolves its scratch layout and options.
+int prepare_render(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outputs& out, const vgr_workspace& ws,
+                   const vgr_options& opt, ScratchLayout& layout_out, int64_t& tiles_out, vgr_options& resolved_out) {
   int status = validate_options(opt, cams);
   if (status != 0) return status;
   if (sc.gaussian_count < 0 || sc.gaussian_count > 0x7fffffffLL) {
@@ -302,6 +295,34 @@ extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras
   resolved.cameras_per_block = layout.cameras_per_block;
   if (resolved.raster_batch == 0) resolved.raster_batch = 256;
 
+  layout_out = layout;
+  tiles_out = tiles;
+  resolved_out = resolved;
+  return 0;
+}
+
+}  // namespace
+}  // namespace vgr
+
+extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras, const vgr_outputs* outputs,
+                              const vgr_workspace* workspace, const vgr_options* options, void* stream_handle,
+                              int64_t* selector_out) {
+  if (scene == nullptr || cameras == nullptr || outputs == nullptr || workspace == nullptr || options == nullptr) {
+    set_error("vgr_render: null argument struct");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const vgr_scene& sc = *scene;
+  const vgr_cameras& cams = *cameras;
+  const vgr_outputs& out = *outputs;
+  const vgr_workspace& ws = *workspace;
+  const vgr_options& opt = *options;
+  hipStream_t stream = static_cast<hipStream_t>(stream_handle);
+  ScratchLayout layout;
+  int64_t tiles = 0;
+  vgr_options resolved;
+  int status = prepare_render(sc, cams, out, ws, opt, layout, tiles, resolved);
+  if (status != 0) return status;
+
   if (opt.stage_event_count != 0 && (opt.stage_event_count != 5 || opt.stage_events == nullptr)) {
     set_error("stage_events must hold exactly 5 events");
     return VGR_E_INVALID_ARGUMENT;
@@ -343,6 +364,65 @@ extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras
   }
   if (selector_out != nullptr) *selector_out = sorted_in_out;
   return 0;
+}
+
+extern "C" int32_t vgr_render_backward(const vgr_scene* scene, const vgr_cameras* cameras,
+                                       const vgr_outputs* outputs, const vgr_workspace* workspace,
+                                       const vgr_options* options, const vgr_gradients* gradients,
+                                       void* stream_handle) {
+  if (scene == nullptr || cameras == nullptr || outputs == nullptr || workspace == nullptr || options == nullptr ||
+      gradients == nullptr) {
+    set_error("vgr_render_backward: null argument struct");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const vgr_scene& sc = *scene;
+  const vgr_cameras& cams = *cameras;
+  const vgr_outputs& out = *outputs;
+  const vgr_workspace& ws = *workspace;
+  const vgr_gradients& grads = *gradients;
+  ScratchLayout layout;
+  int64_t tiles = 0;
+  vgr_options resolved;
+  int status = prepare_render(sc, cams, out, ws, *options, layout, tiles, resolved);
+  if (status != 0) return status;
+  if (out.final_transmittance == nullptr || out.last_blended == nullptr || !resolved.full_sensor_output ||
+      resolved.output_srgb) {
+    set_error("backward needs the forward's transmittance checkpoints, full sensor output and linear RGB");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (grads.grad_rgb == nullptr || grads.record_grads == nullptr) {
+    set_error("backward needs grad_rgb and the record_grads workspace");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const bool scene_grads = grads.grad_means != nullptr || grads.grad_covariances != nullptr ||
+                           grads.grad_opacities != nullptr || grads.grad_features != nullptr;
+  if (scene_grads && (grads.grad_means == nullptr || grads.grad_covariances == nullptr ||
+                      grads.grad_opacities == nullptr || grads.grad_features == nullptr)) {
+    set_error("scene gradients are all given or all NULL");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (grads.grad_viewmats != nullptr && grads.camera_grads == nullptr) {
+    set_error("pose gradients need the camera_grads workspace");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (resolved.reuse_projection) {
+    set_error("backward replays the forward's own projection; reuse_projection must be 0");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  return launch_backward(sc, cams, out, resolved, ws, layout, static_cast<const char*>(ws.scratch), grads,
+                         static_cast<hipStream_t>(stream_handle));
+}
+
+extern "C" int32_t vgr_covariance_backward(const float* scales, const float* rotations, const float* grad_covariances,
+                                           int64_t count, float* grad_scales, float* grad_rotations,
+                                           void* stream_handle) {
+  if (count < 0 || (count > 0 && (scales == nullptr || rotations == nullptr || grad_covariances == nullptr ||
+                                  grad_scales == nullptr || grad_rotations == nullptr))) {
+    set_error("vgr_covariance_backward: null tensor or negative count");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  return launch_covariance_backward(scales, rotations, grad_covariances, count, grad_scales, grad_rotations,
+                                    static_cast<hipStream_t>(stream_handle));
 }
 
 namespace vgr {

@@ -12,14 +12,12 @@
 // The per-pixel arithmetic is the reference's, term by term, with vgr_expf in
 // place of __expf.  The semantic winner is tracked as a Gaussian index and the
 // int64 semantic label is gathered once per pixel at the end.
-#include "vgr_internal.h"
+#include "vgr_composite_device.h"
 
 namespace vgr {
 namespace {
 
-// Native 4-wide vector (first-class in LLVM, so per-thread arrays of it stay in
-// registers; HIP's float4 wrapper struct defeats scalar replacement).
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+using namespace dev;
 
 struct CompositeArgs {
   const CameraParams* table;
@@ -41,107 +39,20 @@ struct CompositeArgs {
   int uniform_tiles;     // tiles per camera when every camera has the same resolution, else 0
   int output_srgb;
   float semantic_min_alpha;
+  float* final_transmittance;  // optional per-pixel checkpoint (training forward)
+  int32_t* last_blended;       // optional: tile-list position one past the last blended splat
 };
 
-__device__ __forceinline__ int find_active_camera(const CameraParams* table, int active_count, int64_t block) {
-  int lo = 0, hi = active_count - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (table[mid].active_tile_begin <= block) {
-      lo = mid;
-    } else {
-      hi = mid - 1;
-    }
-  }
-  return lo;
-}
-
-// Loads the records of one batch of the sorted list into registers.
-template <int kThreads, int kBatch, int kPerThread>
-__device__ __forceinline__ void gather_batch(const CompositeArgs& args, int64_t batch_start, int64_t range_end,
-                                             int lane, f32x4 (&a)[kPerThread], f32x4 (&b)[kPerThread],
-                                             f32x4 (&c)[kPerThread]) {
-#pragma unroll
-  for (int k = 0; k < kPerThread; ++k) {
-    const int64_t index = batch_start + k * kThreads + lane;
-    if (k * kThreads + lane < kBatch && index < range_end) {
-      const int32_t record = args.sorted_values[index];
-      if (record >= 0 && record < args.visible_capacity) {
-        const f32x4* src = reinterpret_cast<const f32x4*>(args.records + record);
-        a[k] = src[0];
-        b[k] = src[1];
-        c[k] = src[2];
-      } else {
-        // Only reachable with a corrupted list: contributes nothing (opacity 0).
-        a[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        b[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        c[k] = f32x4{0.0f, 0.0f, __int_as_float(-1), 0.0f};
-      }
-    }
-  }
-}
-
-// Margin subtracted from the per-Gaussian log-threshold: far larger than the
-// error of vgr_logf/vgr_expf and of the alpha product, so "power below the
-// threshold" implies the exact alpha test fails.
-constexpr float kThresholdMargin = 1.0e-4f;
-
-// Staged form of one record.  The power of the reference,
-//   -0.5 * (c0*dx*dx + 2*c1*dx*dy + c2*dy*dy),
-// is evaluated as (A*dx*dx + B*dx*dy) + C*dy*dy with A = -0.5*c0, B = -c1,
-// C = -0.5*c2: scaling by powers of two commutes with rounding, so the value is
-// bit-identical and two multiplies cheaper.
-struct StagedSplat {
-  f32x4 geometry;  // mean_x, mean_y, A, B
-  f32x4 shading;   // C, opacity, power threshold, depth
-  f32x4 colour;    // red, green, blue, gaussian id bits
-};
-
-__device__ __forceinline__ StagedSplat stage_record(const f32x4& a, const f32x4& b, const f32x4& c) {
-  StagedSplat s;
-  const float opacity = b.y;
-  // power < threshold  =>  min(0.99, opacity * exp(power)) < 1/255 exactly.
-  const float threshold = opacity > 0.0f ? vgr_logf(VGR_ALPHA_THRESHOLD / opacity) - kThresholdMargin
-                                         : __int_as_float(0x7f800000);
-  s.geometry = f32x4{a.x, a.y, -0.5f * a.z, -a.w};
-  s.shading = f32x4{-0.5f * b.x, opacity, threshold, b.z};
-  s.colour = f32x4{b.w, c.x, c.y, c.z};
-  return s;
-}
-
-// Half extents of a box containing every pixel offset whose float-evaluated
-// power can reach the threshold: the ellipse d^T C d <= R^2 (R^2 = -2 threshold)
-// inflated by a bound on the rounding error of the power expression, which
-// grows with the conic's condition.  Returns false when no finite box is
-// certain (non-positive-definite or extremely anisotropic conic).
-__device__ __forceinline__ bool splat_extent(const f32x4& a, const f32x4& b, float threshold, float& half_x,
-                                             float& half_y) {
-  const double c0 = a.z, c1 = a.w, c2 = b.x;
-  const double det = c0 * c2 - c1 * c1;
-  if (!(det > 0.0) || !(threshold < 0.0f)) return false;
-  const double sxx = c2 / det;
-  const double syy = c0 / det;
-  const double condition = (fabs(c0) + 2.0 * fabs(c1) + fabs(c2)) * (sxx + syy);
-  if (!(condition < 1.0e5)) return false;
-  const double r2 = -2.0 * static_cast<double>(threshold) * (1.0 + 1.0e-6 * condition) * 1.001 + 1.0e-6;
-  half_x = static_cast<float>(sqrt(r2 * sxx)) + 1.0e-3f;
-  half_y = static_cast<float>(sqrt(r2 * syy)) + 1.0e-3f;
-  return true;
-}
-
-// Renders one (active camera, tile) work item with the whole workgroup.
-template <int kTileSize, int kBatch, bool kFull, bool kCount>
+// Renders one (active camera, tile) work item with the whole workgroup.  With
+// kCheckpoint the final transmittance and the list position one past the last
+// blended splat are stored per pixel for the backward pass.
+template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint>
 __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_t block) {
-  constexpr int kThreads = kTileSize * kTileSize;
-  constexpr int kWaves = kThreads / kWave;
-  constexpr int kRowsPerWave = kWave / kTileSize;
-  constexpr int kPerThread = (kBatch + kThreads - 1) / kThreads;
+  using Lists = WaveLists<kTileSize, kBatch>;
+  constexpr int kThreads = Lists::kThreads;
+  constexpr int kPerThread = Lists::kPerThread;
   static_assert(kThreads % kWave == 0, "tile must be a whole number of waves");
-  __shared__ f32x4 s_geometry[kBatch];
-  __shared__ f32x4 s_shading[kBatch];
-  __shared__ f32x4 s_colour[kBatch];
-  __shared__ uint16_t s_list[kWaves][kBatch];  // per-wave ordered sub-lists of batch slots
-  __shared__ uint32_t s_totals[kPerThread][kWaves][kWaves];  // [slice][source wave][list]
+  __shared__ Lists s;
 
   int active;
   int64_t image_tile;
@@ -161,7 +72,6 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
   const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
   const int lane = threadIdx.x;
   const int wave = lane / kWave;
-  const int wave_lane = lane & (kWave - 1);
   const int pixel_x = tile_x * kTileSize + lane % kTileSize;
   const int pixel_y = tile_y * kTileSize + lane / kTileSize;
   const bool valid_pixel = pixel_x < cam.width && pixel_y < cam.height;
@@ -178,88 +88,31 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
   int best_gaussian = -1;
   bool done = !valid_pixel;
   int evaluations = 0;
+  int last_blended = 0;
 
   const int64_t range_start = args.tile_starts[global_tile];
   const int64_t range_end = args.tile_ends[global_tile];
   if (range_start >= 0 && range_end > range_start) {
     f32x4 next_a[kPerThread] = {}, next_b[kPerThread] = {}, next_c[kPerThread] = {};
-    gather_batch<kThreads, kBatch, kPerThread>(args, range_start, range_end, lane, next_a, next_b, next_c);
+    gather_batch<kThreads, kBatch, kPerThread>(args.sorted_values, args.records, args.visible_capacity, range_start,
+                                               range_end, lane, next_a, next_b, next_c);
     for (int64_t batch_start = range_start; batch_start < range_end; batch_start += kBatch) {
       const int count = static_cast<int>(range_end - batch_start < kBatch ? range_end - batch_start : kBatch);
       __syncthreads();  // previous batch fully consumed
-      uint32_t wave_bits[kPerThread];
-#pragma unroll
-      for (int k = 0; k < kPerThread; ++k) {
-        const int slot = k * kThreads + lane;
-        wave_bits[k] = 0u;
-        if (slot < count) {
-          const StagedSplat staged = stage_record(next_a[k], next_b[k], next_c[k]);
-          s_geometry[slot] = staged.geometry;
-          s_shading[slot] = staged.shading;
-          s_colour[slot] = staged.colour;
-          float half_x, half_y;
-          if (splat_extent(next_a[k], next_b[k], staged.shading.z, half_x, half_y)) {
-            const float mx = next_a[k].x, my = next_a[k].y;
-            const float gap_x = fabsf(mx - fminf(fmaxf(mx, rect_x0), rect_x1));
-            if (gap_x <= half_x) {
-#pragma unroll
-              for (int w = 0; w < kWaves; ++w) {
-                const float y0 = rect_y_base + static_cast<float>(w * kRowsPerWave);
-                const float y1 = y0 + static_cast<float>(kRowsPerWave - 1);
-                const float gap_y = fabsf(my - fminf(fmaxf(my, y0), y1));
-                if (gap_y <= half_y) wave_bits[k] |= 1u << w;
-              }
-            }
-          } else {
-            wave_bits[k] = (1u << kWaves) - 1u;  // no certain box: every wave evaluates it
-          }
-        }
-      }
-      // Ordered compaction of each wave's sub-list.  Slots are ordered by
-      // (slice k, source wave, lane), which is batch-slot order = depth order.
-      const uint64_t below = wave_lane == 0 ? 0ull : (~0ull >> (kWave - wave_lane));
-      uint32_t prefix[kPerThread][kWaves];
-#pragma unroll
-      for (int k = 0; k < kPerThread; ++k) {
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-          const uint64_t vote = __ballot((wave_bits[k] >> w) & 1u);
-          prefix[k][w] = static_cast<uint32_t>(__popcll(vote & below));
-          if (wave_lane == 0) s_totals[k][wave][w] = static_cast<uint32_t>(__popcll(vote));
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < kPerThread; ++k) {
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-          if ((wave_bits[k] >> w) & 1u) {
-            uint32_t position = prefix[k][w];
-            for (int kk = 0; kk < k; ++kk)
-              for (int q = 0; q < kWaves; ++q) position += s_totals[kk][q][w];
-            for (int q = 0; q < wave; ++q) position += s_totals[k][q][w];
-            s_list[w][position] = static_cast<uint16_t>(k * kThreads + lane);
-          }
-        }
-      }
-      __syncthreads();
+      build_wave_lists<kTileSize, kBatch>(s, next_a, next_b, next_c, count, lane, rect_x0, rect_x1, rect_y_base);
       if (batch_start + kBatch < range_end) {
-        gather_batch<kThreads, kBatch, kPerThread>(args, batch_start + kBatch, range_end, lane, next_a, next_b,
-                                                   next_c);
+        gather_batch<kThreads, kBatch, kPerThread>(args.sorted_values, args.records, args.visible_capacity,
+                                                   batch_start + kBatch, range_end, lane, next_a, next_b, next_c);
       }
       if (!done) {
-        uint32_t list_count = 0;
-#pragma unroll
-        for (int k = 0; k < kPerThread; ++k)
-#pragma unroll
-          for (int q = 0; q < kWaves; ++q) list_count += s_totals[k][q][wave];
+        const uint32_t list_count = s.count(wave);
         for (uint32_t i = 0; i < list_count; ++i) {
-          const int item = s_list[wave][i];
+          const int item = s.list[wave][i];
           if (kCount) ++evaluations;
-          const f32x4 g = s_geometry[item];
+          const f32x4 g = s.geometry[item];
           const float delta_x = px - g.x;
           const float delta_y = py - g.y;
-          const f32x4 sh = s_shading[item];
+          const f32x4 sh = s.shading[item];
           const float power = g.z * delta_x * delta_x + g.w * delta_x * delta_y + sh.x * delta_y * delta_y;
           if (power > 0.0f || power < sh.z) continue;
           const float splat_alpha = fminf(VGR_MAX_ALPHA, sh.y * vgr_expf(power));
@@ -269,7 +122,7 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
             done = true;
             break;
           }
-          const f32x4 col = s_colour[item];
+          const f32x4 col = s.colour[item];
           const float weight = transmittance * splat_alpha;
           red = fmaf(weight, col.x, red);
           green = fmaf(weight, col.y, green);
@@ -281,6 +134,7 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
               best_gaussian = __float_as_int(col.w);
             }
           }
+          if (kCheckpoint) last_blended = static_cast<int>(batch_start - range_start) + item + 1;
           transmittance = next_transmittance;
         }
       }
@@ -313,17 +167,21 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
                           best_gaussian >= 0 && best_gaussian < args.gaussian_count;
     args.semantic[pixel] = labelled ? args.semantic_ids[best_gaussian] : -1;
   }
+  if (kCheckpoint) {
+    args.final_transmittance[pixel] = transmittance;
+    args.last_blended[pixel] = last_blended;
+  }
 }
 
-template <int kTileSize, int kBatch, bool kFull, bool kCount>
+template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint>
 __global__ __launch_bounds__(kTileSize* kTileSize) void composite_kernel(CompositeArgs args) {
-  composite_tile<kTileSize, kBatch, kFull, kCount>(args, blockIdx.x);
+  composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint>(args, blockIdx.x);
 }
 
 // Persistent form: a fixed grid whose workgroups pull (camera, tile) items from
 // a device counter until the queue is empty; every workgroup exits once the
 // counter passes the item count.
-template <int kTileSize, int kBatch, bool kFull, bool kCount>
+template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint>
 __global__ __launch_bounds__(kTileSize* kTileSize) void composite_persistent_kernel(CompositeArgs args) {
   __shared__ int64_t s_item;
   for (;;) {
@@ -332,26 +190,28 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_persistent_ker
     const int64_t item = s_item;
     __syncthreads();
     if (item >= args.total_items) break;
-    composite_tile<kTileSize, kBatch, kFull, kCount>(args, item);
+    composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint>(args, item);
   }
 }
 
-template <int kTileSize, int kBatch, bool kFull, bool kCount>
+template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint = false>
 void launch_variant(const CompositeArgs& args, int64_t blocks, int persistent, hipStream_t stream) {
   const dim3 block(kTileSize * kTileSize);
   if (persistent > 0) {
     const int64_t grid = blocks < persistent ? blocks : persistent;
-    hipLaunchKernelGGL((composite_persistent_kernel<kTileSize, kBatch, kFull, kCount>),
+    hipLaunchKernelGGL((composite_persistent_kernel<kTileSize, kBatch, kFull, kCount, kCheckpoint>),
                        dim3(static_cast<unsigned>(grid)), block, 0, stream, args);
   } else {
-    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, kFull, kCount>), dim3(static_cast<unsigned>(blocks)),
+    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, kFull, kCount, kCheckpoint>), dim3(static_cast<unsigned>(blocks)),
                        block, 0, stream, args);
   }
 }
 
 template <int kTileSize, int kBatch>
 void launch_full_or_rgb(bool full, const CompositeArgs& args, int64_t blocks, int persistent, hipStream_t stream) {
-  if (args.statistics != nullptr) {
+  if (args.final_transmittance != nullptr) {
+    launch_variant<kTileSize, kBatch, true, false, true>(args, blocks, persistent, stream);
+  } else if (args.statistics != nullptr) {
     if (full) {
       launch_variant<kTileSize, kBatch, true, true>(args, blocks, persistent, stream);
     } else {
@@ -406,6 +266,13 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const v
   args.active_count = cameras.active_count;
   args.output_srgb = options.output_srgb;
   args.semantic_min_alpha = options.semantic_min_alpha;
+  args.final_transmittance = outputs.final_transmittance;
+  args.last_blended = outputs.last_blended;
+  if (args.final_transmittance != nullptr &&
+      (args.last_blended == nullptr || options.full_sensor_output == 0 || options.output_srgb != 0)) {
+    set_error("transmittance checkpoints need last_blended, full sensor output and linear RGB");
+    return VGR_E_INVALID_ARGUMENT;
+  }
   int64_t blocks;
   if (cameras.widths == nullptr) {
     const int tiles_x = (options.width + options.tile_size - 1) / options.tile_size;

@@ -93,6 +93,11 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras,
                      const vgr_outputs& outputs, const vgr_options& options,
                      const vgr_workspace& ws, const ScratchLayout& layout,
                      const char* scratch, bool sorted_in_out, hipStream_t stream);
+int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_outputs& outputs,
+                    const vgr_options& options, const vgr_workspace& ws, const ScratchLayout& layout,
+                    const char* scratch, const vgr_gradients& grads, hipStream_t stream);
+int launch_covariance_backward(const float* scales, const float* rotations, const float* grad_covariances,
+                               int64_t count, float* grad_scales, float* grad_rotations, hipStream_t stream);
 int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream);
 int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream);
 int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream);

@@ -16,221 +16,15 @@
 // Each thread owns one Gaussian and loops over the cameras of its group, so the
 // scene attributes are read from HBM once per group instead of once per
 // (camera, Gaussian) pair; camera parameters are wave-uniform scalar loads.
-#include "vgr_internal.h"
+#include "vgr_project_device.h"
 
 namespace vgr {
 namespace {
 
-constexpr float kShC1 = 0.4886025119029199f;
-constexpr float kShC2_0 = 1.0925484305920792f;
-constexpr float kShC2_1 = -1.0925484305920792f;
-constexpr float kShC2_2 = 0.31539156525252005f;
-constexpr float kShC2_3 = -1.0925484305920792f;
-constexpr float kShC2_4 = 0.5462742152960396f;
-constexpr float kShC3_0 = -0.5900435899266435f;
-constexpr float kShC3_1 = 2.890611442640554f;
-constexpr float kShC3_2 = -0.4570457994644658f;
-constexpr float kShC3_3 = 0.3731763325901154f;
-constexpr float kShC3_4 = -0.4570457994644658f;
-constexpr float kShC3_5 = 1.445305721320277f;
-constexpr float kShC3_6 = -0.5900435899266435f;
+using namespace dev;
 
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlockThreads * kScanItems;
-
-__device__ __forceinline__ int float_to_int_clamped(float value) {
-  // ceilf/floorf results of finite splat extents; clamp keeps the int
-  // conversion defined for pathological covariances.
-  if (!(value < 1073741824.0f)) return 1073741824;
-  if (!(value > -1073741824.0f)) return -1073741824;
-  return static_cast<int>(value);
-}
-
-struct Projected {
-  float mean_x, mean_y;
-  float conic_0, conic_1, conic_2;
-  float depth, opacity;
-  int radius_x, radius_y;
-};
-
-// Screen-space projection of one (camera, Gaussian) pair.  Operation order
-// follows renderer_cuda.cu:614-1055 term by term (left-to-right sums, no
-// contraction), with vgr_logf in place of __logf.
-__device__ __forceinline__ bool project_pair(
-    const CameraParams& cam, float m0, float m1, float m2, float opacity,
-    const float* cov, float near_plane, float far_plane, float support_sigma,
-    float covariance_epsilon, bool antialiased, Projected& out) {
-  if (!(opacity >= VGR_ALPHA_THRESHOLD)) return false;
-  const float* e = cam.env;
-  const float env_x = e[0] * m0 + e[1] * m1 + e[2] * m2 + e[3];
-  const float env_y = e[4] * m0 + e[5] * m1 + e[6] * m2 + e[7];
-  const float env_z = e[8] * m0 + e[9] * m1 + e[10] * m2 + e[11];
-  const float* v = cam.view;
-  const float camera_x = v[0] * env_x + v[1] * env_y + v[2] * env_z + v[3];
-  const float camera_y = v[4] * env_x + v[5] * env_y + v[6] * env_z + v[7];
-  const float camera_z = v[8] * env_x + v[9] * env_y + v[10] * env_z + v[11];
-  if (camera_z < near_plane || camera_z > far_plane) return false;
-
-  const float inverse_z = 1.0f / camera_z;
-  const float mean_x = cam.focal_x * camera_x * inverse_z + cam.center_x;
-  const float mean_y = cam.focal_y * camera_y * inverse_z + cam.center_y;
-
-  const float* r = cam.rotation;
-  float rc[9];
-#pragma unroll
-  for (int row = 0; row < 3; ++row) {
-    const float r0 = r[row * 3 + 0];
-    const float r1 = r[row * 3 + 1];
-    const float r2 = r[row * 3 + 2];
-    rc[row * 3 + 0] = r0 * cov[0] + r1 * cov[1] + r2 * cov[2];
-    rc[row * 3 + 1] = r0 * cov[1] + r1 * cov[3] + r2 * cov[4];
-    rc[row * 3 + 2] = r0 * cov[2] + r1 * cov[4] + r2 * cov[5];
-  }
-  const float c00 = rc[0] * r[0] + rc[1] * r[1] + rc[2] * r[2];
-  const float c01 = rc[0] * r[3] + rc[1] * r[4] + rc[2] * r[5];
-  const float c02 = rc[0] * r[6] + rc[1] * r[7] + rc[2] * r[8];
-  const float c11 = rc[3] * r[3] + rc[4] * r[4] + rc[5] * r[5];
-  const float c12 = rc[3] * r[6] + rc[4] * r[7] + rc[5] * r[8];
-  const float c22 = rc[6] * r[6] + rc[7] * r[7] + rc[8] * r[8];
-
-  const float clamped_x = camera_z * fminf(cam.limit_x_pos, fmaxf(-cam.limit_x_neg, camera_x * inverse_z));
-  const float clamped_y = camera_z * fminf(cam.limit_y_pos, fmaxf(-cam.limit_y_neg, camera_y * inverse_z));
-  const float inverse_z_squared = inverse_z * inverse_z;
-  const float j00 = cam.focal_x * inverse_z;
-  const float j02 = -cam.focal_x * clamped_x * inverse_z_squared;
-  const float j11 = cam.focal_y * inverse_z;
-  const float j12 = -cam.focal_y * clamped_y * inverse_z_squared;
-  const float raw00 = j00 * j00 * c00 + 2.0f * j00 * j02 * c02 + j02 * j02 * c22;
-  const float cov01 = j00 * j11 * c01 + j00 * j12 * c02 + j02 * j11 * c12 + j02 * j12 * c22;
-  const float raw11 = j11 * j11 * c11 + 2.0f * j11 * j12 * c12 + j12 * j12 * c22;
-  const float cov00 = raw00 + covariance_epsilon;
-  const float cov11 = raw11 + covariance_epsilon;
-  const float determinant = cov00 * cov11 - cov01 * cov01;
-  if (!(determinant > 0.0f)) return false;
-  if (antialiased) {
-    const float original = raw00 * raw11 - cov01 * cov01;
-    const float compensation = sqrtf(fmaxf(
-        VGR_MIN_ANTIALIAS_COMPENSATION * VGR_MIN_ANTIALIAS_COMPENSATION,
-        original / determinant));
-    opacity *= compensation;
-    if (!(opacity >= VGR_ALPHA_THRESHOLD)) return false;
-  }
-  const float opacity_extend = sqrtf(2.0f * vgr_logf(opacity / VGR_ALPHA_THRESHOLD));
-  const float extend = fminf(support_sigma, opacity_extend);
-  const int radius_x = float_to_int_clamped(ceilf(extend * sqrtf(cov00)));
-  const int radius_y = float_to_int_clamped(ceilf(extend * sqrtf(cov11)));
-  if (radius_x <= 0 && radius_y <= 0) return false;
-  if (mean_x + static_cast<float>(radius_x) <= 0.0f ||
-      mean_y + static_cast<float>(radius_y) <= 0.0f ||
-      mean_x - static_cast<float>(radius_x) >= static_cast<float>(cam.width) ||
-      mean_y - static_cast<float>(radius_y) >= static_cast<float>(cam.height)) {
-    return false;
-  }
-  const float inverse_determinant = 1.0f / determinant;
-  out.mean_x = mean_x;
-  out.mean_y = mean_y;
-  out.conic_0 = cov11 * inverse_determinant;
-  out.conic_1 = -cov01 * inverse_determinant;
-  out.conic_2 = cov00 * inverse_determinant;
-  out.depth = camera_z;
-  out.opacity = fminf(fmaxf(opacity, 0.0f), 1.0f);
-  out.radius_x = radius_x;
-  out.radius_y = radius_y;
-  return true;
-}
-
-struct TileRect {
-  int min_x, min_y, max_x, max_y;
-};
-
-// Tile rectangle of a projected splat (renderer_cuda.cu:1729-1748).
-__device__ __forceinline__ TileRect tile_rect(const Projected& p, int tile_size, int tiles_x, int tiles_y) {
-  const float ts = static_cast<float>(tile_size);
-  TileRect rect;
-  rect.min_x = max(0, static_cast<int>(floorf(p.mean_x / ts - static_cast<float>(p.radius_x) / ts)));
-  rect.min_y = max(0, static_cast<int>(floorf(p.mean_y / ts - static_cast<float>(p.radius_y) / ts)));
-  rect.max_x = min(tiles_x, static_cast<int>(ceilf(p.mean_x / ts + static_cast<float>(p.radius_x) / ts)));
-  rect.max_y = min(tiles_y, static_cast<int>(ceilf(p.mean_y / ts + static_cast<float>(p.radius_y) / ts)));
-  return rect;
-}
-
-__device__ __forceinline__ int rect_tiles(const TileRect& rect) {
-  if (rect.min_x >= rect.max_x || rect.min_y >= rect.max_y) return 0;
-  return (rect.max_x - rect.min_x) * (rect.max_y - rect.min_y);
-}
-
-// View-dependent colour: base RGB plus real-SH bands 1..degree evaluated at the
-// unit direction from the camera centre to the Gaussian (3DGS basis/constants),
-// clamped at zero.  f[3k + c] holds band k, channel c (held in registers).
-template <int kDegree>
-__device__ __forceinline__ void sh_colour(const float (&f)[3 * (kDegree + 1) * (kDegree + 1)], float dx, float dy,
-                                          float dz, float* rgb) {
-  constexpr int kBands = (kDegree + 1) * (kDegree + 1);
-  rgb[0] = f[0];
-  rgb[1] = f[1];
-  rgb[2] = f[2];
-  if (kDegree <= 0) return;
-  const float norm2 = dx * dx + dy * dy + dz * dz;
-  float x = 0.0f, y = 0.0f, z = 0.0f;
-  if (norm2 > 0.0f) {
-    const float inverse_norm = 1.0f / sqrtf(norm2);
-    x = dx * inverse_norm;
-    y = dy * inverse_norm;
-    z = dz * inverse_norm;
-  }
-  float basis[16];
-  basis[1] = -kShC1 * y;
-  basis[2] = kShC1 * z;
-  basis[3] = -kShC1 * x;
-  if (kDegree >= 2) {
-    const float xx = x * x, yy = y * y, zz = z * z;
-    const float xy = x * y, yz = y * z, xz = x * z;
-    basis[4] = kShC2_0 * xy;
-    basis[5] = kShC2_1 * yz;
-    basis[6] = kShC2_2 * (2.0f * zz - xx - yy);
-    basis[7] = kShC2_3 * xz;
-    basis[8] = kShC2_4 * (xx - yy);
-    if (kDegree >= 3) {
-      basis[9] = kShC3_0 * y * (3.0f * xx - yy);
-      basis[10] = kShC3_1 * xy * z;
-      basis[11] = kShC3_2 * y * (4.0f * zz - xx - yy);
-      basis[12] = kShC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
-      basis[13] = kShC3_4 * x * (4.0f * zz - xx - yy);
-      basis[14] = kShC3_5 * z * (xx - yy);
-      basis[15] = kShC3_6 * x * (xx - 3.0f * yy);
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    float acc = 0.0f;
-#pragma unroll
-    for (int k = 1; k < kBands; ++k) acc = fmaf(basis[k], f[3 * k + c], acc);
-    rgb[c] = fmaxf(f[c] + acc, 0.0f);
-  }
-}
-
-// Loads the 3*(degree+1)^2 colour channels of Gaussian g into registers, with
-// 16-byte loads when the row stride allows it.
-template <int kDegree>
-__device__ __forceinline__ void load_features(const float* __restrict__ features, int64_t g, int width,
-                                              float (&f)[3 * (kDegree + 1) * (kDegree + 1)]) {
-  constexpr int kChannels = 3 * (kDegree + 1) * (kDegree + 1);
-  const float* row = features + g * width;
-  if (kChannels % 4 == 0 && (width & 3) == 0 && (reinterpret_cast<uintptr_t>(features) & 15u) == 0) {
-    const float4* row4 = reinterpret_cast<const float4*>(row);
-#pragma unroll
-    for (int i = 0; i < kChannels / 4; ++i) {
-      const float4 v = row4[i];
-      f[4 * i + 0] = v.x;
-      f[4 * i + 1] = v.y;
-      f[4 * i + 2] = v.z;
-      f[4 * i + 3] = v.w;
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < kChannels; ++i) f[i] = row[i];
-  }
-}
 
 __global__ void build_camera_table_kernel(
     const float* __restrict__ viewmats, const float* __restrict__ intrinsics,
@@ -356,71 +150,6 @@ __global__ void build_camera_table_kernel(
   table[a] = p;
 }
 
-struct ProjectionArgs {
-  const float* means;
-  const float* covariances;
-  const float* opacities;
-  const float* features;
-  const CameraParams* table;
-  int64_t gaussian_count;
-  int64_t chunks;
-  int active_count;
-  int cameras_per_block;
-  int feature_width;
-  int sh_degree;
-  int tile_size;
-  int antialiased;
-  float near_plane, far_plane, support_sigma, covariance_epsilon;
-  const int32_t* gaussian_ids;   // registration-order id per packed entry, or null (identity)
-  const int32_t* item_blocks;    // kept (group, chunk) blocks in order, or null (every block)
-  const uint64_t* item_masks;    // cameras of the group that may see the chunk
-  const int64_t* item_count;     // device count of kept blocks
-};
-
-// Resolves the (group, chunk) block and camera mask processed by this workgroup.
-// Returns false for workgroups beyond the kept item count.
-__device__ __forceinline__ bool resolve_block(const ProjectionArgs& args, int64_t& block, uint64_t& mask) {
-  block = blockIdx.x;
-  mask = ~0ull;
-  if (args.item_blocks == nullptr) return true;
-  if (block >= *args.item_count) return false;
-  mask = args.item_masks[block];
-  block = args.item_blocks[block];
-  return true;
-}
-
-// Pair results for one camera of the group; identical in both passes.
-__device__ __forceinline__ bool evaluate_pair(const ProjectionArgs& args, const CameraParams& cam, int64_t g,
-                                              bool loaded, float m0, float m1, float m2, float opacity,
-                                              const float* cov, Projected& p, TileRect& rect, int& tiles) {
-  tiles = 0;
-  if (!loaded || cam.scene_slot < 0 || g < cam.gaussian_begin || g >= cam.gaussian_end) return false;
-  if (!project_pair(cam, m0, m1, m2, opacity, cov, args.near_plane, args.far_plane, args.support_sigma,
-                    args.covariance_epsilon, args.antialiased != 0, p)) {
-    return false;
-  }
-  rect = tile_rect(p, args.tile_size, cam.tiles_x, cam.tiles_y);
-  tiles = rect_tiles(rect);
-  return true;
-}
-
-__device__ __forceinline__ void load_gaussian(const ProjectionArgs& args, int64_t g, bool& loaded, float& m0,
-                                              float& m1, float& m2, float& opacity, float* cov) {
-  loaded = g < args.gaussian_count;
-  m0 = m1 = m2 = opacity = 0.0f;
-  for (int i = 0; i < 6; ++i) cov[i] = 0.0f;
-  if (!loaded) return;
-  m0 = args.means[g * 3 + 0];
-  m1 = args.means[g * 3 + 1];
-  m2 = args.means[g * 3 + 2];
-  opacity = args.opacities[g];
-  for (int i = 0; i < 6; ++i) cov[i] = args.covariances[g * 6 + i];
-}
-
-__device__ __forceinline__ int64_t wave_sum_i64(int64_t value) {
-  for (int offset = 32; offset > 0; offset >>= 1) value += __shfl_xor(value, offset, kWave);
-  return value;
-}
 
 __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(ProjectionArgs args,
                                                                          int64_t* __restrict__ block_counts) {
@@ -470,20 +199,6 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
     }
     block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 0] = v;
     block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 1] = t;
-  }
-}
-
-// Inclusive wave scan of two int32 values.
-__device__ __forceinline__ void wave_inclusive_scan2(int& a, int& b) {
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int offset = 1; offset < kWave; offset <<= 1) {
-    const int ta = __shfl_up(a, offset, kWave);
-    const int tb = __shfl_up(b, offset, kWave);
-    if (lane >= offset) {
-      a += ta;
-      b += tb;
-    }
   }
 }
 
@@ -925,9 +640,8 @@ void scan_pairs(const int64_t* counts, int64_t n, int64_t* bases, int64_t* parti
   }
 }
 
-int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_options& options,
-                      const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, hipStream_t stream) {
-  ProjectionArgs args;
+void make_projection_args(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_options& options,
+                          const ScratchLayout& layout, const char* scratch, ProjectionArgs& args) {
   args.means = scene.means;
   args.covariances = scene.covariances;
   args.opacities = scene.opacities;
@@ -949,6 +663,21 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
   args.item_blocks = nullptr;
   args.item_masks = nullptr;
   args.item_count = nullptr;
+  if (layout.projection_blocks > 0 && layout.culling && scene.chunk_bounds != nullptr) {
+    args.item_blocks = reinterpret_cast<const int32_t*>(scratch + layout.item_blocks);
+    args.item_masks = reinterpret_cast<const uint64_t*>(scratch + layout.item_masks);
+    args.item_count = reinterpret_cast<const int64_t*>(scratch + layout.item_totals);
+  }
+}
+
+int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_options& options,
+                      const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, hipStream_t stream) {
+  ProjectionArgs args;
+  make_projection_args(scene, cameras, options, layout, scratch, args);
+  const bool culled = args.item_blocks != nullptr;
+  args.item_blocks = nullptr;  // the culling kernels below take the full candidate grid
+  args.item_masks = nullptr;
+  args.item_count = nullptr;
 
   int64_t* block_counts = reinterpret_cast<int64_t*>(scratch + layout.block_counts);
   int64_t* block_bases = reinterpret_cast<int64_t*>(scratch + layout.block_bases);
@@ -956,7 +685,7 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
   // Partials live in the tail of block_bases' region, which holds 2*(blocks + scan_blocks) entries.
   int64_t* partials = block_bases + 2 * blocks;
 
-  if (blocks > 0 && layout.culling && scene.chunk_bounds != nullptr) {
+  if (culled) {
     int64_t* cull_flags = reinterpret_cast<int64_t*>(scratch + layout.cull_counts);
     int64_t* cull_positions = reinterpret_cast<int64_t*>(scratch + layout.cull_bases);
     uint64_t* cull_masks = reinterpret_cast<uint64_t*>(scratch + layout.cull_masks);

@@ -1,0 +1,296 @@
+"""Differentiable rendering: ``torch.autograd.Function`` over the gfx950 kernels.
+
+The reference renderer is forward-only (``KERNEL_ARCHITECTURE.md:454-460``);
+this module adds the training path on top of the same forward:
+
+* forward  = ``vgr_precompute_covariances`` + ``vgr_render`` with per-pixel
+  checkpoints (final transmittance, last blended list position);
+* backward = ``vgr_render_backward`` (compositing back to front from the
+  checkpoint, then projection / EWA / SH / pose per visible pair) and
+  ``vgr_covariance_backward`` (scales and quaternions).
+
+Parameters follow the 3DGS convention: means, log-scales, unnormalized WXYZ
+quaternions, opacity logits and SH features; ``exp`` and ``sigmoid`` are
+applied here in torch so autograd chains them.  Camera poses are the
+world->camera ``viewmats`` [B,4,4]; :func:`se3_exp` maps a 6-DoF twist to a
+pose for pose optimization (``viewmats = se3_exp(xi) @ base``).
+
+Every forward reads the device counters once to verify that the visible and
+intersection buffers held the whole batch (and re-renders with grown
+capacity when they did not), so gradients never come from a truncated list.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import torch
+
+from . import native as native_module
+from .native import CAMERA_GRAD_FLOATS, RECORD_FLOATS, RECORD_GRAD_FLOATS, infer_sh_degree
+
+_GROWTH = 1.25
+
+
+def se3_exp(xi: torch.Tensor) -> torch.Tensor:
+    """[B,6] twists (rho, phi) -> [B,4,4] rigid transforms exp(xi^) (Rodrigues, differentiable)."""
+    if xi.ndim != 2 or xi.shape[1] != 6:
+        raise ValueError("se3_exp expects [B,6] twists")
+    rho, phi = xi[:, :3], xi[:, 3:]
+    theta2 = (phi * phi).sum(-1, keepdim=True)
+    small = theta2 < 1e-8
+    theta2_safe = torch.where(small, torch.ones_like(theta2), theta2)
+    theta = torch.sqrt(theta2_safe)
+    a = torch.where(small, 1.0 - theta2 / 6.0, torch.sin(theta) / theta)
+    b = torch.where(small, 0.5 - theta2 / 24.0, (1.0 - torch.cos(theta)) / theta2_safe)
+    c = torch.where(small, 1.0 / 6.0 - theta2 / 120.0, (theta - torch.sin(theta)) / (theta2_safe * theta))
+    zero = torch.zeros_like(phi[:, 0])
+    k = torch.stack((zero, -phi[:, 2], phi[:, 1], phi[:, 2], zero, -phi[:, 0], -phi[:, 1], phi[:, 0], zero),
+                    dim=-1).reshape(-1, 3, 3)
+    eye = torch.eye(3, dtype=xi.dtype, device=xi.device).expand_as(k)
+    kk = k @ k
+    rotation = eye + a[..., None] * k + b[..., None] * kk
+    jacobian = eye + b[..., None] * k + c[..., None] * kk
+    translation = (jacobian @ rho.unsqueeze(-1)).squeeze(-1)
+    top = torch.cat((rotation, translation.unsqueeze(-1)), dim=-1)
+    bottom = torch.zeros((xi.shape[0], 1, 4), dtype=xi.dtype, device=xi.device)
+    bottom[:, 0, 3] = 1.0
+    return torch.cat((top, bottom), dim=1)
+
+
+@dataclass
+class _Workspace:
+    records: torch.Tensor
+    keys_in: torch.Tensor
+    keys_out: torch.Tensor
+    values_in: torch.Tensor
+    values_out: torch.Tensor
+    tile_starts: torch.Tensor
+    tile_ends: torch.Tensor
+    counters: torch.Tensor
+    scratch: torch.Tensor
+
+    def as_list(self) -> list[torch.Tensor]:
+        return [self.records, self.keys_in, self.keys_out, self.values_in, self.values_out, self.tile_starts,
+                self.tile_ends, self.counters, self.scratch]
+
+
+class DifferentiableRenderer:
+    """Training-path renderer: same forward as the service backend, plus gradients.
+
+    ``render(...)`` returns rgb [B,H,W,3], depth [B,H,W,1], alpha [B,H,W,1]
+    (differentiable) and semantic_id [B,H,W,1] (int64, not differentiable).
+    Linear RGB only (no sRGB transfer on this path).
+    """
+
+    def __init__(self, *, height: int, width: int, device: torch.device | str | None = None,
+                 near_plane: float = 0.01, far_plane: float = 100.0, gaussian_support_sigma: float = 3.33,
+                 covariance_epsilon: float = 0.3, rasterize_mode: str = "classic", semantic_min_alpha: float = 0.01,
+                 tile_size: int = 16, raster_batch: int = 0, cameras_per_block: int = 0, sort_digit_bits: int = 0,
+                 visible_capacity: int | None = None, intersections_per_visible: float = 4.0,
+                 native: native_module.NativeRenderer | None = None) -> None:
+        if rasterize_mode not in ("classic", "antialiased"):
+            raise ValueError("rasterize_mode must be 'classic' or 'antialiased'.")
+        if tile_size not in (8, 16):
+            raise ValueError("tile_size must be 8 or 16.")
+        if height <= 0 or width <= 0:
+            raise ValueError("height and width must be positive.")
+        self.height = int(height)
+        self.width = int(width)
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.near_plane = float(near_plane)
+        self.far_plane = float(far_plane)
+        self.gaussian_support_sigma = float(gaussian_support_sigma)
+        self.covariance_epsilon = float(covariance_epsilon)
+        self.antialiased = rasterize_mode == "antialiased"
+        self.semantic_min_alpha = float(semantic_min_alpha)
+        self.tile_size = int(tile_size)
+        self.raster_batch = int(raster_batch)
+        self.cameras_per_block = int(cameras_per_block)
+        self.sort_digit_bits = int(sort_digit_bits)
+        self.visible_capacity = visible_capacity
+        self.intersections_per_visible = float(intersections_per_visible)
+        self.intersection_capacity: int | None = None
+        self.native = native if native is not None else native_module.load_native_module()
+        self.last_counters: dict[str, int] | None = None
+        self.check_capacity = True
+
+    # -- workspace ------------------------------------------------------------------------
+    def _tiles(self, batch: int) -> int:
+        ts = self.tile_size
+        return batch * ((self.width + ts - 1) // ts) * ((self.height + ts - 1) // ts)
+
+    def _workspace(self, batch: int, gaussians: int) -> _Workspace:
+        if self.visible_capacity is None:
+            self.visible_capacity = max(1, batch * gaussians)
+        if self.intersection_capacity is None:
+            self.intersection_capacity = max(1, int(math.ceil(self.visible_capacity * self.intersections_per_visible)))
+        visible = min(int(self.visible_capacity), 0x7FFFFFFF)
+        intersections = min(int(self.intersection_capacity), (1 << 30) - 1)
+        device = self.device
+        scratch_bytes = self.native.scratch_bytes(
+            max_views=batch, height=self.height, width=self.width, gaussians=gaussians,
+            intersection_capacity=intersections, tile_size=self.tile_size, near_plane=self.near_plane,
+            far_plane=self.far_plane, sort_digit_bits=self.sort_digit_bits, cameras_per_block=self.cameras_per_block)
+        tiles = self._tiles(batch)
+        return _Workspace(
+            records=torch.empty((visible, RECORD_FLOATS), dtype=torch.float32, device=device),
+            keys_in=torch.empty(intersections, dtype=torch.uint64, device=device),
+            keys_out=torch.empty(intersections, dtype=torch.uint64, device=device),
+            values_in=torch.empty(intersections, dtype=torch.int32, device=device),
+            values_out=torch.empty(intersections, dtype=torch.int32, device=device),
+            tile_starts=torch.empty(tiles, dtype=torch.int32, device=device),
+            tile_ends=torch.empty(tiles, dtype=torch.int32, device=device),
+            counters=torch.zeros(5, dtype=torch.int64, device=device),
+            scratch=torch.empty(scratch_bytes, dtype=torch.uint8, device=device))
+
+    # -- forward ---------------------------------------------------------------------------
+    def _forward(self, means, scales, quats, opacities, features, viewmats, intrinsics, env_xforms, semantic_ids,
+                 sh_degree):
+        batch = int(viewmats.shape[0])
+        gaussians = int(means.shape[0])
+        device = self.device
+        covariances = torch.empty((gaussians, 6), dtype=torch.float32, device=device)
+        self.native.precompute_covariances(scales, quats, covariances)
+        pixels = batch * self.height * self.width
+        outputs = {
+            "rgb": torch.empty((batch, self.height, self.width, 3), dtype=torch.float32, device=device),
+            "depth": torch.empty((batch, self.height, self.width, 1), dtype=torch.float32, device=device),
+            "alpha": torch.empty((batch, self.height, self.width, 1), dtype=torch.float32, device=device),
+            "semantic_id": torch.empty((batch, self.height, self.width, 1), dtype=torch.int64, device=device),
+        }
+        transmittance = torch.empty(pixels, dtype=torch.float32, device=device)
+        last_blended = torch.empty(pixels, dtype=torch.int32, device=device)
+        registered = torch.zeros(1, dtype=torch.int64, device=device)
+        offsets = torch.tensor([0, gaussians], dtype=torch.int64, device=device)
+        scene_ids = torch.zeros(batch, dtype=torch.int64, device=device)
+        active = torch.empty(0, dtype=torch.int64, device=device)
+        scene = [means, covariances, opacities, features, semantic_ids, registered, offsets]
+        cameras = [viewmats, intrinsics, env_xforms, scene_ids, active]
+        for _ in range(4):
+            workspace = self._workspace(batch, gaussians)
+            self.native.render(
+                scene, cameras, [outputs["rgb"], outputs["depth"], outputs["alpha"], outputs["semantic_id"]],
+                workspace.as_list(), height=self.height, width=self.width, max_scene_gaussians=gaussians,
+                near_plane=self.near_plane, far_plane=self.far_plane,
+                gaussian_support_sigma=self.gaussian_support_sigma, covariance_epsilon=self.covariance_epsilon,
+                antialiased=self.antialiased, semantic_min_alpha=self.semantic_min_alpha,
+                ray_gaussian_evaluation=False, tile_size=self.tile_size, depth_bucket_count=0,
+                depth_bucket_group_size=0, compact_projection_cache=False, materialize_projected_records=False,
+                reuse_projection=False, output_srgb=False, deterministic=True, full_sensor_output=True,
+                fixed_capacity_sort=False, sh_degree=sh_degree, sort_digit_bits=self.sort_digit_bits,
+                raster_batch=self.raster_batch, cameras_per_block=self.cameras_per_block,
+                checkpoints=(transmittance, last_blended))
+            call = self.native.last_call
+            if not self.check_capacity:
+                break
+            visible, intersections = (int(v) for v in workspace.counters[:2].tolist())
+            self.last_counters = {"visible_gaussians": visible, "tile_intersections": intersections}
+            grow = False
+            if visible > workspace.records.shape[0]:
+                self.visible_capacity = int(math.ceil(visible * _GROWTH))
+                self.intersection_capacity = max(int(self.intersection_capacity or 0),
+                                                 int(math.ceil(intersections * _GROWTH)))
+                grow = True
+            if intersections > workspace.keys_in.numel():
+                self.intersection_capacity = int(math.ceil(intersections * _GROWTH))
+                grow = True
+            if not grow:
+                break
+            if self.intersection_capacity >= (1 << 30) or self.visible_capacity > 0x7FFFFFFF:
+                raise RuntimeError(f"batch needs {visible} visible records and {intersections} intersections, "
+                                   "beyond the int32 workspace limits; render fewer cameras per call.")
+        else:
+            raise RuntimeError("capacity growth did not converge")
+        state = {"call": call, "workspace": workspace, "covariances": covariances, "transmittance": transmittance,
+                 "last_blended": last_blended, "scene": scene, "cameras": cameras, "outputs": outputs}
+        return outputs, state
+
+    def render(self, means: torch.Tensor, log_scales: torch.Tensor, quats: torch.Tensor,
+               opacity_logits: torch.Tensor, features: torch.Tensor, viewmats: torch.Tensor,
+               intrinsics: torch.Tensor, *, semantic_ids: torch.Tensor | None = None, sh_degree: int | None = None,
+               env_xforms: torch.Tensor | None = None) -> dict[str, torch.Tensor]:
+        for name, tensor in (("means", means), ("log_scales", log_scales), ("quats", quats),
+                             ("opacity_logits", opacity_logits), ("features", features), ("viewmats", viewmats),
+                             ("intrinsics", intrinsics)):
+            if tensor.device != self.device or tensor.dtype != torch.float32:
+                raise ValueError(f"{name} must be float32 on {self.device}")
+        g = int(means.shape[0])
+        if means.shape != (g, 3) or log_scales.shape != (g, 3) or quats.shape != (g, 4) or opacity_logits.shape != (g,):
+            raise ValueError("means [G,3], log_scales [G,3], quats [G,4], opacity_logits [G] expected")
+        if features.ndim != 2 or features.shape[0] != g:
+            raise ValueError("features must be [G,F]")
+        batch = int(viewmats.shape[0])
+        if viewmats.shape != (batch, 4, 4) or intrinsics.shape != (batch, 3, 3):
+            raise ValueError("viewmats [B,4,4] and intrinsics [B,3,3] expected")
+        degree = infer_sh_degree(int(features.shape[1])) if sh_degree is None else int(sh_degree)
+        if semantic_ids is None:
+            semantic_ids = torch.zeros(g, dtype=torch.int64, device=self.device)
+        if env_xforms is None:
+            env_xforms = torch.eye(4, dtype=torch.float32, device=self.device).expand(batch, 4, 4).contiguous()
+        scales = torch.exp(log_scales)
+        opacities = torch.sigmoid(opacity_logits)
+        rgb, depth, alpha, semantic = _RasterizeFunction.apply(
+            means, scales, quats, opacities, features, viewmats, intrinsics, env_xforms, semantic_ids, self, degree)
+        return {"rgb": rgb, "depth": depth, "alpha": alpha, "semantic_id": semantic}
+
+
+class _RasterizeFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means, scales, quats, opacities, features, viewmats, intrinsics, env_xforms, semantic_ids,
+                renderer: DifferentiableRenderer, sh_degree: int):
+        contiguous = [t.contiguous() for t in (means, scales, quats, opacities, features, viewmats, intrinsics,
+                                               env_xforms)]
+        means_c, scales_c, quats_c, opacities_c, features_c, viewmats_c, intrinsics_c, env_c = contiguous
+        outputs, state = renderer._forward(means_c, scales_c, quats_c, opacities_c, features_c, viewmats_c,
+                                           intrinsics_c, env_c, semantic_ids.contiguous(), sh_degree)
+        ctx.renderer = renderer
+        ctx.state = state
+        ctx.inputs = (scales_c, quats_c)
+        ctx.pose = bool(ctx.needs_input_grad[5])
+        ctx.scene_grads = any(ctx.needs_input_grad[:5])
+        ctx.mark_non_differentiable(outputs["semantic_id"])
+        return outputs["rgb"], outputs["depth"], outputs["alpha"], outputs["semantic_id"]
+
+    @staticmethod
+    def backward(ctx, grad_rgb, grad_depth, grad_alpha, grad_semantic):
+        renderer: DifferentiableRenderer = ctx.renderer
+        state = ctx.state
+        device = renderer.device
+        outputs = state["outputs"]
+        if grad_rgb is None:
+            grad_rgb = torch.zeros_like(outputs["rgb"])
+        grad_rgb = grad_rgb.contiguous().float()
+        grad_depth = grad_depth.contiguous().float() if grad_depth is not None else None
+        grad_alpha = grad_alpha.contiguous().float() if grad_alpha is not None else None
+        if grad_depth is not None:
+            # depth is +inf where nothing was blended; its gradient there carries no information.
+            grad_depth = torch.where(torch.isfinite(outputs["depth"]), grad_depth, torch.zeros_like(grad_depth))
+        means, covariances, opacities, features = (state["scene"][0], state["covariances"], state["scene"][2],
+                                                   state["scene"][3])
+        workspace: _Workspace = state["workspace"]
+        record_grads = torch.empty((workspace.records.shape[0], RECORD_GRAD_FLOATS), dtype=torch.float32,
+                                   device=device)
+        scene_grads = None
+        if ctx.scene_grads:
+            scene_grads = (torch.zeros_like(means), torch.zeros_like(covariances), torch.zeros_like(opacities),
+                           torch.zeros_like(features))
+        grad_viewmats = camera_grads = None
+        batch = int(state["cameras"][0].shape[0])
+        if ctx.pose:
+            grad_viewmats = torch.zeros((batch, 4, 4), dtype=torch.float32, device=device)
+            camera_grads = torch.empty((batch, CAMERA_GRAD_FLOATS), dtype=torch.float32, device=device)
+        renderer.native.render_backward(state["call"], grad_rgb, grad_depth, grad_alpha, record_grads,
+                                        scene_grads=scene_grads, grad_viewmats=grad_viewmats,
+                                        camera_grads=camera_grads)
+        grad_means = grad_scales = grad_quats = grad_opacities = grad_features = None
+        if scene_grads is not None:
+            grad_means, grad_covariances, grad_opacities, grad_features = scene_grads
+            scales, quats = ctx.inputs
+            grad_scales = torch.empty_like(scales)
+            grad_quats = torch.empty_like(quats)
+            renderer.native.covariance_backward(scales, quats, grad_covariances, grad_scales, grad_quats)
+        ctx.state = None
+        return (grad_means, grad_scales, grad_quats, grad_opacities, grad_features, grad_viewmats, None, None, None,
+                None, None)

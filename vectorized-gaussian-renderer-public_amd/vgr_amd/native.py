@@ -42,6 +42,9 @@ WORKSPACE_TENSORS = (
     "scratch",
 )
 RECORD_FLOATS = 12  # 48-byte visible record, see vgr_record in include/vgr.h
+RECORD_GRAD_FLOATS = 10  # dL/d (mean_x, mean_y, conic_0..2, opacity, depth, r, g, b) per visible record
+CAMERA_GRAD_FLOATS = 24  # per-camera pose partials of the backward pass
+ABI_VERSION = 2
 
 
 class _Scene(ctypes.Structure):
@@ -86,6 +89,24 @@ class _Outputs(ctypes.Structure):
         ("depth", ctypes.c_void_p),
         ("alpha", ctypes.c_void_p),
         ("semantic", ctypes.c_void_p),
+        ("final_transmittance", ctypes.c_void_p),
+        ("last_blended", ctypes.c_void_p),
+    ]
+
+
+class _Gradients(ctypes.Structure):
+    _fields_ = [
+        ("grad_rgb", ctypes.c_void_p),
+        ("grad_depth", ctypes.c_void_p),
+        ("grad_alpha", ctypes.c_void_p),
+        ("record_grads", ctypes.c_void_p),
+        ("grad_means", ctypes.c_void_p),
+        ("grad_covariances", ctypes.c_void_p),
+        ("grad_opacities", ctypes.c_void_p),
+        ("grad_features", ctypes.c_void_p),
+        ("grad_viewmats", ctypes.c_void_p),
+        ("camera_grads", ctypes.c_void_p),
+        ("sorted_selector", ctypes.c_int64),
     ]
 
 
@@ -163,6 +184,12 @@ EXPORTED_SYMBOLS: dict[str, tuple[type, tuple]] = {
         (ctypes.POINTER(_Scene), ctypes.POINTER(_Cameras), ctypes.POINTER(_Outputs), ctypes.POINTER(_Workspace),
          ctypes.POINTER(_Options), _P, ctypes.POINTER(_I64)),
     ),
+    "vgr_render_backward": (
+        _I32,
+        (ctypes.POINTER(_Scene), ctypes.POINTER(_Cameras), ctypes.POINTER(_Outputs), ctypes.POINTER(_Workspace),
+         ctypes.POINTER(_Options), ctypes.POINTER(_Gradients), _P),
+    ),
+    "vgr_covariance_backward": (_I32, (_P, _P, _P, _I64, _P, _P, _P)),
     "vgr_prepare_chunked_active_ids": (_I32, (_P, _I64, _P, _P, _I64, _I64, _P)),
     "vgr_aggregate_chunk_counters": (_I32, (_P, _I64, _I64, _P, _P, _P)),
     "vgr_fill_i64": (_I32, (_P, _I64, _I64, _P)),
@@ -220,8 +247,9 @@ class NativeRenderer:
     def __init__(self, library: ctypes.CDLL | None = None) -> None:
         self.lib = library if library is not None else load_library()
         version = int(self.lib.vgr_abi_version())
-        if version != 1:
-            raise RuntimeError(f"libvgr_hip ABI version {version} is not supported by this binding (expected 1).")
+        if version != ABI_VERSION:
+            raise RuntimeError(f"libvgr_hip ABI version {version} is not supported by this binding "
+                               f"(expected {ABI_VERSION}).")
 
     # -- helpers --------------------------------------------------------------------------
     def _check(self, status: int, what: str) -> None:
@@ -347,12 +375,15 @@ class NativeRenderer:
         resolutions: tuple[torch.Tensor, torch.Tensor, torch.Tensor, int] | None = None,
         stage_events: Sequence[torch.cuda.Event] | None = None,
         statistics: torch.Tensor | None = None,
+        checkpoints: tuple[torch.Tensor, torch.Tensor] | None = None,
     ) -> int:
         """One render submission; returns the sorted-buffer selector (1: *_out holds the result).
 
         ``stage_events`` (5 recorded torch events) are re-recorded at the stage
         boundaries on the render stream; ``statistics`` (int64[4] on the device)
-        receives the compositor's evaluation count.
+        receives the compositor's evaluation count.  ``checkpoints`` (float32
+        final transmittance and int32 last-blended position, one per pixel) makes
+        the forward store what :meth:`render_backward` needs.
         """
         if ray_gaussian_evaluation:
             raise ValueError("ray_gaussian_evaluation is not available on the gfx950 tile path.")
@@ -419,6 +450,16 @@ class NativeRenderer:
         c_outputs = _Outputs(rgb=_ptr(rgb), depth=_ptr(depth) if full_sensor_output else None,
                              alpha=_ptr(alpha) if full_sensor_output else None,
                              semantic=_ptr(semantic) if full_sensor_output else None)
+        if checkpoints is not None:
+            transmittance, last_blended = checkpoints
+            _require_dtypes((transmittance, torch.float32, "final_transmittance"),
+                            (last_blended, torch.int32, "last_blended"))
+            if transmittance.numel() * 3 != rgb.numel() or last_blended.numel() * 3 != rgb.numel():
+                raise ValueError("checkpoints need one value per output pixel")
+            if transmittance.device != device or last_blended.device != device:
+                raise ValueError("checkpoints must be on the render device")
+            c_outputs.final_transmittance = _ptr(transmittance)
+            c_outputs.last_blended = _ptr(last_blended)
         c_workspace = _Workspace(
             records=_ptr(records), radii=None, keys_in=_ptr(keys_in), keys_out=_ptr(keys_out),
             values_in=_ptr(values_in), values_out=_ptr(values_out), tile_starts=_ptr(tile_starts),
@@ -451,7 +492,88 @@ class NativeRenderer:
             self._check(self.lib.vgr_render(ctypes.byref(c_scene), ctypes.byref(c_cameras), ctypes.byref(c_outputs),
                                             ctypes.byref(c_workspace), ctypes.byref(c_options), _stream(device),
                                             ctypes.byref(selector)), "vgr_render")
+        self.last_call = RenderCall(c_scene, c_cameras, c_outputs, c_workspace, c_options, int(selector.value),
+                                    device, event_array)
         return int(selector.value)
+
+    def render_backward(self, call: "RenderCall", grad_rgb: torch.Tensor, grad_depth: torch.Tensor | None,
+                        grad_alpha: torch.Tensor | None, record_grads: torch.Tensor, *,
+                        scene_grads: tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor] | None,
+                        grad_viewmats: torch.Tensor | None = None,
+                        camera_grads: torch.Tensor | None = None) -> None:
+        """Backward of the checkpointed forward ``call`` (its tensors untouched since).
+
+        ``scene_grads`` = (means [G,3], covariances [G,6], opacities [G],
+        features [G,F]) float32, zero-filled by the caller and added to.
+        ``grad_viewmats`` [B,4,4] float32 (zeroed, added to) with ``camera_grads``
+        workspace [active, 24] requests camera-pose gradients.
+        """
+        if call.outputs.final_transmittance is None:
+            raise ValueError("render_backward needs a forward rendered with checkpoints")
+        device = call.device
+        for name, tensor in (("grad_rgb", grad_rgb), ("grad_depth", grad_depth), ("grad_alpha", grad_alpha),
+                             ("record_grads", record_grads), ("grad_viewmats", grad_viewmats),
+                             ("camera_grads", camera_grads)):
+            if tensor is None:
+                continue
+            if tensor.dtype != torch.float32 or tensor.device != device or not tensor.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous float32 tensor on {device}")
+        if record_grads.numel() < call.workspace.visible_capacity * RECORD_GRAD_FLOATS:
+            raise ValueError("record_grads must hold 10 floats per visible record")
+        gradients = _Gradients(grad_rgb=_ptr(grad_rgb), grad_depth=_ptr(grad_depth), grad_alpha=_ptr(grad_alpha),
+                               record_grads=_ptr(record_grads), sorted_selector=call.selector)
+        if scene_grads is not None:
+            means, covariances, opacities, features = scene_grads
+            for tensor in scene_grads:
+                if tensor.dtype != torch.float32 or tensor.device != device or not tensor.is_contiguous():
+                    raise ValueError("scene gradients must be contiguous float32 tensors on the render device")
+            if (means.numel() != 3 * call.scene.gaussian_count or covariances.numel() != 6 * call.scene.gaussian_count
+                    or opacities.numel() != call.scene.gaussian_count
+                    or features.numel() != call.scene.feature_width * call.scene.gaussian_count):
+                raise ValueError("scene gradient shapes must match the scene tensors")
+            gradients.grad_means = _ptr(means)
+            gradients.grad_covariances = _ptr(covariances)
+            gradients.grad_opacities = _ptr(opacities)
+            gradients.grad_features = _ptr(features)
+        if grad_viewmats is not None:
+            if camera_grads is None or camera_grads.numel() < call.cameras.active_count * CAMERA_GRAD_FLOATS:
+                raise ValueError("pose gradients need camera_grads workspace of 24 floats per active camera")
+            if grad_viewmats.numel() < 16 * call.cameras.batch:
+                raise ValueError("grad_viewmats must be [B,4,4]")
+            gradients.grad_viewmats = _ptr(grad_viewmats)
+            gradients.camera_grads = _ptr(camera_grads)
+        with torch.cuda.device(device):
+            self._check(self.lib.vgr_render_backward(
+                ctypes.byref(call.scene), ctypes.byref(call.cameras), ctypes.byref(call.outputs),
+                ctypes.byref(call.workspace), ctypes.byref(call.options), ctypes.byref(gradients), _stream(device)),
+                "vgr_render_backward")
+
+    def covariance_backward(self, scales: torch.Tensor, rotations: torch.Tensor, grad_covariances: torch.Tensor,
+                            grad_scales: torch.Tensor, grad_rotations: torch.Tensor) -> None:
+        count = int(scales.shape[0])
+        if count == 0:
+            return
+        with torch.cuda.device(scales.device):
+            self._check(self.lib.vgr_covariance_backward(_ptr(scales), _ptr(rotations), _ptr(grad_covariances), count,
+                                                         _ptr(grad_scales), _ptr(grad_rotations),
+                                                         _stream(scales.device)), "vgr_covariance_backward")
+
+
+class RenderCall:
+    """The C structs of one vgr_render call, kept for its backward pass."""
+
+    def __init__(self, scene, cameras, outputs, workspace, options, selector, device, event_array) -> None:
+        self.scene = scene
+        self.cameras = cameras
+        self.outputs = outputs
+        self.workspace = workspace
+        self.options = options
+        self.selector = selector
+        self.device = device
+        self.options.stage_events = None
+        self.options.stage_event_count = 0
+        self.options.statistics = None
+        del event_array
 
 
 def _require_dtypes(*items: tuple[torch.Tensor, torch.dtype, str]) -> None:
