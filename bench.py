@@ -46,6 +46,10 @@ def parse_args() -> argparse.Namespace:
     parser.add_argument("--raster-batch", type=int, default=0)
     parser.add_argument("--cameras-per-block", type=int, default=0)
     parser.add_argument("--sort-tile-items", type=int, default=0)
+    parser.add_argument("--workloads", default="c4,c5,c3",
+                        help="extra workloads reported beside the main line: c4 (fwd+bwd), c5 (robotics batch, "
+                             "batched-persistent vs plain, pose backward), c3 (uneven multi-scene batch); '' for none")
+    parser.add_argument("--extra-steps", type=int, default=10, help="timed steps of each extra workload")
     parser.add_argument("--copy-calibration-mib", type=int, default=0,
                         help="also run one device-to-device copy of this size (PMC byte-counter calibration)")
     return parser.parse_args()
@@ -199,6 +203,19 @@ def main() -> int:
                               and parity["depth_max_rel"] <= 1e-4 and parity["depth_background_equal"]
                               and parity["semantic_mismatches"] == 0 and spin_overflows == 0)
 
+    del outputs, backend, service
+    torch.cuda.empty_cache()
+    extras = {}
+    for name in [w for w in args.workloads.split(",") if w]:
+        runner = {"c4": bench_c4_fwd_bwd, "c5": bench_c5_robotics, "c3": bench_c3_uneven}[name]
+        print(f"[bench] workload {name}", file=sys.stderr, flush=True)
+        try:
+            extras[name] = runner(args, torch, device)
+        except (RuntimeError, ValueError) as error:  # reported, never hides the main line
+            extras[name] = {"error": f"{type(error).__name__}: {error}"[:500]}
+        print(f"[bench] workload {name}: {json.dumps(extras[name])}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
     line = {
         "metric": "Mpix/s fwd and fwd+bwd, batched-persistent vs plain, 1 MI355X, at tolerance",
         "value": round(value, 2),
@@ -219,9 +236,199 @@ def main() -> int:
         "stages": stages,
         "cpu_baseline": cpu_baseline,
         "parity": parity,
+        "workloads": extras,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def timed(torch, fn, steps: int, warmup: int) -> tuple[float, float]:
+    """(mean GPU ms per call from events on the current stream, wall ms per call)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    wall = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - wall) * 1000.0 / steps
+    return start.elapsed_time(stop) / steps, wall_ms
+
+
+def bench_c4_fwd_bwd(args, torch, device) -> dict:
+    """16 cameras 512x512 around 500k SH3 Gaussians; L1 + depth-L1 loss against targets
+    rendered from a perturbed copy; one step = forward + loss + backward of every parameter."""
+    from vgr_amd import synthetic
+    from vgr_amd.autograd import DifferentiableRenderer
+
+    seed, size, focal, cameras = 20261204, 512, 443.4, 16
+    scene = synthetic.gaussian_scene(500_000, seed=seed, sh_degree=3)
+    rig = synthetic.orbit_rig(cameras, size, size, focal, seed=seed, radius=(10.0, 15.0))
+    g = torch.Generator().manual_seed(seed + 1)
+    logits = torch.logit(scene.opacities.double()).float()
+    params = [scene.means, torch.log(scene.scales), scene.rotations, logits, scene.features]
+    params = [p.to(device).contiguous() for p in params]
+    target_means = (scene.means + 0.05 * torch.randn(scene.means.shape, generator=g)).to(device)
+    target_logits = (logits + 0.3 * torch.randn(logits.shape, generator=g)).to(device)
+    viewmats = rig.viewmats.to(device)
+    intrinsics = rig.intrinsics.to(device)
+    renderer = DifferentiableRenderer(height=size, width=size, device=device)
+    with torch.no_grad():
+        target = renderer.render(target_means, params[1], params[2], target_logits, params[4], viewmats, intrinsics)
+        target = {k: v.clone() for k, v in target.items()}
+    mask = (target["alpha"] > 0.5) & torch.isfinite(target["depth"])
+    leaves = [p.clone().requires_grad_(True) for p in params]
+
+    def loss_of(out):
+        depth = torch.where(mask, out["depth"], torch.zeros_like(out["depth"]))
+        depth_target = torch.where(mask, target["depth"], torch.zeros_like(target["depth"]))
+        depth_term = (depth - depth_target).abs().sum() / mask.sum().clamp(min=1)
+        return (out["rgb"] - target["rgb"]).abs().mean() + depth_term
+
+    def forward_only():
+        with torch.no_grad():
+            renderer.render(*leaves, viewmats, intrinsics)
+
+    def step():
+        for leaf in leaves:
+            leaf.grad = None
+        loss_of(renderer.render(*leaves, viewmats, intrinsics)).backward()
+
+    fwd_ms, fwd_wall = timed(torch, forward_only, args.extra_steps, 2)
+    step_ms, step_wall = timed(torch, step, args.extra_steps, 2)
+    pixels = cameras * size * size
+    fwd = max(fwd_ms, fwd_wall)
+    both = max(step_ms, step_wall)
+    finite = all(bool(torch.isfinite(leaf.grad).all()) for leaf in leaves)
+    return {"workload": f"{cameras} cameras {size}x{size}, 500k Gaussians SH3 FP32, L1 + depth-L1 loss, grads of "
+                        "means/log-scales/quats/opacity logits/SH",
+            "fwd_ms": round(fwd, 3), "fwd_bwd_ms": round(both, 3), "fwd_bwd_over_fwd": round(both / fwd, 3),
+            "fwd_mpix_s": round(pixels / 1e6 / (fwd / 1e3), 1),
+            "fwd_bwd_mpix_s": round(pixels / 1e6 / (both / 1e3), 1),
+            "counters": renderer.last_counters, "grads_finite": finite,
+            "note": "forward includes one counter read (host sync) that verifies capacity"}
+
+
+def bench_c5_robotics(args, torch, device) -> dict:
+    """1024 cameras 160x120 along 32 smooth trajectories in a 5M-Gaussian SH1 room:
+    batched-persistent (graph replay) vs the plain single submission, plus a pose-only backward."""
+    from vgr_amd import synthetic
+    from vgr_amd.autograd import DifferentiableRenderer
+    from vgr_amd.batched import BatchedRenderer, CameraSpec
+
+    seed, width, height, focal = 20261205, 160, 120, 139.0
+    scene = synthetic.gaussian_scene(5_000_000, seed=seed, sh_degree=1, semantic_classes=256, layout="room")
+    rig = synthetic.trajectory_rig(32, 32, width, height, focal, seed=seed)
+    viewmats = rig.viewmats.to(device)
+    intrinsics = rig.intrinsics.to(device)
+    gpu_scene = scene.to(device)
+    renderer = BatchedRenderer(device=device)
+    renderer.register_scene(0, **gpu_scene.as_kwargs())
+    specs = [CameraSpec(0, width, height)] * rig.batch
+    renderer.plan(specs)
+    counters = renderer.calibrate(viewmats, intrinsics)
+    renderer.capture()
+    batched_ms, batched_wall = timed(torch, renderer.render, args.extra_steps, 2)
+    plain_ms, plain_wall = timed(torch, renderer.render_plain, args.extra_steps, 1)
+    renderer.render()
+    torch.cuda.synchronize()
+    group = renderer.plain_groups[0]
+    flat = renderer.outputs
+    equal = all(torch.equal(flat[name].view(rig.batch, height, width, -1), group["outputs"][name])
+                for name in ("rgb", "depth", "alpha", "semantic_id"))
+    statistics = torch.zeros(4, dtype=torch.int64, device=device)
+    renderer.graph = None
+    renderer._submit(statistics=statistics)
+    torch.cuda.synchronize()
+    kept = int(statistics[2])
+    pixels = rig.batch * width * height
+    batched = max(batched_ms, batched_wall)
+    plain = max(plain_ms, plain_wall)
+    result = {"workload": "1024 cameras 160x120 (fx=fy=139) on 32 Catmull-Rom trajectories in a 5M-Gaussian SH1 "
+                          "room, rgb+depth+alpha+semantic",
+              "batched_graph_ms": round(batched, 3), "plain_ms": round(plain, 3),
+              "batched_mpix_s": round(pixels / 1e6 / (batched / 1e3), 1),
+              "plain_mpix_s": round(pixels / 1e6 / (plain / 1e3), 1),
+              "batched_over_plain": round(plain / batched, 3), "outputs_bitwise_equal": equal,
+              "counters": counters, "kept_cull_items": kept}
+    del renderer
+    torch.cuda.empty_cache()
+    # Pose-only backward on the plain training path.
+    trainer = DifferentiableRenderer(height=height, width=width, device=device, cameras_per_block=16)
+    logits = torch.logit(gpu_scene.opacities.double()).float()
+    frozen = [gpu_scene.means, torch.log(gpu_scene.scales), gpu_scene.rotations, logits, gpu_scene.features]
+    xi = torch.zeros((rig.batch, 6), device=device, requires_grad=True)
+    from vgr_amd.autograd import se3_exp
+
+    def pose_step():
+        xi.grad = None
+        views = (se3_exp(xi) @ viewmats).contiguous()
+        out = trainer.render(*frozen, views, intrinsics, semantic_ids=gpu_scene.semantic_ids)
+        (out["rgb"].mean() + out["alpha"].mean()).backward()
+
+    def forward_only():
+        with torch.no_grad():
+            trainer.render(*frozen, viewmats, intrinsics, semantic_ids=gpu_scene.semantic_ids)
+
+    fwd_ms, fwd_wall = timed(torch, forward_only, args.extra_steps, 1)
+    pose_ms, pose_wall = timed(torch, pose_step, args.extra_steps, 1)
+    result.update({"pose_fwd_ms": round(max(fwd_ms, fwd_wall), 3),
+                   "pose_fwd_bwd_ms": round(max(pose_ms, pose_wall), 3),
+                   "pose_fwd_bwd_over_fwd": round(max(pose_ms, pose_wall) / max(fwd_ms, fwd_wall), 3),
+                   "pose_grads_finite": bool(torch.isfinite(xi.grad).all())})
+    return result
+
+
+def bench_c3_uneven(args, torch, device) -> dict:
+    """256 cameras with resolutions from {128^2, 320x240, 640x480, 1280x720} over 4 scenes
+    (200k/500k/1M/2M SH3), round-robin: batched-persistent graph replay vs plain per-group path."""
+    from vgr_amd import synthetic
+    from vgr_amd.batched import BatchedRenderer, CameraSpec
+
+    seed = 20261203
+    sizes = (200_000, 500_000, 1_000_000, 2_000_000)
+    resolutions = ((128, 128), (320, 240), (640, 480), (1280, 720))
+    g = torch.Generator().manual_seed(seed)
+    choice = torch.randint(0, 4, (256,), generator=g).tolist()
+    renderer = BatchedRenderer(device=device)
+    for k, count in enumerate(sizes):
+        renderer.register_scene(k, **synthetic.gaussian_scene(count, seed=seed + k, sh_degree=3).to(device).as_kwargs())
+    specs, views, ks = [], [], []
+    rig = synthetic.orbit_rig(256, 1, 1, 1.0, seed=seed, radius=(10.0, 15.0))
+    for i in range(256):
+        w, h = resolutions[choice[i]]
+        specs.append(CameraSpec(i % 4, w, h))
+        views.append(rig.viewmats[i])
+        ks.append(synthetic.pinhole(1, w, h, (w / 2) / math.tan(math.radians(30)))[0])
+    viewmats = torch.stack(views).to(device)
+    intrinsics = torch.stack(ks).to(device)
+    renderer.plan(specs)
+    counters = renderer.calibrate(viewmats, intrinsics)
+    renderer.capture()
+    batched_ms, batched_wall = timed(torch, renderer.render, max(3, args.extra_steps // 2), 1)
+    plain_ms, plain_wall = timed(torch, renderer.render_plain, max(3, args.extra_steps // 2), 1)
+    renderer.render()
+    torch.cuda.synchronize()
+    equal = True
+    for group in renderer.plain_groups:
+        for local, camera in enumerate(group["index_list"]):
+            view = renderer.camera_outputs(camera)
+            for name in ("rgb", "depth", "alpha", "semantic_id"):
+                equal = equal and torch.equal(view[name], group["outputs"][name][local])
+    pixels = sum(s.width * s.height for s in specs)
+    batched = max(batched_ms, batched_wall)
+    plain = max(plain_ms, plain_wall)
+    return {"workload": "256 cameras, resolutions U{128^2, 320x240, 640x480, 1280x720}, 4 scenes 200k/500k/1M/2M "
+                        "SH3 round-robin",
+            "batched_graph_ms": round(batched, 3), "plain_ms": round(plain, 3),
+            "batched_mpix_s": round(pixels / 1e6 / (batched / 1e3), 1),
+            "plain_mpix_s": round(pixels / 1e6 / (plain / 1e3), 1),
+            "batched_over_plain": round(plain / batched, 3), "outputs_bitwise_equal": equal, "counters": counters,
+            "plain_groups": len(renderer.plain_groups)}
 
 
 if __name__ == "__main__":

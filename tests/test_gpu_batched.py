@@ -145,3 +145,25 @@ def test_spatial_order_keeps_gaussian_id_tie_order():
                              gaussian_support_sigma=3.0)
     for name in ("rgb", "alpha", "depth", "semantic_id"):
         np.testing.assert_array_equal(out[name].reshape(expected[name].shape), expected[name], err_msg=name)
+
+
+def test_autotuned_plan_matches_oracle(tmp_path):
+    from vgr_amd.autotune import Autotuner, tune_batched
+
+    scene = synthetic.gaussian_scene(30_000, seed=131, sh_degree=2)
+    resolutions = [(160, 120)] * 3 + [(96, 64)] * 2
+    specs = [CameraSpec(0, w, h) for w, h in resolutions]
+    views, k = mixed_rig(132, resolutions)
+    renderer = build([(0, scene)], specs)
+    tuner = Autotuner({"tile_size": (16, 8), "raster_batch": (256, 64), "sort_digit_bits": (8, 6),
+                       "cameras_per_block": (8, 2)}, cache_path=tmp_path / "tune.json", repeats=1)
+    best = tune_batched(renderer, views.to(device()), k.to(device()), tuner=tuner, force=True)
+    print(f"[batched] autotuned {best} from {len(tuner.history)} configurations", flush=True)
+    assert (tmp_path / "tune.json").is_file()
+    out = flat_numpy(renderer.render())
+    expected = oracle.render([scene.as_kwargs()], [0], views, k, [0] * len(specs), height=1, width=1,
+                             resolutions=resolutions, sh_degree=2, tile_size=best["tile_size"])
+    report = compare_images(out, expected)
+    print(f"[batched] tuned vs oracle: {report}", flush=True)
+    for name in ("rgb", "alpha", "depth", "semantic_id"):
+        np.testing.assert_array_equal(out[name].reshape(expected[name].shape), expected[name], err_msg=name)

@@ -123,7 +123,8 @@ class DifferentiableRenderer:
 
     def _workspace(self, batch: int, gaussians: int) -> _Workspace:
         if self.visible_capacity is None:
-            self.visible_capacity = max(1, batch * gaussians)
+            # First guess; the counter check after the render grows it to the demand.
+            self.visible_capacity = max(1, min(batch * gaussians, 1 << 25))
         if self.intersection_capacity is None:
             self.intersection_capacity = max(1, int(math.ceil(self.visible_capacity * self.intersections_per_visible)))
         visible = min(int(self.visible_capacity), 0x7FFFFFFF)

@@ -114,6 +114,20 @@ class BatchedRenderer:
         self.graph: torch.cuda.CUDAGraph | None = None
         self.workspace: dict[str, torch.Tensor] = {}
 
+    def configure(self, *, tile_size: int | None = None, raster_batch: int | None = None,
+                  sort_digit_bits: int | None = None, cameras_per_block: int | None = None,
+                  sort_tile_items: int | None = None) -> None:
+        """Change launch parameters; takes effect at the next plan()."""
+        if tile_size is not None:
+            if tile_size not in (8, 16):
+                raise ValueError("tile_size must be 8 or 16.")
+            self.options["tile_size"] = int(tile_size)
+        for name, value in (("raster_batch", raster_batch), ("sort_digit_bits", sort_digit_bits),
+                            ("cameras_per_block", cameras_per_block), ("sort_tile_items", sort_tile_items)):
+            if value is not None:
+                self.tuning[name] = int(value)
+        self.graph = None
+
     # ----------------------------------------------------------------------- scenes
     def register_scene(self, scene_id: int, *, means, scales, rotations, opacities, features, semantic_ids) -> None:
         if scene_id in self._scenes:
@@ -208,6 +222,7 @@ class BatchedRenderer:
         g = packed["means"].shape[0]
         if visible_capacity is None:
             visible_capacity = min(b * g, max(1, b * 200_000))
+        visible_capacity = min(int(visible_capacity), 0x7FFFFFFF)
         if intersection_capacity is None:
             intersection_capacity = min((1 << 30) - 1, 3 * visible_capacity)
         self._allocate(visible_capacity, intersection_capacity)
