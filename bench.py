@@ -63,6 +63,8 @@ def parse_args() -> argparse.Namespace:
                         help="extra workloads reported beside the main line: c4 (fwd+bwd), c5 (robotics batch, "
                              "batched-persistent vs plain, pose backward), c3 (uneven multi-scene batch); '' for none")
     parser.add_argument("--extra-steps", type=int, default=10, help="timed steps of each extra workload")
+    parser.add_argument("--skip-main", action="store_true",
+                        help="run only the extra workloads (profiling aid); prints their JSON")
     parser.add_argument("--copy-calibration-mib", type=int, default=0,
                         help="also run one device-to-device copy of this size (PMC byte-counter calibration)")
     return parser.parse_args()
@@ -83,6 +85,14 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     device = torch.device("cuda", 0)
+    if args.skip_main:
+        extras = {}
+        for name in [w for w in args.workloads.split(",") if w]:
+            runner = {"c4": bench_c4_fwd_bwd, "c5": bench_c5_robotics, "c3": bench_c3_uneven}[name]
+            extras[name] = runner(args, torch, device)
+            print(f"[bench] workload {name}: {json.dumps(extras[name])}", file=sys.stderr, flush=True)
+        print(json.dumps({"workloads": extras}), flush=True)
+        return 0
     width, height, focal, seed = 640, 480, 554.0, 20261202
     scene = synthetic.gaussian_scene(args.gaussians, seed=seed, sh_degree=3)
     rig = synthetic.orbit_rig(args.cameras, width, height, focal, seed=seed, radius=(10.0, 15.0))

@@ -167,3 +167,30 @@ def test_autotuned_plan_matches_oracle(tmp_path):
     print(f"[batched] tuned vs oracle: {report}", flush=True)
     for name in ("rgb", "alpha", "depth", "semantic_id"):
         np.testing.assert_array_equal(out[name].reshape(expected[name].shape), expected[name], err_msg=name)
+
+
+def test_split_submissions_match_single_submission():
+    scene = synthetic.gaussian_scene(40_000, seed=141, sh_degree=1, layout="room", semantic_classes=256)
+    rig = synthetic.trajectory_rig(2, 6, 160, 120, 139.0, seed=141)
+    specs = [CameraSpec(0, 160, 120)] * rig.batch
+    single = build([(0, scene)], specs)
+    counters = single.calibrate(rig.viewmats.to(device()), rig.intrinsics.to(device()))
+    assert counters["submissions"] == 1
+    reference = flat_numpy(single.outputs)
+    limit = max(1, counters["tile_intersections"] // 3)
+    split = build([(0, scene)], specs, max_intersections_per_submission=limit)
+    split_counters = split.calibrate(rig.viewmats.to(device()), rig.intrinsics.to(device()))
+    print(f"[batched] {split_counters['submissions']} submissions for limit {limit}: {split_counters}", flush=True)
+    assert split_counters["submissions"] >= 3
+    assert split_counters["tile_intersections"] == counters["tile_intersections"]
+    assert split_counters["visible_gaussians"] == counters["visible_gaussians"]
+    split.capture()
+    out = flat_numpy(split.render())
+    for name in out:
+        np.testing.assert_array_equal(out[name], reference[name], err_msg=name)
+    split.render_plain()
+    for group in split.plain_groups:
+        for local, camera in enumerate(group["index_list"]):
+            view = split.camera_outputs(camera)
+            for name in ("rgb", "depth", "alpha", "semantic_id"):
+                torch.testing.assert_close(view[name], group["outputs"][name][local], rtol=0, atol=0)

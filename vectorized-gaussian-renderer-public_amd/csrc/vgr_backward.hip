@@ -16,8 +16,9 @@
 // Compositing backward (one workgroup per (camera, tile), one lane per pixel)
 // walks the tile list back to front from the checkpointed final
 // transmittance: T_i = T_{i+1} / (1 - alpha_i) is recovered by division and
-// the colour / depth / alpha "behind" sums are rebuilt on the way, so no
-// per-intersection state is stored.  The same per-wave ordered sub-lists as
+// the colour / depth / alpha composited "behind" each splat is rebuilt on the
+// way (as one scalar: its dot product with the pixel's upstream gradient), so
+// no per-intersection state is stored.  The same per-wave ordered sub-lists as
 // the forward are rebuilt from the same records, and every skip test is the
 // forward's own, so exactly the forward's blended (pixel, splat) pairs
 // contribute.  Per-splat gradients are summed over the wave's 64 pixels with
@@ -57,6 +58,17 @@ __device__ __forceinline__ float wave_sum_to_last_lane(float v) {
   v += dpp_term<0x140, 0xF>(v);  // row_mirror
   v += dpp_term<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
   v += dpp_term<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+// Sums of rows 0+1 (valid in lane 31) and rows 2+3 (valid in lane 63): the
+// row sums of wave_sum_to_last_lane, then row 0 into row 1 and row 2 into row 3.
+__device__ __forceinline__ float half_wave_sum(float v) {
+  v += dpp_term<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v += dpp_term<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v += dpp_term<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_term<0x140, 0xF>(v);  // row_mirror
+  v += dpp_term<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
   return v;
 }
 
@@ -171,7 +183,9 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_backward_kerne
   if (tile_last <= 0 || range_start < 0 || range_end_full <= range_start) return;
   const int64_t range_end = min(range_end_full, range_start + static_cast<int64_t>(tile_last));
 
-  float behind_r = 0.0f, behind_g = 0.0f, behind_b = 0.0f, behind_d = 0.0f, behind_a = 0.0f;
+  // behind = (colour, depth, alpha) composited behind the current splat, dotted with
+  // the pixel's upstream gradient (one scalar: the recursion is linear).
+  float behind = 0.0f;
   const int64_t batches = (range_end - range_start + kBatch - 1) / kBatch;
   f32x4 next_a[kPerThread] = {}, next_b[kPerThread] = {}, next_c[kPerThread] = {};
   int32_t next_ids[kPerThread];
@@ -219,22 +233,20 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_backward_kerne
       float d_mx = 0.0f, d_my = 0.0f, d_c0 = 0.0f, d_c1 = 0.0f, d_c2 = 0.0f, d_o = 0.0f;
       float d_z = 0.0f, d_r = 0.0f, d_g = 0.0f, d_b = 0.0f;
       if (blended) {
+        // Gradient arithmetic only (the replayed forward decisions above stay uncontracted).
+#pragma clang fp contract(fast)
         const f32x4 col = s.colour[item];
         const float keep = 1.0f - splat_alpha;
-        const float before = transmittance / keep;
+        // keep >= 0.01; the hardware reciprocal (1 ulp) is ample for a gradient.
+        const float before = transmittance * __builtin_amdgcn_rcpf(keep);
         const float weight = splat_alpha * before;
         d_r = weight * grad_r;
         d_g = weight * grad_g;
         d_b = weight * grad_b;
         d_z = weight * grad_d;
-        const float d_alpha = ((col.x - behind_r) * grad_r + (col.y - behind_g) * grad_g +
-                               (col.z - behind_b) * grad_b + (sh.w - behind_d) * grad_d +
-                               (1.0f - behind_a) * grad_a) * before;
-        behind_r = splat_alpha * col.x + keep * behind_r;
-        behind_g = splat_alpha * col.y + keep * behind_g;
-        behind_b = splat_alpha * col.z + keep * behind_b;
-        behind_d = splat_alpha * sh.w + keep * behind_d;
-        behind_a = splat_alpha + keep * behind_a;
+        const float own = col.x * grad_r + col.y * grad_g + col.z * grad_b + sh.w * grad_d + grad_a;
+        const float d_alpha = (own - behind) * before;
+        behind = splat_alpha * own + keep * behind;
         transmittance = before;
         if (sh.y * gaussian < VGR_MAX_ALPHA) {
           d_o = d_alpha * gaussian;
@@ -272,6 +284,260 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_backward_kerne
         atomic_add_f32(out + 8, d_g);
         atomic_add_f32(out + 9, d_b);
       }
+    }
+  }
+}
+
+// Wave-per-tile form for 16x16 tiles: one wave64 owns the whole tile and every
+// lane four pixels (column lane % 16, rows lane / 16 + 4k).  Pixel k of every
+// lane lies in the tile's k-th band of four rows, so the per-band reach mask of
+// a splat (the same conservative ellipse box as the forward's per-wave lists)
+// is a uniform branch.  Each lane sums its four pixels' gradients of a splat;
+// DPP then sums each half of the wave, the two half sums go to LDS, and after
+// the batch every lane adds up one slot's halves and issues that splat's
+// atomics, so the atomics of a batch are 10 vector instructions.
+template <int kBatch>
+__global__ __launch_bounds__(kWave) void composite_backward_wave_kernel(CompositeBackwardArgs args) {
+  constexpr int kTileSize = 16;
+  constexpr int kBands = 4;
+  constexpr int kPerLane = kBatch / kWave;
+  static_assert(kBatch % kWave == 0 && kBatch <= 512, "batch must be a multiple of 64 and at most 512");
+  __shared__ f32x4 s_geometry[kBatch];
+  __shared__ f32x4 s_shading[kBatch];
+  __shared__ f32x4 s_colour[kBatch];
+  __shared__ int32_t s_record[kBatch];
+  __shared__ uint16_t s_list[kBatch];
+  // Per batch slot: sums of its ten gradients over each half of the wave (padded
+  // to 12 floats for 16-byte stores) and whether any pixel blended it.
+  __shared__ f32x4 s_partial[kBatch][2][3];
+  __shared__ uint32_t s_blended[kBatch];
+
+  const int64_t block = blockIdx.x;
+  int active;
+  int64_t image_tile;
+  if (args.uniform_tiles > 0) {
+    active = static_cast<int>(block / args.uniform_tiles);
+    if (active >= args.active_count) return;
+    image_tile = block - static_cast<int64_t>(active) * args.uniform_tiles;
+  } else {
+    active = find_active_camera(args.table, args.active_count, block);
+    image_tile = block - args.table[active].active_tile_begin;
+    if (image_tile >= static_cast<int64_t>(args.table[active].tiles_x) * args.table[active].tiles_y) return;
+  }
+  const CameraParams& cam = args.table[active];
+  if (cam.camera < 0) return;
+  const int tile_x = static_cast<int>(image_tile % cam.tiles_x);
+  const int tile_y = static_cast<int>(image_tile / cam.tiles_x);
+  const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
+  const int lane = threadIdx.x;
+  const int half = lane / (kWave / 2);
+  const int pixel_x = tile_x * kTileSize + (lane % kTileSize);
+  const float px = static_cast<float>(pixel_x) + 0.5f;
+  const float rect_x0 = static_cast<float>(tile_x * kTileSize) + 0.5f;
+  const float rect_x1 = rect_x0 + static_cast<float>(kTileSize - 1);
+  const float rect_y_base = static_cast<float>(tile_y * kTileSize) + 0.5f;
+
+  float py[kBands], transmittance[kBands], grad_r[kBands], grad_g[kBands], grad_b[kBands], grad_d[kBands],
+      grad_a[kBands], behind[kBands];
+  int last[kBands];
+  int lane_last = 0;
+#pragma unroll
+  for (int k = 0; k < kBands; ++k) {
+    const int pixel_y = tile_y * kTileSize + lane / kTileSize + 4 * k;
+    py[k] = static_cast<float>(pixel_y) + 0.5f;
+    last[k] = 0;
+    transmittance[k] = 1.0f;
+    grad_r[k] = grad_g[k] = grad_b[k] = grad_d[k] = grad_a[k] = 0.0f;
+    behind[k] = 0.0f;  // what lies behind, dotted with the upstream gradient
+    if (pixel_x < cam.width && pixel_y < cam.height) {
+      const int64_t pixel = cam.pixel_offset + static_cast<int64_t>(pixel_y) * cam.width + pixel_x;
+      last[k] = args.last_blended[pixel];
+      transmittance[k] = args.final_transmittance[pixel];
+      grad_r[k] = args.grad_rgb[pixel * 3 + 0];
+      grad_g[k] = args.grad_rgb[pixel * 3 + 1];
+      grad_b[k] = args.grad_rgb[pixel * 3 + 2];
+      grad_a[k] = args.grad_alpha != nullptr ? args.grad_alpha[pixel] : 0.0f;
+      const float accumulated = 1.0f - transmittance[k];
+      if (args.grad_depth != nullptr && accumulated > 1.0e-8f) {
+        const float upstream = args.grad_depth[pixel];
+        if (upstream != 0.0f) {
+          grad_d[k] = upstream / accumulated;
+          grad_a[k] -= upstream * args.depth[pixel] / accumulated;
+        }
+      }
+      lane_last = max(lane_last, last[k]);
+    }
+  }
+  int tile_last = lane_last;
+  for (int offset = 32; offset > 0; offset >>= 1) tile_last = max(tile_last, __shfl_xor(tile_last, offset, kWave));
+  // Per band, the largest last-blended position of its 64 pixels: list entries at or
+  // beyond it are skipped for the whole band.
+  tile_last = __builtin_amdgcn_readfirstlane(tile_last);
+  int band_last[kBands];
+#pragma unroll
+  for (int k = 0; k < kBands; ++k) {
+    int value = last[k];
+    for (int offset = 32; offset > 0; offset >>= 1) value = max(value, __shfl_xor(value, offset, kWave));
+    band_last[k] = __builtin_amdgcn_readfirstlane(value);
+  }
+  const int64_t range_start = args.tile_starts[global_tile];
+  const int64_t range_end_full = args.tile_ends[global_tile];
+  if (tile_last <= 0 || range_start < 0 || range_end_full <= range_start) return;
+  const int64_t range_end = min(range_end_full, range_start + static_cast<int64_t>(tile_last));
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+
+  const int64_t batches = (range_end - range_start + kBatch - 1) / kBatch;
+  f32x4 next_a[kPerLane] = {}, next_b[kPerLane] = {}, next_c[kPerLane] = {};
+  int32_t next_ids[kPerLane];
+  {
+    const int64_t start = range_start + (batches - 1) * kBatch;
+    gather_batch<kWave, kBatch, kPerLane>(args.sorted_values, args.records, args.visible_capacity, start, range_end,
+                                          lane, next_a, next_b, next_c);
+    gather_batch_ids<kWave, kBatch, kPerLane>(args.sorted_values, args.visible_capacity, start, range_end, lane,
+                                              next_ids);
+  }
+  for (int64_t b = batches - 1; b >= 0; --b) {
+    const int64_t batch_start = range_start + b * kBatch;
+    const int count = static_cast<int>(range_end - batch_start < kBatch ? range_end - batch_start : kBatch);
+    __syncthreads();  // previous batch fully consumed
+    uint32_t list_count = 0;
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+      const int slot = k * kWave + lane;
+      uint32_t bands = 0u;
+      if (slot < count) {
+        const StagedSplat staged = stage_record(next_a[k], next_b[k], next_c[k]);
+        s_geometry[slot] = staged.geometry;
+        s_shading[slot] = staged.shading;
+        s_colour[slot] = staged.colour;
+        s_record[slot] = next_ids[k];
+        s_blended[slot] = 0u;
+        float half_x, half_y;
+        if (splat_extent(next_a[k], next_b[k], staged.shading.z, half_x, half_y)) {
+          const float mx = next_a[k].x, my = next_a[k].y;
+          if (fabsf(mx - fminf(fmaxf(mx, rect_x0), rect_x1)) <= half_x) {
+#pragma unroll
+            for (int band = 0; band < kBands; ++band) {
+              const float y0 = rect_y_base + static_cast<float>(4 * band);
+              const float y1 = y0 + 3.0f;
+              if (fabsf(my - fminf(fmaxf(my, y0), y1)) <= half_y) bands |= 1u << band;
+            }
+          }
+        } else {
+          bands = (1u << kBands) - 1u;
+        }
+      }
+      const uint64_t vote = __ballot(bands != 0u);
+      if (bands != 0u) {
+        s_list[list_count + static_cast<uint32_t>(__popcll(vote & below))] =
+            static_cast<uint16_t>(slot | (bands << 12));
+      }
+      list_count += static_cast<uint32_t>(__popcll(vote));
+    }
+    __syncthreads();
+    if (b > 0) {
+      const int64_t start = batch_start - kBatch;
+      gather_batch<kWave, kBatch, kPerLane>(args.sorted_values, args.records, args.visible_capacity, start,
+                                            range_end, lane, next_a, next_b, next_c);
+      gather_batch_ids<kWave, kBatch, kPerLane>(args.sorted_values, args.visible_capacity, start, range_end, lane,
+                                                next_ids);
+    }
+    const int position_base = static_cast<int>(batch_start - range_start);
+    for (uint32_t i = list_count; i-- > 0;) {
+      const uint32_t entry = s_list[i];
+      const int item = static_cast<int>(entry & 0x1ffu);
+      const uint32_t bands = entry >> 12;
+      const f32x4 g = s_geometry[item];
+      const f32x4 sh = s_shading[item];
+      const f32x4 col = s_colour[item];
+      const int position = position_base + item;
+      const float delta_x = px - g.x;
+      float d_mx = 0.0f, d_my = 0.0f, d_c0 = 0.0f, d_c1 = 0.0f, d_c2 = 0.0f, d_o = 0.0f;
+      float d_z = 0.0f, d_r = 0.0f, d_g = 0.0f, d_b = 0.0f;
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < kBands; ++k) {
+        // Uniform skips: the band is out of the splat's reach, or every pixel of the
+        // band stopped blending before this list position.
+        if (((bands >> k) & 1u) == 0u || position >= band_last[k]) continue;
+        const float delta_y = py[k] - g.y;
+        const float power = g.z * delta_x * delta_x + g.w * delta_x * delta_y + sh.x * delta_y * delta_y;
+        const bool reachable = position < last[k] && !(power > 0.0f || power < sh.z);
+        if (__ballot(reachable) == 0ull) continue;
+        // From here the band runs without divergent branches: lanes that do not
+        // blend the splat take alpha = 0 (keep = 1, nothing changes) and zero
+        // gradients, so every lane executes the same straight-line code.
+        const float exact = vgr_expf(power);
+        const float splat_alpha = fminf(VGR_MAX_ALPHA, sh.y * exact);
+        const bool blended = reachable && splat_alpha >= VGR_ALPHA_THRESHOLD;
+        if (__ballot(blended) == 0ull) continue;
+        any = true;
+        {
+        // Gradient arithmetic only (the replayed forward decisions above stay uncontracted).
+#pragma clang fp contract(fast)
+        const float alpha = blended ? splat_alpha : 0.0f;
+        const float gaussian = blended && sh.y * exact < VGR_MAX_ALPHA ? exact : 0.0f;  // 0 where alpha is capped
+        const float keep = 1.0f - alpha;
+        const float before = transmittance[k] * __builtin_amdgcn_rcpf(keep);
+        const float weight = alpha * before;
+        d_r += weight * grad_r[k];
+        d_g += weight * grad_g[k];
+        d_b += weight * grad_b[k];
+        d_z += weight * grad_d[k];
+        const float own = col.x * grad_r[k] + col.y * grad_g[k] + col.z * grad_b[k] + sh.w * grad_d[k] + grad_a[k];
+        const float d_alpha = (own - behind[k]) * before;
+        behind[k] = alpha * own + keep * behind[k];
+        transmittance[k] = before;
+        d_o += d_alpha * gaussian;
+        const float d_power = d_alpha * gaussian * sh.y;  // = d_alpha * alpha below the cap
+        d_mx -= d_power * (2.0f * g.z * delta_x + g.w * delta_y);
+        d_my -= d_power * (g.w * delta_x + 2.0f * sh.x * delta_y);
+        d_c0 -= 0.5f * d_power * delta_x * delta_x;
+        d_c1 -= d_power * delta_x * delta_y;
+        d_c2 -= 0.5f * d_power * delta_y * delta_y;
+        }
+      }
+      if (__ballot(any) == 0ull) continue;
+      // Half-wave sums now; the two halves are added per slot after the batch, where
+      // every lane owns one slot and issues its atomics (one vector atomic per value).
+      d_mx = half_wave_sum(d_mx);
+      d_my = half_wave_sum(d_my);
+      d_c0 = half_wave_sum(d_c0);
+      d_c1 = half_wave_sum(d_c1);
+      d_c2 = half_wave_sum(d_c2);
+      d_o = half_wave_sum(d_o);
+      d_z = half_wave_sum(d_z);
+      d_r = half_wave_sum(d_r);
+      d_g = half_wave_sum(d_g);
+      d_b = half_wave_sum(d_b);
+      if ((lane & (kWave / 2 - 1)) == kWave / 2 - 1) {
+        s_partial[item][half][0] = f32x4{d_mx, d_my, d_c0, d_c1};
+        s_partial[item][half][1] = f32x4{d_c2, d_o, d_z, d_r};
+        s_partial[item][half][2] = f32x4{d_g, d_b, 0.0f, 0.0f};
+      }
+      if (lane == 0) s_blended[item] = 1u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+      const int slot = k * kWave + lane;
+      if (slot >= count || s_blended[slot] == 0u) continue;
+      const int32_t record = s_record[slot];
+      if (record < 0) continue;
+      const f32x4 p0 = s_partial[slot][0][0] + s_partial[slot][1][0];
+      const f32x4 p1 = s_partial[slot][0][1] + s_partial[slot][1][1];
+      const f32x4 p2 = s_partial[slot][0][2] + s_partial[slot][1][2];
+      float* out = args.record_grads + static_cast<int64_t>(record) * kRecordGrads;
+      atomic_add_f32(out + 0, p0.x);
+      atomic_add_f32(out + 1, p0.y);
+      atomic_add_f32(out + 2, p0.z);
+      atomic_add_f32(out + 3, p0.w);
+      atomic_add_f32(out + 4, p1.x);
+      atomic_add_f32(out + 5, p1.y);
+      atomic_add_f32(out + 6, p1.z);
+      atomic_add_f32(out + 7, p1.w);
+      atomic_add_f32(out + 8, p2.x);
+      atomic_add_f32(out + 9, p2.y);
     }
   }
 }
@@ -546,7 +812,18 @@ struct ProjectionBackwardArgs {
   float* grad_opacities;
   float* grad_features;
   float* camera_grads;
+  int exclusive;  // one camera group: every Gaussian's gradients are owned by one thread
 };
+
+// Adds a per-Gaussian gradient: plain read-modify-write when one thread owns the
+// Gaussian (a single camera group), a float atomic otherwise.
+__device__ __forceinline__ void add_gradient(float* address, float value, bool exclusive) {
+  if (exclusive) {
+    *address += value;
+  } else {
+    atomic_add_f32(address, value);
+  }
+}
 
 template <int kDegree, bool kScene, bool kPose>
 __global__ __launch_bounds__(kBlockThreads) void projection_backward_kernel(ProjectionBackwardArgs args) {
@@ -660,15 +937,16 @@ __global__ __launch_bounds__(kBlockThreads) void projection_backward_kernel(Proj
     visible_base += total_visible;
   }
   if (!kScene || !touched) return;
-  atomic_add_f32(args.grad_means + g * 3 + 0, grad_mean[0]);
-  atomic_add_f32(args.grad_means + g * 3 + 1, grad_mean[1]);
-  atomic_add_f32(args.grad_means + g * 3 + 2, grad_mean[2]);
+  const bool exclusive = args.exclusive != 0;
+  add_gradient(args.grad_means + g * 3 + 0, grad_mean[0], exclusive);
+  add_gradient(args.grad_means + g * 3 + 1, grad_mean[1], exclusive);
+  add_gradient(args.grad_means + g * 3 + 2, grad_mean[2], exclusive);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) atomic_add_f32(args.grad_covariances + g * 6 + k, grad_cov[k]);
-  atomic_add_f32(args.grad_opacities + g, grad_opacity);
+  for (int k = 0; k < 6; ++k) add_gradient(args.grad_covariances + g * 6 + k, grad_cov[k], exclusive);
+  add_gradient(args.grad_opacities + g, grad_opacity, exclusive);
   float* row = args.grad_features + g * fw.feature_width;
 #pragma unroll
-  for (int k = 0; k < kChannels; ++k) atomic_add_f32(row + k, grad_f[k]);
+  for (int k = 0; k < kChannels; ++k) add_gradient(row + k, grad_f[k], exclusive);
 }
 
 // dL/d viewmat of every active camera from the accumulated partials:
@@ -800,6 +1078,30 @@ void launch_composite_backward(const CompositeBackwardArgs& args, int64_t blocks
                      dim3(kTileSize * kTileSize), 0, stream, args);
 }
 
+template <int kBatch>
+void launch_composite_backward_wave(const CompositeBackwardArgs& args, int64_t blocks, hipStream_t stream) {
+  hipLaunchKernelGGL((composite_backward_wave_kernel<kBatch>), dim3(static_cast<unsigned>(blocks)), dim3(kWave), 0,
+                     stream, args);
+}
+
+void launch_composite_backward_tile16(int batch, const CompositeBackwardArgs& args, int64_t blocks,
+                                      hipStream_t stream) {
+  switch (batch) {
+    case 64:
+      launch_composite_backward_wave<64>(args, blocks, stream);
+      break;
+    case 128:
+      launch_composite_backward_wave<128>(args, blocks, stream);
+      break;
+    case 512:
+      launch_composite_backward_wave<512>(args, blocks, stream);
+      break;
+    default:
+      launch_composite_backward_wave<256>(args, blocks, stream);
+      break;
+  }
+}
+
 template <int kTileSize>
 void launch_composite_backward_batch(int batch, const CompositeBackwardArgs& args, int64_t blocks,
                                      hipStream_t stream) {
@@ -878,7 +1180,9 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
     if (options.tile_size == 8) {
       launch_composite_backward_batch<8>(options.raster_batch, cargs, tile_blocks, stream);
     } else {
-      launch_composite_backward_batch<16>(options.raster_batch, cargs, tile_blocks, stream);
+      // The backward's batch is independent of the forward's (decisions are per pair);
+      // 64 keeps the prefetch registers small enough for three or more waves per SIMD.
+      launch_composite_backward_tile16(64, cargs, tile_blocks, stream);
     }
   }
 
@@ -892,6 +1196,7 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
   pargs.grad_opacities = grads.grad_opacities;
   pargs.grad_features = grads.grad_features;
   pargs.camera_grads = grads.camera_grads;
+  pargs.exclusive = layout.projection_groups == 1 ? 1 : 0;
   const bool scene_grads = grads.grad_means != nullptr;
   const int64_t blocks = layout.projection_blocks;
   if (blocks > 0 && (scene_grads || pose)) {

@@ -115,6 +115,9 @@ class DifferentiableRenderer:
         self.native = native if native is not None else native_module.load_native_module()
         self.last_counters: dict[str, int] | None = None
         self.check_capacity = True
+        self.max_visible = 1 << 27
+        self.max_intersections = (1 << 30) - 1
+        self.camera_chunks = 1
 
     # -- workspace ------------------------------------------------------------------------
     def _tiles(self, batch: int) -> int:
@@ -133,7 +136,8 @@ class DifferentiableRenderer:
         scratch_bytes = self.native.scratch_bytes(
             max_views=batch, height=self.height, width=self.width, gaussians=gaussians,
             intersection_capacity=intersections, tile_size=self.tile_size, near_plane=self.near_plane,
-            far_plane=self.far_plane, sort_digit_bits=self.sort_digit_bits, cameras_per_block=self.cameras_per_block)
+            far_plane=self.far_plane, sort_digit_bits=self.sort_digit_bits,
+            cameras_per_block=self.cameras_per_block or min(64, max(1, batch)))
         tiles = self._tiles(batch)
         return _Workspace(
             records=torch.empty((visible, RECORD_FLOATS), dtype=torch.float32, device=device),
@@ -152,6 +156,9 @@ class DifferentiableRenderer:
         batch = int(viewmats.shape[0])
         gaussians = int(means.shape[0])
         device = self.device
+        # One camera group per projection workgroup (up to 64 cameras) lets the backward
+        # own each Gaussian's gradient in one thread (no atomics).
+        cameras_per_block = self.cameras_per_block or min(64, max(1, batch))
         covariances = torch.empty((gaussians, 6), dtype=torch.float32, device=device)
         self.native.precompute_covariances(scales, quats, covariances)
         pixels = batch * self.height * self.width
@@ -181,27 +188,27 @@ class DifferentiableRenderer:
                 depth_bucket_group_size=0, compact_projection_cache=False, materialize_projected_records=False,
                 reuse_projection=False, output_srgb=False, deterministic=True, full_sensor_output=True,
                 fixed_capacity_sort=False, sh_degree=sh_degree, sort_digit_bits=self.sort_digit_bits,
-                raster_batch=self.raster_batch, cameras_per_block=self.cameras_per_block,
+                raster_batch=self.raster_batch, cameras_per_block=cameras_per_block,
                 checkpoints=(transmittance, last_blended))
             call = self.native.last_call
             if not self.check_capacity:
                 break
             visible, intersections = (int(v) for v in workspace.counters[:2].tolist())
             self.last_counters = {"visible_gaussians": visible, "tile_intersections": intersections}
+            if visible > self.max_visible or intersections > self.max_intersections:
+                factor = max(visible / self.max_visible, intersections / self.max_intersections)
+                raise _SplitCameras(math.ceil(factor * _GROWTH))
             grow = False
             if visible > workspace.records.shape[0]:
-                self.visible_capacity = int(math.ceil(visible * _GROWTH))
+                self.visible_capacity = min(int(math.ceil(visible * _GROWTH)), 0x7FFFFFFF)
                 self.intersection_capacity = max(int(self.intersection_capacity or 0),
-                                                 int(math.ceil(intersections * _GROWTH)))
+                                                 min(int(math.ceil(intersections * _GROWTH)), (1 << 30) - 1))
                 grow = True
             if intersections > workspace.keys_in.numel():
-                self.intersection_capacity = int(math.ceil(intersections * _GROWTH))
+                self.intersection_capacity = min(int(math.ceil(intersections * _GROWTH)), (1 << 30) - 1)
                 grow = True
             if not grow:
                 break
-            if self.intersection_capacity >= (1 << 30) or self.visible_capacity > 0x7FFFFFFF:
-                raise RuntimeError(f"batch needs {visible} visible records and {intersections} intersections, "
-                                   "beyond the int32 workspace limits; render fewer cameras per call.")
         else:
             raise RuntimeError("capacity growth did not converge")
         state = {"call": call, "workspace": workspace, "covariances": covariances, "transmittance": transmittance,
@@ -232,9 +239,31 @@ class DifferentiableRenderer:
             env_xforms = torch.eye(4, dtype=torch.float32, device=self.device).expand(batch, 4, 4).contiguous()
         scales = torch.exp(log_scales)
         opacities = torch.sigmoid(opacity_logits)
-        rgb, depth, alpha, semantic = _RasterizeFunction.apply(
-            means, scales, quats, opacities, features, viewmats, intrinsics, env_xforms, semantic_ids, self, degree)
-        return {"rgb": rgb, "depth": depth, "alpha": alpha, "semantic_id": semantic}
+        while True:
+            chunks = max(1, min(self.camera_chunks, batch))
+            bounds = [round(i * batch / chunks) for i in range(chunks + 1)]
+            try:
+                parts = [_RasterizeFunction.apply(means, scales, quats, opacities, features, viewmats[a:b],
+                                                  intrinsics[a:b], env_xforms[a:b], semantic_ids, self, degree)
+                         for a, b in zip(bounds[:-1], bounds[1:])]
+                break
+            except _SplitCameras as split:
+                if chunks >= batch:
+                    raise RuntimeError("a single camera exceeds the workspace limits") from None
+                # Each call holds its workspace until backward; smaller calls bound both.
+                self.camera_chunks = min(batch, chunks * max(2, split.factor))
+        names = ("rgb", "depth", "alpha", "semantic_id")
+        if len(parts) == 1:
+            return dict(zip(names, parts[0]))
+        return {name: torch.cat([p[i] for p in parts], dim=0) for i, name in enumerate(names)}
+
+
+class _SplitCameras(Exception):
+    """One call's demand exceeds the int32 / 2^30 workspace limits: render fewer cameras per call."""
+
+    def __init__(self, factor: int) -> None:
+        super().__init__(f"split cameras by {factor}")
+        self.factor = int(factor)
 
 
 class _RasterizeFunction(torch.autograd.Function):

@@ -86,6 +86,8 @@ class BatchedRenderer:
         raster_batch: int = 0,
         cameras_per_block: int = 0,
         capacity_headroom: float = 1.15,
+        max_visible_per_submission: int = 1 << 28,
+        max_intersections_per_submission: int = (1 << 30) - 1,
         native_module: Any | None = None,
     ) -> None:
         self.device = torch.device(device)
@@ -108,6 +110,9 @@ class BatchedRenderer:
         self.tuning = dict(sort_digit_bits=sort_digit_bits, sort_tile_items=sort_tile_items,
                            raster_batch=raster_batch, cameras_per_block=cameras_per_block)
         self.capacity_headroom = capacity_headroom
+        self.max_visible = int(max_visible_per_submission)
+        self.max_intersections = min(int(max_intersections_per_submission), (1 << 30) - 1)
+        self.chunk_count = 1
         self._scenes: dict[int, dict[str, torch.Tensor]] = {}
         self._packed: dict[str, torch.Tensor] | None = None
         self.cameras: list[CameraSpec] = []
@@ -222,9 +227,11 @@ class BatchedRenderer:
         g = packed["means"].shape[0]
         if visible_capacity is None:
             visible_capacity = min(b * g, max(1, b * 200_000))
-        visible_capacity = min(int(visible_capacity), 0x7FFFFFFF)
+        visible_capacity = min(int(visible_capacity), self.max_visible)
         if intersection_capacity is None:
-            intersection_capacity = min((1 << 30) - 1, 3 * visible_capacity)
+            intersection_capacity = 3 * visible_capacity
+        intersection_capacity = min(int(intersection_capacity), self.max_intersections)
+        self._layout_chunks(self.chunk_count)
         self._allocate(visible_capacity, intersection_capacity)
         # Plain-path groups: one submission per (scene, resolution).
         groups: dict[tuple[int, int, int], list[int]] = {}
@@ -243,17 +250,32 @@ class BatchedRenderer:
                                           index_list=indices, outputs=outputs))
         self.graph = None
 
+    def _layout_chunks(self, count: int) -> None:
+        """Split the cameras into ``count`` consecutive submissions of near-equal size."""
+        b = len(self.cameras)
+        count = max(1, min(int(count), b))
+        ts = self.options["tile_size"]
+        bounds = [round(i * b / count) for i in range(count + 1)]
+        self.chunk_count = count
+        self.chunks = []
+        for start, stop in zip(bounds[:-1], bounds[1:]):
+            tiles = sum(math.ceil(c.width / ts) * math.ceil(c.height / ts) for c in self.cameras[start:stop])
+            self.chunks.append((start, stop, tiles))
+        self.chunk_counters = torch.zeros((count, 5), dtype=torch.int64, device=self.device)
+        self.graph = None
+
     def _scratch_bytes(self, intersection_capacity: int) -> int:
         packed = self._pack()
         g = packed["means"].shape[0]
-        batched = self.native.scratch_bytes(
-            max_views=len(self.cameras), height=1, width=1, gaussians=g, intersection_capacity=intersection_capacity,
+        batched = max(self.native.scratch_bytes(
+            max_views=stop - start, height=1, width=1, gaussians=g, intersection_capacity=intersection_capacity,
             tile_size=self.options["tile_size"], near_plane=self.options["near_plane"],
             far_plane=self.options["far_plane"], sort_digit_bits=self.tuning["sort_digit_bits"],
             cameras_per_block=self.tuning["cameras_per_block"], sort_tile_items=self.tuning["sort_tile_items"],
-            chunk_culling=self.chunk_culling, resolutions=[(c.width, c.height) for c in self.cameras])
+            chunk_culling=self.chunk_culling, resolutions=[(c.width, c.height) for c in self.cameras[start:stop]])
+            for start, stop, _ in self.chunks)
         plain = self.native.scratch_bytes(
-            max_views=len(self.cameras), height=max(c.height for c in self.cameras),
+            max_views=max(stop - start for start, stop, _ in self.chunks), height=max(c.height for c in self.cameras),
             width=max(c.width for c in self.cameras), gaussians=g, intersection_capacity=intersection_capacity,
             tile_size=self.options["tile_size"], near_plane=self.options["near_plane"],
             far_plane=self.options["far_plane"], sort_digit_bits=self.tuning["sort_digit_bits"],
@@ -265,9 +287,9 @@ class BatchedRenderer:
         self.workspace = {}
         torch.cuda.empty_cache()
         ts = self.options["tile_size"]
-        plain_tiles = len(self.cameras) * math.ceil(max(c.width for c in self.cameras) / ts) * \
-            math.ceil(max(c.height for c in self.cameras) / ts)
-        tiles = max(self.total_tiles, plain_tiles)
+        plain_tiles = max(stop - start for start, stop, _ in self.chunks) * \
+            math.ceil(max(c.width for c in self.cameras) / ts) * math.ceil(max(c.height for c in self.cameras) / ts)
+        tiles = max(max(tiles for _, _, tiles in self.chunks), plain_tiles)
         self.workspace = {
             "visible_records": torch.empty((visible_capacity, RECORD_FLOATS), device=dev, dtype=torch.float32),
             "keys_in": torch.empty(intersection_capacity, device=dev, dtype=torch.uint64),
@@ -299,23 +321,38 @@ class BatchedRenderer:
         extra: dict[str, Any] = {}
         if stage_events is not None:
             extra["stage_events"] = stage_events
+        chunk_statistics = None
         if statistics is not None:
-            extra["statistics"] = statistics
-        selector = self.native.render(
-            scene, cameras, outputs, self._workspace_list(), 1, 1, 0, self.options["near_plane"],
-            self.options["far_plane"], self.options["gaussian_support_sigma"], self.options["covariance_epsilon"],
-            self.options["antialiased"], self.options["semantic_min_alpha"], False, self.options["tile_size"],
-            4096, 64, False, False, False, self.options["output_srgb"], True, self.full_sensor_output, False,
-            sh_degree=self.sh_degree, persistent_composite=self.persistent_composite,
-            gaussian_ids=packed["gaussian_ids"] if self.spatial_order else None,
-            chunk_bounds=packed["chunk_bounds"] if self.chunk_culling else None,
-            resolutions=(self.widths, self.heights, self.pixel_offsets, self.total_tiles), **self.tuning,
-            **extra)
-        self._selector = selector
+            statistics.zero_()
+            chunk_statistics = torch.zeros_like(statistics)
+            extra["statistics"] = chunk_statistics
+        for index, (start, stop, tiles) in enumerate(self.chunks):
+            if self.chunk_count > 1:
+                cameras = [t[start:stop] for t in (self.viewmats, self.intrinsics, self.env_xforms, self.scene_ids)]
+                cameras.append(self.empty_active)
+            resolutions = (self.widths[start:stop], self.heights[start:stop], self.pixel_offsets[start:stop], tiles)
+            self._selector = self.native.render(
+                scene, cameras, outputs, self._workspace_list(), 1, 1, 0, self.options["near_plane"],
+                self.options["far_plane"], self.options["gaussian_support_sigma"], self.options["covariance_epsilon"],
+                self.options["antialiased"], self.options["semantic_min_alpha"], False, self.options["tile_size"],
+                4096, 64, False, False, False, self.options["output_srgb"], True, self.full_sensor_output, False,
+                sh_degree=self.sh_degree, persistent_composite=self.persistent_composite,
+                gaussian_ids=packed["gaussian_ids"] if self.spatial_order else None,
+                chunk_bounds=packed["chunk_bounds"] if self.chunk_culling else None,
+                resolutions=resolutions, **self.tuning, **extra)
+            self.chunk_counters[index].copy_(self.workspace["counters"])
+            if chunk_statistics is not None:
+                statistics.add_(chunk_statistics)
 
     def read_counters(self) -> dict[str, int]:
+        """Counters summed over the submissions of the plan (``max_*`` keys: the largest one)."""
         torch.cuda.synchronize(self.device)
-        return dict(zip(COUNTER_NAMES, (int(v) for v in self.workspace["counters"].cpu().tolist())))
+        rows = self.chunk_counters.cpu().tolist()
+        totals = dict(zip(COUNTER_NAMES, (int(sum(column)) for column in zip(*rows))))
+        totals["max_visible_gaussians"] = max(int(r[0]) for r in rows)
+        totals["max_tile_intersections"] = max(int(r[1]) for r in rows)
+        totals["submissions"] = len(rows)
+        return totals
 
     def check_capacity(self) -> dict[str, int]:
         counters = self.read_counters()
@@ -346,18 +383,28 @@ class BatchedRenderer:
                   max_retries: int = 4) -> dict[str, int]:
         """Eager renders that size the capacities to the demand (x headroom)."""
         self.set_cameras(viewmats, intrinsics, env_xforms)
+        counters: dict[str, int] = {}
         for _ in range(max_retries + 1):
             self._submit()
             counters = self.read_counters()
-            if not (counters["visible_overflow"] or counters["intersection_overflow"]):
+            over_limit = (counters["max_visible_gaussians"] > self.max_visible
+                          or counters["max_tile_intersections"] > self.max_intersections)
+            if not (over_limit or counters["visible_overflow"] or counters["intersection_overflow"]):
                 return counters
-            visible = max(self.visible_capacity, math.ceil(counters["visible_gaussians"] * self.capacity_headroom))
-            intersections = max(self.intersection_capacity,
-                                math.ceil(counters["tile_intersections"] * self.capacity_headroom))
-            if intersections > (1 << 30) - 1:
-                raise RendererCapacityError("intersection demand exceeds the 2^30 sort limit",
-                                            telemetry={"counters": counters})
-            self._allocate(visible, intersections)
+            visible = math.ceil(counters["max_visible_gaussians"] * self.capacity_headroom)
+            intersections = math.ceil(counters["max_tile_intersections"] * self.capacity_headroom)
+            if over_limit or visible > self.max_visible or intersections > self.max_intersections:
+                # One submission cannot hold a chunk: split the cameras further (the
+                # counters count the full demand, so the split is sized in one step).
+                if self.chunk_count >= len(self.cameras):
+                    raise RendererCapacityError("a single camera exceeds the per-submission capacity limits",
+                                                telemetry={"counters": counters})
+                factor = max(visible / self.max_visible, intersections / self.max_intersections)
+                self._layout_chunks(min(len(self.cameras), math.ceil(self.chunk_count * factor)))
+                visible = min(visible, self.max_visible)
+                intersections = min(intersections, self.max_intersections)
+            self._allocate(min(max(self.visible_capacity, visible), self.max_visible),
+                           min(max(self.intersection_capacity, intersections), self.max_intersections))
         raise RendererCapacityError("capacity calibration did not converge", telemetry={"counters": counters})
 
     def capture(self) -> torch.cuda.CUDAGraph:
@@ -401,16 +448,23 @@ class BatchedRenderer:
         plain = packed["plain"]
         scene = [plain["means"], plain["covariances"], plain["opacities"], plain["features"],
                  plain["semantic_ids"], packed["registered_scene_ids"], packed["scene_offsets"]]
+        per_submission = max(stop - start for start, stop, _ in self.chunks)
         for group in self.plain_groups:
             idx = group["indices"]
-            cameras = [self.viewmats.index_select(0, idx), self.intrinsics.index_select(0, idx),
-                       self.env_xforms.index_select(0, idx), self.scene_ids.index_select(0, idx), self.empty_active]
             o = group["outputs"]
-            self.native.render(
-                scene, cameras, [o["rgb"], o["depth"], o["alpha"], o["semantic_id"]], self._workspace_list(),
-                group["height"], group["width"], 0, self.options["near_plane"], self.options["far_plane"],
-                self.options["gaussian_support_sigma"], self.options["covariance_epsilon"],
-                self.options["antialiased"], self.options["semantic_min_alpha"], False, self.options["tile_size"],
-                4096, 64, False, False, False, self.options["output_srgb"], True, self.full_sensor_output, False,
-                sh_degree=self.sh_degree, **self.tuning)
+            for start in range(0, idx.numel(), per_submission):
+                part = idx[start:start + per_submission]
+                cameras = [self.viewmats.index_select(0, part), self.intrinsics.index_select(0, part),
+                           self.env_xforms.index_select(0, part), self.scene_ids.index_select(0, part),
+                           self.empty_active]
+                stop = start + part.numel()
+                outputs = [o["rgb"][start:stop], o["depth"][start:stop], o["alpha"][start:stop],
+                           o["semantic_id"][start:stop]]
+                self.native.render(
+                    scene, cameras, outputs, self._workspace_list(),
+                    group["height"], group["width"], 0, self.options["near_plane"], self.options["far_plane"],
+                    self.options["gaussian_support_sigma"], self.options["covariance_epsilon"],
+                    self.options["antialiased"], self.options["semantic_min_alpha"], False, self.options["tile_size"],
+                    4096, 64, False, False, False, self.options["output_srgb"], True, self.full_sensor_output, False,
+                    sh_degree=self.sh_degree, **self.tuning)
         return self.plain_groups
