@@ -178,3 +178,34 @@ def test_checkpointed_forward_equals_plain_forward():
     np.testing.assert_array_equal(got["rgb"], expected["rgb"])
     np.testing.assert_array_equal(got["alpha"], expected["alpha"])
     np.testing.assert_array_equal(got["depth"], expected["depth"])
+
+
+def test_deferred_capacity_check_fails_loud_and_recovers():
+    from vgr_amd.hip_backend import RendererCapacityError
+
+    size, degree = 64, 3
+    params, views, intrinsics = setup(shift=1.0)
+    renderer = DifferentiableRenderer(height=size, width=size, device=device(), capacity_check="deferred")
+    leaves = [p.to(device()).clone().requires_grad_(True) for p in params[:5]]
+    views_d, k_d = views.to(device()), intrinsics.to(device())
+
+    def step():
+        for leaf in leaves:
+            leaf.grad = None
+        out = renderer.render(*leaves, views_d, k_d, sh_degree=degree)
+        out["rgb"].sum().backward()
+        return [leaf.grad.clone() for leaf in leaves]
+
+    reference = step()  # first call: synchronous calibration
+    assert all(bool(torch.isfinite(g).all()) for g in reference)
+    again = step()      # deferred: no host synchronization, same result
+    for a, b in zip(again, reference):
+        torch.testing.assert_close(a, b, rtol=1e-5, atol=1e-6)
+    renderer.visible_capacity = 16  # buffers far too small for the next render
+    poisoned = step()
+    assert any(bool(torch.isnan(g).any()) for g in poisoned), "overflowed backward must not be silent"
+    with pytest.raises(RendererCapacityError):
+        step()
+    recovered = step()  # buffers were grown: synchronous again, then correct
+    for a, b in zip(recovered, reference):
+        torch.testing.assert_close(a, b, rtol=1e-5, atol=1e-6)

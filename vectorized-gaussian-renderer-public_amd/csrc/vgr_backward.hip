@@ -426,6 +426,12 @@ __global__ __launch_bounds__(kWave) void composite_backward_wave_kernel(Composit
         } else {
           bands = (1u << kBands) - 1u;
         }
+        // Drop bands whose pixels all stopped blending before this list position.
+        const int position = static_cast<int>(batch_start - range_start) + slot;
+#pragma unroll
+        for (int band = 0; band < kBands; ++band) {
+          if (position >= band_last[band]) bands &= ~(1u << band);
+        }
       }
       const uint64_t vote = __ballot(bands != 0u);
       if (bands != 0u) {
@@ -1006,13 +1012,18 @@ __global__ void camera_grads_finalize_kernel(const CameraParams* __restrict__ ta
 }
 
 // Zeroes the gradient rows of the stored visible records (count read on the device).
+// Device-side guard: when the forward overflowed a buffer (its lists are then
+// incomplete) every record gradient starts as NaN, so the scene and pose
+// gradients come out NaN instead of silently wrong.
 __global__ void zero_record_grads_kernel(float* __restrict__ grads, const int64_t* __restrict__ counters,
-                                         int64_t capacity) {
+                                         int64_t capacity, int64_t intersection_capacity) {
+  const bool overflow = counters[0] > capacity || counters[1] > intersection_capacity;
   const int64_t visible = min(counters[0], capacity);
   const int64_t n = visible * kRecordGrads;
+  const float value = overflow ? __int_as_float(0x7fc00000) : 0.0f;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    grads[i] = 0.0f;
+    grads[i] = value;
   }
 }
 
@@ -1142,7 +1153,7 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
   if (cameras.active_count <= 0) return 0;
   const CameraParams* table = reinterpret_cast<const CameraParams*>(scratch + layout.camera_table);
   hipLaunchKernelGGL(zero_record_grads_kernel, dim3(1024), dim3(kBlockThreads), 0, stream, grads.record_grads,
-                     ws.counters, ws.visible_capacity);
+                     ws.counters, ws.visible_capacity, ws.intersection_capacity);
   const bool pose = grads.grad_viewmats != nullptr;
   if (pose) launch_fill_f32(grads.camera_grads, static_cast<int64_t>(cameras.active_count) * kCameraGrads, 0.0f,
                             stream);

@@ -89,7 +89,7 @@ class DifferentiableRenderer:
                  covariance_epsilon: float = 0.3, rasterize_mode: str = "classic", semantic_min_alpha: float = 0.01,
                  tile_size: int = 16, raster_batch: int = 0, cameras_per_block: int = 0, sort_digit_bits: int = 0,
                  visible_capacity: int | None = None, intersections_per_visible: float = 4.0,
-                 native: native_module.NativeRenderer | None = None) -> None:
+                 capacity_check: str = "sync", native: native_module.NativeRenderer | None = None) -> None:
         if rasterize_mode not in ("classic", "antialiased"):
             raise ValueError("rasterize_mode must be 'classic' or 'antialiased'.")
         if tile_size not in (8, 16):
@@ -113,8 +113,18 @@ class DifferentiableRenderer:
         self.intersections_per_visible = float(intersections_per_visible)
         self.intersection_capacity: int | None = None
         self.native = native if native is not None else native_module.load_native_module()
+        if capacity_check not in ("sync", "deferred"):
+            raise ValueError("capacity_check must be 'sync' or 'deferred'.")
         self.last_counters: dict[str, int] | None = None
-        self.check_capacity = True
+        # "sync": every forward reads its counters (one host synchronization) and
+        # re-renders with grown buffers on overflow.  "deferred": after the first
+        # (synchronous) call, counters are copied to pinned memory without waiting
+        # and checked at the next call, which raises RendererCapacityError (after
+        # growing the buffers) if an earlier render overflowed; the backward of an
+        # overflowed render yields NaN gradients on the device, never silent ones.
+        self.capacity_check = capacity_check
+        self._calibrated = False
+        self._pending: list[tuple[torch.cuda.Event, torch.Tensor, int, int]] = []
         self.max_visible = 1 << 27
         self.max_intersections = (1 << 30) - 1
         self.camera_chunks = 1
@@ -191,7 +201,12 @@ class DifferentiableRenderer:
                 raster_batch=self.raster_batch, cameras_per_block=cameras_per_block,
                 checkpoints=(transmittance, last_blended))
             call = self.native.last_call
-            if not self.check_capacity:
+            if self.capacity_check == "deferred" and self._calibrated:
+                pinned = torch.empty(2, dtype=torch.int64, pin_memory=True)
+                pinned.copy_(workspace.counters[:2], non_blocking=True)
+                event = torch.cuda.Event()
+                event.record()
+                self._pending.append((event, pinned, int(workspace.records.shape[0]), int(workspace.keys_in.numel())))
                 break
             visible, intersections = (int(v) for v in workspace.counters[:2].tolist())
             self.last_counters = {"visible_gaussians": visible, "tile_intersections": intersections}
@@ -208,12 +223,44 @@ class DifferentiableRenderer:
                 self.intersection_capacity = min(int(math.ceil(intersections * _GROWTH)), (1 << 30) - 1)
                 grow = True
             if not grow:
+                self._calibrated = True
                 break
         else:
             raise RuntimeError("capacity growth did not converge")
         state = {"call": call, "workspace": workspace, "covariances": covariances, "transmittance": transmittance,
                  "last_blended": last_blended, "scene": scene, "cameras": cameras, "outputs": outputs}
         return outputs, state
+
+    def check_pending(self, wait: bool = False) -> None:
+        """Check counters of earlier deferred renders; raise if one overflowed."""
+        keep = []
+        overflow = None
+        for event, pinned, visible_capacity, intersection_capacity in self._pending:
+            if not (wait or event.query()):
+                keep.append((event, pinned, visible_capacity, intersection_capacity))
+                continue
+            event.synchronize()
+            visible, intersections = int(pinned[0]), int(pinned[1])
+            self.last_counters = {"visible_gaussians": visible, "tile_intersections": intersections}
+            if visible > visible_capacity or intersections > intersection_capacity:
+                overflow = (visible, intersections, visible_capacity, intersection_capacity)
+        self._pending = keep
+        if overflow is None:
+            return
+        visible, intersections, visible_capacity, intersection_capacity = overflow
+        self._pending = []
+        self._calibrated = False  # the next call renders synchronously with grown buffers
+        self.visible_capacity = min(max(int(self.visible_capacity or 0), math.ceil(visible * _GROWTH)), 0x7FFFFFFF)
+        self.intersection_capacity = min(max(int(self.intersection_capacity or 0),
+                                             math.ceil(intersections * _GROWTH)), (1 << 30) - 1)
+        from .hip_backend import RendererCapacityError
+
+        raise RendererCapacityError(
+            f"an earlier deferred render needed {visible} visible records / {intersections} intersections "
+            f"(capacity {visible_capacity} / {intersection_capacity}); its outputs and gradients are invalid "
+            "(gradients are NaN). Buffers have been grown: repeat the step.",
+            telemetry={"schema_version": "renderer-capacity-error/v1", "reason": "deferred-overflow",
+                       "counters": {"visible_gaussians": visible, "tile_intersections": intersections}})
 
     def render(self, means: torch.Tensor, log_scales: torch.Tensor, quats: torch.Tensor,
                opacity_logits: torch.Tensor, features: torch.Tensor, viewmats: torch.Tensor,
@@ -237,6 +284,8 @@ class DifferentiableRenderer:
             semantic_ids = torch.zeros(g, dtype=torch.int64, device=self.device)
         if env_xforms is None:
             env_xforms = torch.eye(4, dtype=torch.float32, device=self.device).expand(batch, 4, 4).contiguous()
+        if self._pending:
+            self.check_pending(wait=len(self._pending) > 8)
         scales = torch.exp(log_scales)
         opacities = torch.sigmoid(opacity_logits)
         while True:
