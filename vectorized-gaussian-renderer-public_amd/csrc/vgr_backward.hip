@@ -61,17 +61,6 @@ __device__ __forceinline__ float wave_sum_to_last_lane(float v) {
   return v;
 }
 
-// Sums of rows 0+1 (valid in lane 31) and rows 2+3 (valid in lane 63): the
-// row sums of wave_sum_to_last_lane, then row 0 into row 1 and row 2 into row 3.
-__device__ __forceinline__ float half_wave_sum(float v) {
-  v += dpp_term<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
-  v += dpp_term<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
-  v += dpp_term<0x141, 0xF>(v);  // row_half_mirror
-  v += dpp_term<0x140, 0xF>(v);  // row_mirror
-  v += dpp_term<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-  return v;
-}
-
 __device__ __forceinline__ void atomic_add_f32(float* address, float value) {
   unsafeAtomicAdd(address, value);
 }
@@ -293,11 +282,13 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_backward_kerne
 // lane lies in the tile's k-th band of four rows, so the per-band reach mask of
 // a splat (the same conservative ellipse box as the forward's per-wave lists)
 // is a uniform branch.  Each lane sums its four pixels' gradients of a splat;
-// DPP then sums each half of the wave, the two half sums go to LDS, and after
-// the batch every lane adds up one slot's halves and issues that splat's
-// atomics, so the atomics of a batch are 10 vector instructions.
+// DPP then sums the wave into lane 63, which parks the sums in LDS, and after
+// the batch every lane issues one slot's atomics, so the atomics of a batch
+// are 10 vector instructions.
+// Five waves per SIMD: at most 102 VGPRs (the 6.6 KB of LDS per wave would allow 24 per CU);
+// the batch is gathered at its start rather than prefetched, to stay within them.
 template <int kBatch>
-__global__ __launch_bounds__(kWave) void composite_backward_wave_kernel(CompositeBackwardArgs args) {
+__global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(CompositeBackwardArgs args) {
   constexpr int kTileSize = 16;
   constexpr int kBands = 4;
   constexpr int kPerLane = kBatch / kWave;
@@ -307,9 +298,9 @@ __global__ __launch_bounds__(kWave) void composite_backward_wave_kernel(Composit
   __shared__ f32x4 s_colour[kBatch];
   __shared__ int32_t s_record[kBatch];
   __shared__ uint16_t s_list[kBatch];
-  // Per batch slot: sums of its ten gradients over each half of the wave (padded
-  // to 12 floats for 16-byte stores) and whether any pixel blended it.
-  __shared__ f32x4 s_partial[kBatch][2][3];
+  // Per batch slot: the wave sums of its ten gradients (padded to 12 floats for
+  // 16-byte stores) and whether any pixel blended it.
+  __shared__ f32x4 s_partial[kBatch][3];
   __shared__ uint32_t s_blended[kBatch];
 
   const int64_t block = blockIdx.x;
@@ -330,7 +321,6 @@ __global__ __launch_bounds__(kWave) void composite_backward_wave_kernel(Composit
   const int tile_y = static_cast<int>(image_tile / cam.tiles_x);
   const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
   const int lane = threadIdx.x;
-  const int half = lane / (kWave / 2);
   const int pixel_x = tile_x * kTileSize + (lane % kTileSize);
   const float px = static_cast<float>(pixel_x) + 0.5f;
   const float rect_x0 = static_cast<float>(tile_x * kTileSize) + 0.5f;
@@ -387,18 +377,15 @@ __global__ __launch_bounds__(kWave) void composite_backward_wave_kernel(Composit
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
 
   const int64_t batches = (range_end - range_start + kBatch - 1) / kBatch;
-  f32x4 next_a[kPerLane] = {}, next_b[kPerLane] = {}, next_c[kPerLane] = {};
-  int32_t next_ids[kPerLane];
-  {
-    const int64_t start = range_start + (batches - 1) * kBatch;
-    gather_batch<kWave, kBatch, kPerLane>(args.sorted_values, args.records, args.visible_capacity, start, range_end,
-                                          lane, next_a, next_b, next_c);
-    gather_batch_ids<kWave, kBatch, kPerLane>(args.sorted_values, args.visible_capacity, start, range_end, lane,
-                                              next_ids);
-  }
   for (int64_t b = batches - 1; b >= 0; --b) {
     const int64_t batch_start = range_start + b * kBatch;
     const int count = static_cast<int>(range_end - batch_start < kBatch ? range_end - batch_start : kBatch);
+    f32x4 next_a[kPerLane] = {}, next_b[kPerLane] = {}, next_c[kPerLane] = {};
+    int32_t next_ids[kPerLane];
+    gather_batch<kWave, kBatch, kPerLane>(args.sorted_values, args.records, args.visible_capacity, batch_start,
+                                          range_end, lane, next_a, next_b, next_c);
+    gather_batch_ids<kWave, kBatch, kPerLane>(args.sorted_values, args.visible_capacity, batch_start, range_end,
+                                              lane, next_ids);
     __syncthreads();  // previous batch fully consumed
     uint32_t list_count = 0;
 #pragma unroll
@@ -441,13 +428,6 @@ __global__ __launch_bounds__(kWave) void composite_backward_wave_kernel(Composit
       list_count += static_cast<uint32_t>(__popcll(vote));
     }
     __syncthreads();
-    if (b > 0) {
-      const int64_t start = batch_start - kBatch;
-      gather_batch<kWave, kBatch, kPerLane>(args.sorted_values, args.records, args.visible_capacity, start,
-                                            range_end, lane, next_a, next_b, next_c);
-      gather_batch_ids<kWave, kBatch, kPerLane>(args.sorted_values, args.visible_capacity, start, range_end, lane,
-                                                next_ids);
-    }
     const int position_base = static_cast<int>(batch_start - range_start);
     for (uint32_t i = list_count; i-- > 0;) {
       const uint32_t entry = s_list[i];
@@ -504,22 +484,22 @@ __global__ __launch_bounds__(kWave) void composite_backward_wave_kernel(Composit
         }
       }
       if (__ballot(any) == 0ull) continue;
-      // Half-wave sums now; the two halves are added per slot after the batch, where
-      // every lane owns one slot and issues its atomics (one vector atomic per value).
-      d_mx = half_wave_sum(d_mx);
-      d_my = half_wave_sum(d_my);
-      d_c0 = half_wave_sum(d_c0);
-      d_c1 = half_wave_sum(d_c1);
-      d_c2 = half_wave_sum(d_c2);
-      d_o = half_wave_sum(d_o);
-      d_z = half_wave_sum(d_z);
-      d_r = half_wave_sum(d_r);
-      d_g = half_wave_sum(d_g);
-      d_b = half_wave_sum(d_b);
-      if ((lane & (kWave / 2 - 1)) == kWave / 2 - 1) {
-        s_partial[item][half][0] = f32x4{d_mx, d_my, d_c0, d_c1};
-        s_partial[item][half][1] = f32x4{d_c2, d_o, d_z, d_r};
-        s_partial[item][half][2] = f32x4{d_g, d_b, 0.0f, 0.0f};
+      // Wave sums land in lane 63, which parks them in LDS; after the batch every
+      // lane owns one slot and issues its atomics (one vector atomic per value).
+      d_mx = wave_sum_to_last_lane(d_mx);
+      d_my = wave_sum_to_last_lane(d_my);
+      d_c0 = wave_sum_to_last_lane(d_c0);
+      d_c1 = wave_sum_to_last_lane(d_c1);
+      d_c2 = wave_sum_to_last_lane(d_c2);
+      d_o = wave_sum_to_last_lane(d_o);
+      d_z = wave_sum_to_last_lane(d_z);
+      d_r = wave_sum_to_last_lane(d_r);
+      d_g = wave_sum_to_last_lane(d_g);
+      d_b = wave_sum_to_last_lane(d_b);
+      if (lane == kWave - 1) {
+        s_partial[item][0] = f32x4{d_mx, d_my, d_c0, d_c1};
+        s_partial[item][1] = f32x4{d_c2, d_o, d_z, d_r};
+        s_partial[item][2] = f32x4{d_g, d_b, 0.0f, 0.0f};
       }
       if (lane == 0) s_blended[item] = 1u;
     }
@@ -530,9 +510,9 @@ __global__ __launch_bounds__(kWave) void composite_backward_wave_kernel(Composit
       if (slot >= count || s_blended[slot] == 0u) continue;
       const int32_t record = s_record[slot];
       if (record < 0) continue;
-      const f32x4 p0 = s_partial[slot][0][0] + s_partial[slot][1][0];
-      const f32x4 p1 = s_partial[slot][0][1] + s_partial[slot][1][1];
-      const f32x4 p2 = s_partial[slot][0][2] + s_partial[slot][1][2];
+      const f32x4 p0 = s_partial[slot][0];
+      const f32x4 p1 = s_partial[slot][1];
+      const f32x4 p2 = s_partial[slot][2];
       float* out = args.record_grads + static_cast<int64_t>(record) * kRecordGrads;
       atomic_add_f32(out + 0, p0.x);
       atomic_add_f32(out + 1, p0.y);
@@ -1095,24 +1075,6 @@ void launch_composite_backward_wave(const CompositeBackwardArgs& args, int64_t b
                      stream, args);
 }
 
-void launch_composite_backward_tile16(int batch, const CompositeBackwardArgs& args, int64_t blocks,
-                                      hipStream_t stream) {
-  switch (batch) {
-    case 64:
-      launch_composite_backward_wave<64>(args, blocks, stream);
-      break;
-    case 128:
-      launch_composite_backward_wave<128>(args, blocks, stream);
-      break;
-    case 512:
-      launch_composite_backward_wave<512>(args, blocks, stream);
-      break;
-    default:
-      launch_composite_backward_wave<256>(args, blocks, stream);
-      break;
-  }
-}
-
 template <int kTileSize>
 void launch_composite_backward_batch(int batch, const CompositeBackwardArgs& args, int64_t blocks,
                                      hipStream_t stream) {
@@ -1192,8 +1154,8 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
       launch_composite_backward_batch<8>(options.raster_batch, cargs, tile_blocks, stream);
     } else {
       // The backward's batch is independent of the forward's (decisions are per pair);
-      // 64 keeps the prefetch registers small enough for three or more waves per SIMD.
-      launch_composite_backward_tile16(64, cargs, tile_blocks, stream);
+      // 64 keeps the prefetch registers and LDS small enough for five waves per SIMD.
+      launch_composite_backward_wave<64>(cargs, tile_blocks, stream);
     }
   }
 
