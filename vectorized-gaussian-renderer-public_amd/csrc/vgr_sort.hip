@@ -148,8 +148,11 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
   constexpr int kWaves = kBlockThreads / kWave;
   constexpr int kWaveRun = kSortTile / kWaves;  // contiguous keys ranked by one wave
   constexpr uint32_t kMask = kRadix - 1;
-  __shared__ uint64_t s_keys[kSortTile];
-  __shared__ int32_t s_values[kSortTile];
+  // Keys and then values pass through the same LDS buffer (two phases), which
+  // halves the pass's LDS footprint and doubles its occupancy.
+  __shared__ uint64_t s_stage[kSortTile];
+  uint64_t* const s_keys = s_stage;
+  int32_t* const s_values = reinterpret_cast<int32_t*>(s_stage);
   __shared__ uint32_t s_count[kWaves][kRadix];
   __shared__ uint32_t s_local_start[kRadix];
   __shared__ int64_t s_global_start[kRadix];
@@ -237,15 +240,13 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
   }
   __syncthreads();
 
-  // Scatter into LDS in block-local sorted order.
+  // Phase 1: keys into LDS in block-local sorted order (positions kept for phase 2).
+  uint32_t positions[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int local = wave * kWaveRun + r * kWave + lane;
-    if (local < valid_count) {
-      const uint32_t position = s_local_start[digits[r]] + s_count[wave][digits[r]] + ranks[r];
-      s_keys[position] = keys[r];
-      s_values[position] = values[r];
-    }
+    positions[r] = s_local_start[digits[r]] + s_count[wave][digits[r]] + ranks[r];
+    if (local < valid_count) s_keys[positions[r]] = keys[r];
   }
 
   // Decoupled look-back for the global offset of every digit.
@@ -278,15 +279,35 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
   }
   __syncthreads();
 
-  for (int j = threadIdx.x; j < valid_count; j += kBlockThreads) {
-    const uint64_t key = s_keys[j];
-    const uint32_t digit = static_cast<uint32_t>(compressed_key(key, kp) >> shift) & kMask;
-    const int64_t out = s_global_start[digit] + j;
-    // Always true for a consistent histogram; keeps a corrupted input in bounds.
-    if (out >= 0 && out < capacity) {
-      keys_out[out] = key;
-      values_out[out] = s_values[j];
+  // Keys out; each thread remembers where its LDS slots go for the values.
+  constexpr int kSlots = kSortTile / kBlockThreads;
+  int64_t destinations[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    const int j = k * kBlockThreads + threadIdx.x;
+    destinations[k] = -1;
+    if (j < valid_count) {
+      const uint64_t key = s_keys[j];
+      const uint32_t digit = static_cast<uint32_t>(compressed_key(key, kp) >> shift) & kMask;
+      const int64_t out = s_global_start[digit] + j;
+      // Always true for a consistent histogram; keeps a corrupted input in bounds.
+      if (out >= 0 && out < capacity) {
+        keys_out[out] = key;
+        destinations[k] = out;
+      }
     }
+  }
+  __syncthreads();  // keys read out of LDS
+  // Phase 2: values through the same LDS buffer.
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int local = wave * kWaveRun + r * kWave + lane;
+    if (local < valid_count) s_values[positions[r]] = values[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kSlots; ++k) {
+    if (destinations[k] >= 0) values_out[destinations[k]] = s_values[k * kBlockThreads + threadIdx.x];
   }
 }
 
