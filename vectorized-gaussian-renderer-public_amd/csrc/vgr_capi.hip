@@ -140,8 +140,9 @@ int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
     set_error("persistent_composite must be in [0, 2^20]; got %d", options.persistent_composite);
     return VGR_E_INVALID_ARGUMENT;
   }
-  if (options.sort_tile_items != 0 && options.sort_tile_items != 8 && options.sort_tile_items != 16) {
-    set_error("sort_tile_items must be 0 (default), 8 or 16; got %d", options.sort_tile_items);
+  if (options.sort_tile_items != 0 && options.sort_tile_items != 8 && options.sort_tile_items != 12 &&
+      options.sort_tile_items != 16) {
+    set_error("sort_tile_items must be 0 (default), 8, 12 or 16; got %d", options.sort_tile_items);
     return VGR_E_INVALID_ARGUMENT;
   }
   const int group = resolve_cameras_per_block(options);

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
   int32_t* const s_values = reinterpret_cast<int32_t*>(s_stage);
   __shared__ uint32_t s_count[kWaves][kRadix];
   __shared__ uint32_t s_local_start[kRadix];
-  __shared__ int64_t s_global_start[kRadix];
+  __shared__ int32_t s_global_start[kRadix];  // capacities are below 2^30
   __shared__ uint32_t s_ticket;
 
   if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1u);
@@ -274,8 +274,8 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
       __hip_atomic_store(&flags[tile * kRadix + threadIdx.x], kFlagPrefix | (exclusive + total), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     }
-    s_global_start[threadIdx.x] = static_cast<int64_t>(digit_base[threadIdx.x]) + exclusive -
-                                  static_cast<int64_t>(s_local_start[threadIdx.x]);
+    s_global_start[threadIdx.x] = static_cast<int32_t>(static_cast<int64_t>(digit_base[threadIdx.x]) + exclusive -
+                                                       static_cast<int64_t>(s_local_start[threadIdx.x]));
   }
   __syncthreads();
 
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
     if (j < valid_count) {
       const uint64_t key = s_keys[j];
       const uint32_t digit = static_cast<uint32_t>(compressed_key(key, kp) >> shift) & kMask;
-      const int64_t out = s_global_start[digit] + j;
+      const int64_t out = static_cast<int64_t>(s_global_start[digit]) + j;
       // Always true for a consistent histogram; keeps a corrupted input in bounds.
       if (out >= 0 && out < capacity) {
         keys_out[out] = key;
@@ -413,6 +413,10 @@ void launch_pass(int items, const uint64_t* keys_in, const int32_t* values_in, u
                  hipStream_t stream) {
   if (items == 8) {
     hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 8>), dim3(static_cast<unsigned>(tiles)),
+                       dim3(kBlockThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
+                       kp, shift, digit_base, flags, ticket, status);
+  } else if (items == 12) {
+    hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 12>), dim3(static_cast<unsigned>(tiles)),
                        dim3(kBlockThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
                        kp, shift, digit_base, flags, ticket, status);
   } else {

@@ -173,8 +173,8 @@ class CustomHipBackend:
             raise ValueError("raster_batch must be 0 (default), 64, 128, 256 or 512.")
         if not 0 <= cameras_per_block <= 64:
             raise ValueError("cameras_per_block must be 0 (default) or in [1, 64].")
-        if sort_tile_items not in (0, 8, 16):
-            raise ValueError("sort_tile_items must be 0 (default), 8 or 16.")
+        if sort_tile_items not in (0, 8, 12, 16):
+            raise ValueError("sort_tile_items must be 0 (default), 8, 12 or 16.")
 
         self.max_visible_records = max_visible_records
         self.max_intersections = max_intersections
