@@ -130,8 +130,8 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_backward_kerne
   const int lane = threadIdx.x;
   const int wave = lane / kWave;
   const int wave_lane = lane & (kWave - 1);
-  const int pixel_x = tile_x * kTileSize + lane % kTileSize;
-  const int pixel_y = tile_y * kTileSize + lane / kTileSize;
+  const int pixel_x = tile_x * kTileSize + Lists::pixel_dx(lane);
+  const int pixel_y = tile_y * kTileSize + Lists::pixel_dy(lane);
   const bool valid_pixel = pixel_x < cam.width && pixel_y < cam.height;
   const float px = static_cast<float>(pixel_x) + 0.5f;
   const float py = static_cast<float>(pixel_y) + 0.5f;

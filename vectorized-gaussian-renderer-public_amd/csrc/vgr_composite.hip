@@ -72,8 +72,8 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
   const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
   const int lane = threadIdx.x;
   const int wave = lane / kWave;
-  const int pixel_x = tile_x * kTileSize + lane % kTileSize;
-  const int pixel_y = tile_y * kTileSize + lane / kTileSize;
+  const int pixel_x = tile_x * kTileSize + Lists::pixel_dx(lane);
+  const int pixel_y = tile_y * kTileSize + Lists::pixel_dy(lane);
   const bool valid_pixel = pixel_x < cam.width && pixel_y < cam.height;
   const float px = static_cast<float>(pixel_x) + 0.5f;
   const float py = static_cast<float>(pixel_y) + 0.5f;
@@ -173,214 +173,16 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
   }
 }
 
-// Wave-per-tile form for 16x16 tiles: one wave64 owns the tile and every lane
-// four pixels (column lane % 16, rows lane / 16 + 4k), so pixel k of every lane
-// lies in the k-th band of four rows.  A splat's reach mask over the bands (the
-// same conservative ellipse box as the per-wave lists) and the per-band "all
-// pixels done" flags are uniform branches; the splat's LDS record is read once
-// for four pixels.  Per pixel the blend sequence and arithmetic are those of
-// composite_tile, so the outputs are bit-identical.
-template <int kBatch, bool kFull, bool kCount, bool kCheckpoint>
-__device__ __forceinline__ void composite_tile_wave(const CompositeArgs& args, int64_t block) {
-  constexpr int kTileSize = 16;
-  constexpr int kBands = 4;
-  constexpr int kPerLane = kBatch / kWave;
-  static_assert(kBatch % kWave == 0, "batch must be a multiple of 64");
-  __shared__ f32x4 s_geometry[kBatch];
-  __shared__ f32x4 s_shading[kBatch];
-  __shared__ f32x4 s_colour[kBatch];
-  __shared__ uint16_t s_list[kBatch];
-
-  int active;
-  int64_t image_tile;
-  if (args.uniform_tiles > 0) {
-    active = static_cast<int>(block / args.uniform_tiles);
-    if (active >= args.active_count) return;
-    image_tile = block - static_cast<int64_t>(active) * args.uniform_tiles;
-  } else {
-    active = find_active_camera(args.table, args.active_count, block);
-    image_tile = block - args.table[active].active_tile_begin;
-    if (image_tile >= static_cast<int64_t>(args.table[active].tiles_x) * args.table[active].tiles_y) return;
-  }
-  const CameraParams& cam = args.table[active];
-  if (cam.camera < 0) return;
-  const int tile_x = static_cast<int>(image_tile % cam.tiles_x);
-  const int tile_y = static_cast<int>(image_tile / cam.tiles_x);
-  const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
-  const int lane = threadIdx.x;
-  const int pixel_x = tile_x * kTileSize + (lane % kTileSize);
-  const float px = static_cast<float>(pixel_x) + 0.5f;
-  const float rect_x0 = static_cast<float>(tile_x * kTileSize) + 0.5f;
-  const float rect_x1 = rect_x0 + static_cast<float>(kTileSize - 1);
-  const float rect_y_base = static_cast<float>(tile_y * kTileSize) + 0.5f;
-  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
-
-  float py[kBands], transmittance[kBands], red[kBands], green[kBands], blue[kBands], depth_sum[kBands];
-  float best_weight[kBands];
-  int best_gaussian[kBands], last_blended[kBands];
-  bool done[kBands];
-  uint32_t open_bands = 0u;  // uniform: bands with a pixel still blending
-#pragma unroll
-  for (int k = 0; k < kBands; ++k) {
-    const int pixel_y = tile_y * kTileSize + lane / kTileSize + 4 * k;
-    py[k] = static_cast<float>(pixel_y) + 0.5f;
-    transmittance[k] = 1.0f;
-    red[k] = green[k] = blue[k] = depth_sum[k] = 0.0f;
-    best_weight[k] = -1.0f;
-    best_gaussian[k] = -1;
-    last_blended[k] = 0;
-    done[k] = !(pixel_x < cam.width && pixel_y < cam.height);
-    if (__ballot(!done[k]) != 0ull) open_bands |= 1u << k;
-  }
-  int evaluations = 0;
-
-  const int64_t range_start = args.tile_starts[global_tile];
-  const int64_t range_end = args.tile_ends[global_tile];
-  if (range_start >= 0 && range_end > range_start && open_bands != 0u) {
-    f32x4 next_a[kPerLane] = {}, next_b[kPerLane] = {}, next_c[kPerLane] = {};
-    gather_batch<kWave, kBatch, kPerLane>(args.sorted_values, args.records, args.visible_capacity, range_start,
-                                          range_end, lane, next_a, next_b, next_c);
-    for (int64_t batch_start = range_start; batch_start < range_end; batch_start += kBatch) {
-      const int count = static_cast<int>(range_end - batch_start < kBatch ? range_end - batch_start : kBatch);
-      __syncthreads();  // previous batch fully consumed
-      uint32_t list_count = 0;
-#pragma unroll
-      for (int k = 0; k < kPerLane; ++k) {
-        const int slot = k * kWave + lane;
-        uint32_t bands = 0u;
-        if (slot < count) {
-          const StagedSplat staged = stage_record(next_a[k], next_b[k], next_c[k]);
-          s_geometry[slot] = staged.geometry;
-          s_shading[slot] = staged.shading;
-          s_colour[slot] = staged.colour;
-          float half_x, half_y;
-          if (splat_extent(next_a[k], next_b[k], staged.shading.z, half_x, half_y)) {
-            const float mx = next_a[k].x, my = next_a[k].y;
-            if (fabsf(mx - fminf(fmaxf(mx, rect_x0), rect_x1)) <= half_x) {
-#pragma unroll
-              for (int band = 0; band < kBands; ++band) {
-                const float y0 = rect_y_base + static_cast<float>(4 * band);
-                const float y1 = y0 + 3.0f;
-                if (fabsf(my - fminf(fmaxf(my, y0), y1)) <= half_y) bands |= 1u << band;
-              }
-            }
-          } else {
-            bands = (1u << kBands) - 1u;
-          }
-          bands &= open_bands;
-        }
-        const uint64_t vote = __ballot(bands != 0u);
-        if (bands != 0u) {
-          s_list[list_count + static_cast<uint32_t>(__popcll(vote & below))] =
-              static_cast<uint16_t>(slot | (bands << 12));
-        }
-        list_count += static_cast<uint32_t>(__popcll(vote));
-      }
-      __syncthreads();
-      if (batch_start + kBatch < range_end) {
-        gather_batch<kWave, kBatch, kPerLane>(args.sorted_values, args.records, args.visible_capacity,
-                                              batch_start + kBatch, range_end, lane, next_a, next_b, next_c);
-      }
-      const int position_base = static_cast<int>(batch_start - range_start);
-      for (uint32_t i = 0; i < list_count; ++i) {
-        const uint32_t entry = s_list[i];
-        const int item = static_cast<int>(entry & 0x1ffu);
-        const uint32_t bands = (entry >> 12) & open_bands;
-        if (bands == 0u) continue;
-        const f32x4 g = s_geometry[item];
-        const f32x4 sh = s_shading[item];
-        const float delta_x = px - g.x;
-#pragma unroll
-        for (int k = 0; k < kBands; ++k) {
-          if (((bands >> k) & 1u) == 0u) continue;  // uniform
-          if (kCount && !done[k]) ++evaluations;
-          const float delta_y = py[k] - g.y;
-          const float power = g.z * delta_x * delta_x + g.w * delta_x * delta_y + sh.x * delta_y * delta_y;
-          if (done[k] || power > 0.0f || power < sh.z) continue;
-          const float splat_alpha = fminf(VGR_MAX_ALPHA, sh.y * vgr_expf(power));
-          if (splat_alpha < VGR_ALPHA_THRESHOLD) continue;
-          const float next_transmittance = transmittance[k] * (1.0f - splat_alpha);
-          if (next_transmittance <= VGR_TRANSMITTANCE_THRESHOLD) {
-            done[k] = true;
-            continue;
-          }
-          const f32x4 col = s_colour[item];
-          const float weight = transmittance[k] * splat_alpha;
-          red[k] = fmaf(weight, col.x, red[k]);
-          green[k] = fmaf(weight, col.y, green[k]);
-          blue[k] = fmaf(weight, col.z, blue[k]);
-          if (kFull) {
-            depth_sum[k] = fmaf(weight, sh.w, depth_sum[k]);
-            if (weight > best_weight[k]) {
-              best_weight[k] = weight;
-              best_gaussian[k] = __float_as_int(col.w);
-            }
-          }
-          if (kCheckpoint) last_blended[k] = position_base + item + 1;
-          transmittance[k] = next_transmittance;
-        }
-        // Close bands whose every pixel has terminated (uniform).
-#pragma unroll
-        for (int k = 0; k < kBands; ++k) {
-          if (((bands >> k) & 1u) != 0u && __ballot(!done[k]) == 0ull) open_bands &= ~(1u << k);
-        }
-        if (open_bands == 0u) break;
-      }
-      if (open_bands == 0u) break;
-    }
-  }
-
-  if (kCount) {
-    int64_t total = evaluations;
-    for (int offset = 32; offset > 0; offset >>= 1) total += __shfl_xor(total, offset, kWave);
-    if (lane == 0 && total != 0) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(args.statistics), static_cast<unsigned long long>(total));
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < kBands; ++k) {
-    const int pixel_y = tile_y * kTileSize + lane / kTileSize + 4 * k;
-    if (!(pixel_x < cam.width && pixel_y < cam.height)) continue;
-    const int64_t pixel = cam.pixel_offset + static_cast<int64_t>(pixel_y) * cam.width + pixel_x;
-    const float accumulated_alpha = 1.0f - transmittance[k];
-    float r = red[k], gr = green[k], b = blue[k];
-    if (args.output_srgb) {
-      r = vgr_linear_to_srgb(r);
-      gr = vgr_linear_to_srgb(gr);
-      b = vgr_linear_to_srgb(b);
-    }
-    args.rgb[pixel * 3 + 0] = r;
-    args.rgb[pixel * 3 + 1] = gr;
-    args.rgb[pixel * 3 + 2] = b;
-    if (kFull) {
-      args.alpha[pixel] = accumulated_alpha;
-      args.depth[pixel] =
-          accumulated_alpha > 1.0e-8f ? depth_sum[k] / accumulated_alpha : __int_as_float(0x7f800000);
-      const bool labelled = best_weight[k] >= 0.0f && accumulated_alpha >= args.semantic_min_alpha &&
-                            best_gaussian[k] >= 0 && best_gaussian[k] < args.gaussian_count;
-      args.semantic[pixel] = labelled ? args.semantic_ids[best_gaussian[k]] : -1;
-    }
-    if (kCheckpoint) {
-      args.final_transmittance[pixel] = transmittance[k];
-      args.last_blended[pixel] = last_blended[k];
-    }
-  }
-}
-
-template <int kBatch, bool kFull, bool kCount, bool kCheckpoint>
-__global__ __launch_bounds__(kWave) void composite_wave_kernel(CompositeArgs args) {
-  composite_tile_wave<kBatch, kFull, kCount, kCheckpoint>(args, blockIdx.x);
-}
-
 // Minimum waves per SIMD for 16x16 tiles (launch-bounds occupancy target): eight
 // means at most 64 VGPRs.  The blend loop is latency-bound, so waves in flight
 // pay more than registers.  The evaluation-counting and persistent variants
 // need a few more registers and take seven and six (no spills).
-template <int kTileSize, bool kCount, bool kPersistent = false>
-constexpr int kCompositeWavesPerSimd = kTileSize == 16 ? (kPersistent ? 6 : (kCount ? 7 : 8)) : 1;
+template <int kTileSize, int kBatch, bool kCount, bool kPersistent = false>
+constexpr int kCompositeWavesPerSimd =
+    kTileSize != 16 ? 1 : (kBatch >= 512 ? 4 : (kPersistent ? 6 : (kCount ? 7 : 8)));  // 512: LDS-limited
 
 template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint>
-__global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTileSize, kCount>)) void composite_kernel(
+__global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTileSize, kBatch, kCount>)) void composite_kernel(
     CompositeArgs args) {
   composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint>(args, blockIdx.x);
 }
@@ -389,7 +191,7 @@ __global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTile
 // a device counter until the queue is empty; every workgroup exits once the
 // counter passes the item count.
 template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint>
-__global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTileSize, kCount, true>)) void
+__global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTileSize, kBatch, kCount, true>)) void
 composite_persistent_kernel(CompositeArgs args) {
   __shared__ int64_t s_item;
   for (;;) {

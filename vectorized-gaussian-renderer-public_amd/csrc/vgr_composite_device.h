@@ -115,8 +115,17 @@ template <int kTileSize, int kBatch>
 struct WaveLists {
   static constexpr int kThreads = kTileSize * kTileSize;
   static constexpr int kWaves = kThreads / kWave;
-  static constexpr int kRowsPerWave = kWave / kTileSize;
   static constexpr int kPerThread = (kBatch + kThreads - 1) / kThreads;
+  // Each wave owns an 8x8 block of the tile (16x16 tiles: four quadrants), the
+  // squarest shape of 64 pixels, so a splat's box reaches as few waves as possible.
+  static constexpr int kBlock = 8;
+  static constexpr int kBlocksX = kTileSize / kBlock;
+  __host__ __device__ static constexpr int pixel_dx(int lane) {
+    return ((lane / kWave) % kBlocksX) * kBlock + (lane & (kBlock - 1));
+  }
+  __host__ __device__ static constexpr int pixel_dy(int lane) {
+    return ((lane / kWave) / kBlocksX) * kBlock + ((lane & (kWave - 1)) / kBlock);
+  }
   f32x4 geometry[kBatch];
   f32x4 shading[kBatch];
   f32x4 colour[kBatch];
@@ -140,6 +149,7 @@ __device__ __forceinline__ void build_wave_lists(WaveLists<kTileSize, kBatch>& s
                                                  const f32x4 (&nc)[WaveLists<kTileSize, kBatch>::kPerThread],
                                                  int count, int lane, float rect_x0, float rect_x1,
                                                  float rect_y_base) {
+  (void)rect_x1;
   using L = WaveLists<kTileSize, kBatch>;
   constexpr int kThreads = L::kThreads;
   constexpr int kWaves = L::kWaves;
@@ -159,15 +169,13 @@ __device__ __forceinline__ void build_wave_lists(WaveLists<kTileSize, kBatch>& s
       float half_x, half_y;
       if (splat_extent(na[k], nb[k], staged.shading.z, half_x, half_y)) {
         const float mx = na[k].x, my = na[k].y;
-        const float gap_x = fabsf(mx - fminf(fmaxf(mx, rect_x0), rect_x1));
-        if (gap_x <= half_x) {
 #pragma unroll
-          for (int w = 0; w < kWaves; ++w) {
-            const float y0 = rect_y_base + static_cast<float>(w * L::kRowsPerWave);
-            const float y1 = y0 + static_cast<float>(L::kRowsPerWave - 1);
-            const float gap_y = fabsf(my - fminf(fmaxf(my, y0), y1));
-            if (gap_y <= half_y) wave_bits[k] |= 1u << w;
-          }
+        for (int w = 0; w < kWaves; ++w) {
+          const float x0 = rect_x0 + static_cast<float>((w % L::kBlocksX) * L::kBlock);
+          const float y0 = rect_y_base + static_cast<float>((w / L::kBlocksX) * L::kBlock);
+          const float gap_x = fabsf(mx - fminf(fmaxf(mx, x0), x0 + static_cast<float>(L::kBlock - 1)));
+          const float gap_y = fabsf(my - fminf(fmaxf(my, y0), y0 + static_cast<float>(L::kBlock - 1)));
+          if (gap_x <= half_x && gap_y <= half_y) wave_bits[k] |= 1u << w;
         }
       } else {
         wave_bits[k] = (1u << kWaves) - 1u;  // no certain box: every wave evaluates it
