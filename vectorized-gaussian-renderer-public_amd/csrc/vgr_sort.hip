@@ -65,12 +65,28 @@ __global__ __launch_bounds__(kBlockThreads) void sort_histogram_kernel(const uin
   __syncthreads();
   const int64_t n = element_count(counters, capacity);
   const uint32_t mask = static_cast<uint32_t>(radix - 1);
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x; i < n;
-       i += static_cast<int64_t>(gridDim.x) * kBlockThreads) {
-    const uint64_t ck = compressed_key(keys[i], kp);
+  // Keys are read coalesced (lane i takes key i of a 64-key run); per pass, a lane
+  // adds to LDS only if it starts a run of equal digits among its wave's keys, and
+  // adds the run's length.  Emission writes a splat's tiles next to each other,
+  // so the high (tile) digits come in long runs that would otherwise serialize up
+  // to 64 lanes on one LDS address.
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t above = lane == kWave - 1 ? 0ull : (~0ull << (lane + 1));
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x;
+       i - lane < n; i += static_cast<int64_t>(gridDim.x) * kBlockThreads) {
+    const bool valid = i < n;
+    const uint64_t ck = valid ? compressed_key(keys[i], kp) : 0ull;
+    const uint64_t live = __ballot(valid);
     for (int p = 0; p < passes; ++p) {
       const uint32_t digit = static_cast<uint32_t>(ck >> (p * digit_bits)) & mask;
-      atomicAdd(&local[p * radix + digit], 1u);
+      const uint32_t before = __shfl_up(digit, 1, kWave);
+      const bool head = valid && (lane == 0 || digit != before);
+      const uint64_t heads = __ballot(head);
+      if (head) {
+        const uint64_t later = (heads & above) | ~live;  // next run head, or the first invalid lane
+        const int end = later == 0ull ? kWave : __ffsll(static_cast<unsigned long long>(later)) - 1;
+        atomicAdd(&local[p * radix + digit], static_cast<uint32_t>(end - lane));
+      }
     }
   }
   __syncthreads();
