@@ -334,15 +334,21 @@ __global__ __launch_bounds__(kBlockThreads) void ranges_kernel(const uint64_t* _
                                                                int32_t* __restrict__ tile_ends,
                                                                int64_t* __restrict__ counters) {
   const int64_t n = element_count(counters_in, capacity);
+  const int lane = threadIdx.x & (kWave - 1);
   int64_t ends = 0;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x; i < n;
+  // One key per lane; neighbours come from the adjacent lanes (lanes 0 and 63
+  // read theirs), so every key is read from memory about once.
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x; i - lane < n;
        i += static_cast<int64_t>(gridDim.x) * kBlockThreads) {
-    const uint64_t tile = keys[i] >> 32;
-    if (tile >= static_cast<uint64_t>(global_tiles)) continue;
-    const bool begins = i == 0 || (keys[i - 1] >> 32) != tile;
-    const bool finishes = i + 1 == n || (keys[i + 1] >> 32) != tile;
-    if (begins) tile_starts[tile] = static_cast<int32_t>(i);
-    if (finishes) {
+    const bool valid = i < n;
+    const uint32_t tile = valid ? static_cast<uint32_t>(keys[i] >> 32) : 0xffffffffu;
+    uint32_t previous = __shfl_up(tile, 1, kWave);
+    uint32_t next = __shfl_down(tile, 1, kWave);
+    if (lane == 0) previous = i > 0 && valid ? static_cast<uint32_t>(keys[i - 1] >> 32) : 0xfffffffeu;
+    if (lane == kWave - 1 || i + 1 >= n) next = i + 1 < n ? static_cast<uint32_t>(keys[i + 1] >> 32) : 0xfffffffeu;
+    if (!valid || tile >= static_cast<uint64_t>(global_tiles)) continue;
+    if (i == 0 || previous != tile) tile_starts[tile] = static_cast<int32_t>(i);
+    if (i + 1 == n || next != tile) {
       tile_ends[tile] = static_cast<int32_t>(i + 1);
       ++ends;
     }
