@@ -269,23 +269,41 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
   if (threadIdx.x < kRadix) {
     uint32_t exclusive = 0;
     if (tile > 0) {
+      // The flags of kWindow predecessors are loaded together (independent loads,
+      // one memory round trip), then consumed in order up to the first inclusive
+      // prefix; a flag not yet published restarts the window from it.
+      constexpr int kWindow = 4;
       int64_t predecessor = tile - 1;
       uint32_t spins = 0;
-      while (true) {
-        const uint32_t flag = __hip_atomic_load(&flags[predecessor * kRadix + threadIdx.x], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t state = flag & ~kFlagValueMask;
-        if (state == 0u) {
+      bool found = false;
+      while (!found) {
+        uint32_t window[kWindow];
+#pragma unroll
+        for (int w = 0; w < kWindow; ++w) {
+          window[w] = predecessor - w >= 0
+                          ? __hip_atomic_load(&flags[(predecessor - w) * kRadix + threadIdx.x], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT)
+                          : 0u;
+        }
+        int consumed = 0;
+#pragma unroll
+        for (int w = 0; w < kWindow; ++w) {
+          if (found || consumed < w) continue;  // stopped at an earlier entry
+          const uint32_t state = window[w] & ~kFlagValueMask;
+          if (state == 0u || predecessor - w < 0) continue;  // not published yet
+          exclusive += window[w] & kFlagValueMask;
+          consumed = w + 1;
+          if (state == kFlagPrefix || predecessor - w == 0) found = true;
+        }
+        if (found) break;
+        predecessor -= consumed;
+        if (consumed == 0) {
           if (++spins > kSpinLimit) {
             atomicAdd(&status[0], 1u);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
-          continue;
         }
-        exclusive += flag & kFlagValueMask;
-        if (state == kFlagPrefix || predecessor == 0) break;
-        --predecessor;
       }
       __hip_atomic_store(&flags[tile * kRadix + threadIdx.x], kFlagPrefix | (exclusive + total), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
