@@ -106,22 +106,12 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
       }
       if (!done) {
         const uint32_t list_count = s.count(wave);
-        for (uint32_t i = 0; i < list_count; ++i) {
-          const int item = s.list[wave][i];
-          if (kCount) ++evaluations;
-          const f32x4 g = s.geometry[item];
-          const float delta_x = px - g.x;
-          const float delta_y = py - g.y;
-          const f32x4 sh = s.shading[item];
-          const float power = g.z * delta_x * delta_x + g.w * delta_x * delta_y + sh.x * delta_y * delta_y;
-          if (power > 0.0f || power < sh.z) continue;
-          const float splat_alpha = fminf(VGR_MAX_ALPHA, sh.y * vgr_expf(power));
-          if (splat_alpha < VGR_ALPHA_THRESHOLD) continue;
+        const int position_base = static_cast<int>(batch_start - range_start);
+        // Blends one evaluated splat; returns true when the pixel terminates.
+        auto blend = [&](int item, const f32x4& sh, float splat_alpha) -> bool {
+          if (splat_alpha < VGR_ALPHA_THRESHOLD) return false;
           const float next_transmittance = transmittance * (1.0f - splat_alpha);
-          if (next_transmittance <= VGR_TRANSMITTANCE_THRESHOLD) {
-            done = true;
-            break;
-          }
+          if (next_transmittance <= VGR_TRANSMITTANCE_THRESHOLD) return true;
           const f32x4 col = s.colour[item];
           const float weight = transmittance * splat_alpha;
           red = fmaf(weight, col.x, red);
@@ -134,8 +124,50 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
               best_gaussian = __float_as_int(col.w);
             }
           }
-          if (kCheckpoint) last_blended = static_cast<int>(batch_start - range_start) + item + 1;
+          if (kCheckpoint) last_blended = position_base + item + 1;
           transmittance = next_transmittance;
+          return false;
+        };
+        uint32_t i = 0;
+        // kUnroll list entries per iteration: their powers and exponentials are
+        // independent and overlap (the loop is latency-bound); blending stays in
+        // list order, so the per-pixel result is unchanged.
+        constexpr int kUnroll = 4;
+        for (; i + kUnroll <= list_count; i += kUnroll) {
+          int items[kUnroll];
+          f32x4 shading[kUnroll];
+          float alphas[kUnroll];
+          bool passes[kUnroll];
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) items[u] = s.list[wave][i + u];
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) {
+            const f32x4 g = s.geometry[items[u]];
+            shading[u] = s.shading[items[u]];
+            const float dx = px - g.x, dy = py - g.y;
+            const float power = g.z * dx * dx + g.w * dx * dy + shading[u].x * dy * dy;
+            passes[u] = !(power > 0.0f || power < shading[u].z);
+            alphas[u] = passes[u] ? fminf(VGR_MAX_ALPHA, shading[u].y * vgr_expf(power)) : 0.0f;
+          }
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) {
+            if (done) break;
+            if (kCount) ++evaluations;
+            if (passes[u] && blend(items[u], shading[u], alphas[u])) done = true;
+          }
+          if (done) break;
+        }
+        while (!done && i < list_count) {
+          const int item = s.list[wave][i++];
+          if (kCount) ++evaluations;
+          const f32x4 g = s.geometry[item];
+          const f32x4 sh = s.shading[item];
+          const float delta_x = px - g.x;
+          const float delta_y = py - g.y;
+          const float power = g.z * delta_x * delta_x + g.w * delta_x * delta_y + sh.x * delta_y * delta_y;
+          if (!(power > 0.0f || power < sh.z)) {
+            if (blend(item, sh, fminf(VGR_MAX_ALPHA, sh.y * vgr_expf(power)))) done = true;
+          }
         }
       }
       if (!__syncthreads_or(done ? 0 : 1)) break;
