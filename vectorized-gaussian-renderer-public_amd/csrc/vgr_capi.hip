@@ -61,7 +61,7 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   layout.sort_key_bits = layout.tile_bits + layout.depth_bits;
   layout.sort_digit_bits = resolve_digit_bits(options);
   layout.sort_passes = (layout.sort_key_bits + layout.sort_digit_bits - 1) / layout.sort_digit_bits;
-  layout.sort_tile = kBlockThreads * (options.sort_tile_items == 0 ? kDefaultSortItems : options.sort_tile_items);
+  layout.sort_tile = kSortThreads * (options.sort_tile_items == 0 ? kDefaultSortItems : options.sort_tile_items);
   layout.sort_tiles = (intersection_capacity + layout.sort_tile - 1) / layout.sort_tile;
   const int64_t radix = 1 << layout.sort_digit_bits;
 

@@ -72,7 +72,8 @@ struct ScratchLayout {
   uint32_t depth_base;     // float bits of the near plane
 };
 
-constexpr int kDefaultSortItems = 16;  // keys per thread per sort tile (tile = 256 * items keys)
+constexpr int kSortThreads = 512;      // threads per onesweep workgroup (8 waves)
+constexpr int kDefaultSortItems = 16;  // keys per thread per sort tile (tile = kSortThreads * items keys)
 
 // Host helpers implemented in vgr_capi.hip.
 void set_error(const char* fmt, ...);

