@@ -68,7 +68,7 @@ def library() -> ctypes.CDLL:
         lib.oracle_precompute_covariances.restype = ctypes.c_int32
         lib.oracle_precompute_covariances.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                                       ctypes.c_void_p]
-        for name in ("oracle_expf", "oracle_logf", "oracle_srgb"):
+        for name in ("oracle_expf", "oracle_expf_nonpositive", "oracle_logf", "oracle_srgb"):
             getattr(lib, name).restype = ctypes.c_float
             getattr(lib, name).argtypes = [ctypes.c_float]
         lib.oracle_sh_colour.restype = None
@@ -99,6 +99,10 @@ def covariances(scales, rotations) -> np.ndarray:
 
 def expf(x: float) -> float:
     return float(library().oracle_expf(x))
+
+
+def expf_nonpositive(x: float) -> float:
+    return float(library().oracle_expf_nonpositive(x))
 
 
 def logf(x: float) -> float:

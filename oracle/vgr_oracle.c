@@ -35,6 +35,7 @@ static __thread char g_error[256];
 const char* oracle_last_error(void) { return g_error; }
 
 float oracle_expf(float x) { return vgr_expf(x); }
+float oracle_expf_nonpositive(float x) { return vgr_expf_nonpositive(x); }
 float oracle_logf(float x) { return vgr_logf(x); }
 float oracle_srgb(float x) { return vgr_linear_to_srgb(x); }
 

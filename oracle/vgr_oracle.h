@@ -90,6 +90,7 @@ int32_t oracle_render(const oracle_scene* scene, const oracle_cameras* cameras,
                       const oracle_options* options, oracle_outputs* outputs);
 /* Elementary functions, exported for accuracy tests. */
 float oracle_expf(float x);
+float oracle_expf_nonpositive(float x);
 float oracle_logf(float x);
 float oracle_srgb(float x);
 /* View-dependent colour of one Gaussian seen from origin (scene coordinates). */

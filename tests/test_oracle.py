@@ -70,6 +70,16 @@ def test_expf_relative_error_bound():
     assert np.all(wide_rel < 1.7e-7 + np.abs(wide) * 7.5e-8)
 
 
+def test_expf_nonpositive_is_expf_bit_for_bit():
+    """The compositor's variant drops only the upper clamp, a no-op for x <= 0 and NaN."""
+    negatives = np.arange(0x80000000, 0xC3000000, 4099, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    specials = np.array([0.0, -0.0, -87.33, -87.34, -88.0, -150.0, -1.0e30, -np.inf, np.nan], dtype=np.float32)
+    for x in np.concatenate([negatives, specials]):
+        a = np.float32(oracle.expf(float(x)))
+        b = np.float32(oracle.expf_nonpositive(float(x)))
+        assert a.view(np.uint32) == b.view(np.uint32) or (np.isnan(a) and np.isnan(b)), (float(x), a, b)
+
+
 def test_logf_error_bound():
     xs = np.concatenate([np.geomspace(1.0, 255.0, 50_001), np.geomspace(1e-30, 1e30, 20_001)]).astype(np.float32)
     values = np.array([oracle.logf(float(x)) for x in xs[::5]], dtype=np.float64)

@@ -438,7 +438,9 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
       const f32x4 col = s_colour[item];
       const int position = position_base + item;
       const float delta_x = px - g.x;
-      float d_mx = 0.0f, d_my = 0.0f, d_c0 = 0.0f, d_c1 = 0.0f, d_c2 = 0.0f, d_o = 0.0f;
+      // Per-lane moments of q = dL/d alpha * G over the lane's pixels; the mean and
+      // conic gradients are linear in them and are formed once per splat below.
+      float m_x = 0.0f, m_y = 0.0f, m_xx = 0.0f, m_xy = 0.0f, m_yy = 0.0f, d_o = 0.0f;
       float d_z = 0.0f, d_r = 0.0f, d_g = 0.0f, d_b = 0.0f;
       bool any = false;
 #pragma unroll
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
         // From here the band runs without divergent branches: lanes that do not
         // blend the splat take alpha = 0 (keep = 1, nothing changes) and zero
         // gradients, so every lane executes the same straight-line code.
-        const float exact = vgr_expf(power);
+        const float exact = vgr_expf_nonpositive(power);  // unused where power > 0 (not reachable)
         const float splat_alpha = fminf(VGR_MAX_ALPHA, sh.y * exact);
         const bool blended = reachable && splat_alpha >= VGR_ALPHA_THRESHOLD;
         if (__ballot(blended) == 0ull) continue;
@@ -474,31 +476,38 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
         const float d_alpha = (own - behind[k]) * before;
         behind[k] = alpha * own + keep * behind[k];
         transmittance[k] = before;
-        d_o += d_alpha * gaussian;
-        const float d_power = d_alpha * gaussian * sh.y;  // = d_alpha * alpha below the cap
-        d_mx -= d_power * (2.0f * g.z * delta_x + g.w * delta_y);
-        d_my -= d_power * (g.w * delta_x + 2.0f * sh.x * delta_y);
-        d_c0 -= 0.5f * d_power * delta_x * delta_x;
-        d_c1 -= d_power * delta_x * delta_y;
-        d_c2 -= 0.5f * d_power * delta_y * delta_y;
+        const float q = d_alpha * gaussian;  // dL/d power = q * opacity (= d_alpha * alpha below the cap)
+        d_o += q;
+        const float q_x = q * delta_x;
+        const float q_y = q * delta_y;
+        m_x += q_x;
+        m_y += q_y;
+        m_xx += q_x * delta_x;
+        m_xy += q_x * delta_y;
+        m_yy += q_y * delta_y;
         }
       }
       if (__ballot(any) == 0ull) continue;
       // Wave sums land in lane 63, which parks them in LDS; after the batch every
       // lane owns one slot and issues its atomics (one vector atomic per value).
-      d_mx = wave_sum_to_last_lane(d_mx);
-      d_my = wave_sum_to_last_lane(d_my);
-      d_c0 = wave_sum_to_last_lane(d_c0);
-      d_c1 = wave_sum_to_last_lane(d_c1);
-      d_c2 = wave_sum_to_last_lane(d_c2);
+      m_x = wave_sum_to_last_lane(m_x);
+      m_y = wave_sum_to_last_lane(m_y);
+      m_xx = wave_sum_to_last_lane(m_xx);
+      m_xy = wave_sum_to_last_lane(m_xy);
+      m_yy = wave_sum_to_last_lane(m_yy);
       d_o = wave_sum_to_last_lane(d_o);
       d_z = wave_sum_to_last_lane(d_z);
       d_r = wave_sum_to_last_lane(d_r);
       d_g = wave_sum_to_last_lane(d_g);
       d_b = wave_sum_to_last_lane(d_b);
       if (lane == kWave - 1) {
-        s_partial[item][0] = f32x4{d_mx, d_my, d_c0, d_c1};
-        s_partial[item][1] = f32x4{d_c2, d_o, d_z, d_r};
+#pragma clang fp contract(fast)
+        // power = g.z dx^2 + g.w dx dy + sh.x dy^2 with dx = px - mean_x, dy = py - mean_y.
+        const float o = sh.y;
+        const float d_mx = -o * (2.0f * g.z * m_x + g.w * m_y);
+        const float d_my = -o * (g.w * m_x + 2.0f * sh.x * m_y);
+        s_partial[item][0] = f32x4{d_mx, d_my, -0.5f * o * m_xx, -o * m_xy};
+        s_partial[item][1] = f32x4{-0.5f * o * m_yy, d_o, d_z, d_r};
         s_partial[item][2] = f32x4{d_g, d_b, 0.0f, 0.0f};
       }
       if (lane == 0) s_blended[item] = 1u;

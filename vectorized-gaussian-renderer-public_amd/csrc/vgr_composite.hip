@@ -147,7 +147,7 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
             const float dx = px - g.x, dy = py - g.y;
             const float power = g.z * dx * dx + g.w * dx * dy + shading[u].x * dy * dy;
             passes[u] = !(power > 0.0f || power < shading[u].z);
-            alphas[u] = passes[u] ? fminf(VGR_MAX_ALPHA, shading[u].y * vgr_expf(power)) : 0.0f;
+            alphas[u] = passes[u] ? fminf(VGR_MAX_ALPHA, shading[u].y * vgr_expf_nonpositive(power)) : 0.0f;
           }
 #pragma unroll
           for (int u = 0; u < kUnroll; ++u) {
@@ -166,7 +166,7 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
           const float delta_y = py - g.y;
           const float power = g.z * delta_x * delta_x + g.w * delta_x * delta_y + sh.x * delta_y * delta_y;
           if (!(power > 0.0f || power < sh.z)) {
-            if (blend(item, sh, fminf(VGR_MAX_ALPHA, sh.y * vgr_expf(power)))) done = true;
+            if (blend(item, sh, fminf(VGR_MAX_ALPHA, sh.y * vgr_expf_nonpositive(power)))) done = true;
           }
         }
       }
