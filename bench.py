@@ -34,7 +34,8 @@ STAGES = ("projection+emission", "sort", "tile_ranges", "composite")
 
 
 def profiled_traffic(kernel: str) -> int | None:
-    """HBM bytes per launch of ``kernel`` from the committed PMC summary of this workload."""
+    """HBM bytes per launch of ``kernel`` (per render for a ``stage:<name>`` entry) from the committed
+    PMC summary of this workload."""
     path = ROOT / "profiles" / "c2_plain_forward_hbm_bytes.md"
     try:
         for line in path.read_text().splitlines():
@@ -159,16 +160,18 @@ def main() -> int:
     value = pixels / 1.0e6 / (ms_per_step / 1000.0)
 
     # Algorithmic work per launch (one render of the batch).
-    pairs = args.cameras * args.gaussians
     visible = counters["visible_gaussians"]
     intersections = counters["tile_intersections"]
     bands = 16
     last_allocation = backend.capacity_stats["last_allocation"] or {}
     passes = int(last_allocation.get("sort_radix_passes_expected", 6))
     global_tiles = args.cameras * math.ceil(width / 16) * math.ceil(height / 16)
+    # Minimum HBM traffic of each stage: the scene is read once (mean, opacity,
+    # covariance, SH), every record and key/value written once; repeated reads of a
+    # Gaussian by the cameras of a group are cache traffic, not algorithmic bytes.
     work_bytes = {
-        "projection+emission": pairs * 16 + visible * (24 + 12 * bands) + visible * RECORD_BYTES
-        + visible * 20 + intersections * 12,
+        "projection+emission": args.gaussians * (16 + 24 + 12 * bands) + visible * RECORD_BYTES
+        + intersections * 12,
         "sort": intersections * (8 + 24 * passes),
         "tile_ranges": intersections * 8 + global_tiles * 8,
     }
@@ -185,7 +188,9 @@ def main() -> int:
     else:
         achieved = work_bytes[dominant] / (stage_ms[dominant] / 1000.0) / 1.0e9
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None}
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                    "traffic": profiled_traffic(f"stage:{dominant}") if default_workload else None,
+                    "traffic_source": "profiles/c2_plain_forward_hbm_bytes.md (FETCH_SIZE, WRITE_SIZE, copy-scaled)"}
     stages = {}
     for name in STAGES:
         entry = {"ms": round(stage_ms[name], 4)}

@@ -164,7 +164,7 @@ typedef struct vgr_options {
   void* const* stage_events;        /* 5 hipEvent_t recorded on the stream: start, after projection +
                                        emission, after sort, after tile ranges, after compositing */
   int32_t stage_event_count;        /* 0 or 5 */
-  int32_t sort_tile_items;          /* keys per thread per sort tile: 0 = default (16); 8, 12 or 16 */
+  int32_t sort_tile_items;          /* keys per thread of a 512-thread sort tile: 0 = default (16); 8, 12 or 16 */
   int64_t* statistics;              /* int64[4], zeroed then filled: [0] compositor (pixel, Gaussian)
                                        evaluations; [1] sort look-back spin overflows; [2] projection
                                        work items kept by chunk culling */
