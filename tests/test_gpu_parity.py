@@ -42,6 +42,20 @@ def test_sh3_multi_camera_deterministic_matches_oracle():
     compare_ordering(backend, expected)
 
 
+@pytest.mark.parametrize("items,digit_bits", [(8, 8), (12, 8), (16, 8), (16, 6), (12, 7)])
+def test_sort_tile_sizes_and_digit_widths_keep_order_bit_exact(items, digit_bits):
+    """Sort tiles of 512 x {8, 12, 16} keys with 6-8-bit digits: several tiles and a partial
+    last one (140,757 intersections), ordering bit-exact against the oracle."""
+    scene = synthetic.gaussian_scene(50_000, seed=31, sh_degree=0)
+    rig = synthetic.orbit_rig(2, 320, 240, 277.0, seed=31, radius=(9.0, 12.0))
+    gpu, backend = render_service([(0, scene)], rig, [0, 0], sort_tile_items=items, sort_digit_bits=digit_bits)
+    expected = render_oracle([(0, scene)], rig, [0, 0], want_sorted=True)
+    n = backend.read_counters()["tile_intersections"]
+    assert n > 3 * 512 * items and n % (512 * items) != 0, n
+    _print(f"sort {items}x512 d{digit_bits}", compare_images(gpu, expected))
+    compare_ordering(backend, expected)
+
+
 def test_sh1_tile8_antialiased_matches_oracle():
     scene = synthetic.gaussian_scene(40_000, seed=11, sh_degree=1)
     rig = synthetic.orbit_rig(3, 200, 150, 173.0, seed=11, radius=(9.0, 12.0))
