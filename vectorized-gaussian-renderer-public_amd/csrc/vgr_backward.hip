@@ -850,8 +850,10 @@ __global__ __launch_bounds__(kBlockThreads) void projection_backward_kernel(Proj
   const int last = min(fw.active_count, first + fw.cameras_per_block);
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
+  // Double-buffered by camera parity (as in the forward's write kernel): one barrier per camera.
   __shared__ int wave_totals[2][kBlockThreads / kWave];
   __shared__ float s_pose[kBlockThreads / kWave][kCameraGrads];
+  int parity = 0;
   for (int a = first; a < last; ++a) {
     if (((mask >> (a - first)) & 1ull) == 0) continue;
     const CameraParams& cam = fw.table[a];
@@ -862,15 +864,15 @@ __global__ __launch_bounds__(kBlockThreads) void projection_backward_kernel(Proj
     int inclusive_visible = visible ? 1 : 0;
     int unused = 0;
     wave_inclusive_scan2(inclusive_visible, unused);
-    if (lane == kWave - 1) wave_totals[0][wave] = inclusive_visible;
+    if (lane == kWave - 1) wave_totals[parity][wave] = inclusive_visible;
     __syncthreads();
     int64_t prefix_visible = 0, total_visible = 0;
 #pragma unroll
     for (int w = 0; w < kBlockThreads / kWave; ++w) {
-      if (w < wave) prefix_visible += wave_totals[0][w];
-      total_visible += wave_totals[0][w];
+      if (w < wave) prefix_visible += wave_totals[parity][w];
+      total_visible += wave_totals[parity][w];
     }
-    __syncthreads();
+    parity ^= 1;
     float pose[kCameraGrads];
     if (kPose) {
 #pragma unroll

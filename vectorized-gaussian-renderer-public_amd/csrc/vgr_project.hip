@@ -203,7 +203,7 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
 }
 
 template <int kDegree>
-__global__ __launch_bounds__(kBlockThreads) void projection_write_kernel(
+__global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
     ProjectionArgs args, const int64_t* __restrict__ block_bases, vgr_record* __restrict__ records,
     int32_t* __restrict__ radii, uint64_t* __restrict__ keys, int32_t* __restrict__ values,
     int64_t visible_capacity, int64_t intersection_capacity) {
@@ -228,7 +228,10 @@ __global__ __launch_bounds__(kBlockThreads) void projection_write_kernel(
   const int last = min(args.active_count, first + args.cameras_per_block);
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
-  __shared__ int wave_totals[2][kBlockThreads / kWave];
+  // Double-buffered by camera parity: a wave can only overwrite a buffer after the
+  // next camera's barrier, which every wave reaches after reading this camera's totals.
+  __shared__ int wave_totals[2][2][kBlockThreads / kWave];
+  int parity = 0;
   for (int a = first; a < last; ++a) {
     if (((mask >> (a - first)) & 1ull) == 0) continue;  // uniform: whole block skips the camera
     const CameraParams& cam = args.table[a];
@@ -240,21 +243,21 @@ __global__ __launch_bounds__(kBlockThreads) void projection_write_kernel(
     int inclusive_tiles = tiles;
     wave_inclusive_scan2(inclusive_visible, inclusive_tiles);
     if (lane == kWave - 1) {
-      wave_totals[0][wave] = inclusive_visible;
-      wave_totals[1][wave] = inclusive_tiles;
+      wave_totals[parity][0][wave] = inclusive_visible;
+      wave_totals[parity][1][wave] = inclusive_tiles;
     }
     __syncthreads();
     int64_t prefix_visible = 0, prefix_tiles = 0, total_visible = 0, total_tiles = 0;
 #pragma unroll
     for (int w = 0; w < kBlockThreads / kWave; ++w) {
       if (w < wave) {
-        prefix_visible += wave_totals[0][w];
-        prefix_tiles += wave_totals[1][w];
+        prefix_visible += wave_totals[parity][0][w];
+        prefix_tiles += wave_totals[parity][1][w];
       }
-      total_visible += wave_totals[0][w];
-      total_tiles += wave_totals[1][w];
+      total_visible += wave_totals[parity][0][w];
+      total_tiles += wave_totals[parity][1][w];
     }
-    __syncthreads();
+    parity ^= 1;
     if (visible) {
       const int64_t record = visible_base + prefix_visible + inclusive_visible - 1;
       const bool stored = record < visible_capacity;
