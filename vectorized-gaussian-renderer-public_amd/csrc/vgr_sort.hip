@@ -73,22 +73,35 @@ __global__ __launch_bounds__(kBlockThreads) void sort_histogram_kernel(const uin
   // adds the run's length.  Emission writes a splat's tiles next to each other,
   // so the high (tile) digits come in long runs that would otherwise serialize up
   // to 64 lanes on one LDS address.
+  // Each wave takes four consecutive 64-key runs per iteration and issues their
+  // loads together, so four loads per lane are in flight.
+  constexpr int kRuns = 4;
   const int lane = threadIdx.x & (kWave - 1);
+  const int64_t wave_id = (static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x) / kWave;
+  const int64_t wave_count = static_cast<int64_t>(gridDim.x) * (kBlockThreads / kWave);
   const uint64_t above = lane == kWave - 1 ? 0ull : (~0ull << (lane + 1));
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x;
-       i - lane < n; i += static_cast<int64_t>(gridDim.x) * kBlockThreads) {
-    const bool valid = i < n;
-    const uint64_t ck = valid ? compressed_key(keys[i], kp) : 0ull;
-    const uint64_t live = __ballot(valid);
-    for (int p = 0; p < passes; ++p) {
-      const uint32_t digit = static_cast<uint32_t>(ck >> (p * digit_bits)) & mask;
-      const uint32_t before = __shfl_up(digit, 1, kWave);
-      const bool head = valid && (lane == 0 || digit != before);
-      const uint64_t heads = __ballot(head);
-      if (head) {
-        const uint64_t later = (heads & above) | ~live;  // next run head, or the first invalid lane
-        const int end = later == 0ull ? kWave : __ffsll(static_cast<unsigned long long>(later)) - 1;
-        atomicAdd(&local[p * radix + digit], static_cast<uint32_t>(end - lane));
+  for (int64_t base = wave_id * kRuns * kWave; base < n; base += wave_count * kRuns * kWave) {
+    uint64_t ck[kRuns];
+#pragma unroll
+    for (int r = 0; r < kRuns; ++r) {
+      const int64_t i = base + r * kWave + lane;
+      ck[r] = i < n ? compressed_key(keys[i], kp) : 0ull;
+    }
+#pragma unroll
+    for (int r = 0; r < kRuns; ++r) {
+      const bool valid = base + r * kWave + lane < n;
+      const uint64_t live = __ballot(valid);
+      if (live == 0ull) break;
+      for (int p = 0; p < passes; ++p) {
+        const uint32_t digit = static_cast<uint32_t>(ck[r] >> (p * digit_bits)) & mask;
+        const uint32_t before = __shfl_up(digit, 1, kWave);
+        const bool head = valid && (lane == 0 || digit != before);
+        const uint64_t heads = __ballot(head);
+        if (head) {
+          const uint64_t later = (heads & above) | ~live;  // next run head, or the first invalid lane
+          const int end = later == 0ull ? kWave : __ffsll(static_cast<unsigned long long>(later)) - 1;
+          atomicAdd(&local[p * radix + digit], static_cast<uint32_t>(end - lane));
+        }
       }
     }
   }
@@ -355,27 +368,40 @@ __global__ __launch_bounds__(kBlockThreads) void ranges_kernel(const uint64_t* _
                                                                int32_t* __restrict__ tile_ends,
                                                                int64_t* __restrict__ counters) {
   const int64_t n = element_count(counters_in, capacity);
+  // Each lane takes kPer consecutive keys (loads issued together); the neighbours
+  // across lanes come from shuffles, and lanes 0 and 63 read the one key beyond
+  // their wave's run, so every key is read from memory about once.
+  constexpr int kPer = 4;
+  constexpr uint32_t kOutside = 0xfffffffeu;  // differs from every tile and from the sentinel tile
   const int lane = threadIdx.x & (kWave - 1);
+  const int64_t wave_id = (static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x) / kWave;
+  const int64_t wave_count = static_cast<int64_t>(gridDim.x) * (kBlockThreads / kWave);
   int64_t ends = 0;
-  // One key per lane; neighbours come from the adjacent lanes (lanes 0 and 63
-  // read theirs), so every key is read from memory about once.
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x; i - lane < n;
-       i += static_cast<int64_t>(gridDim.x) * kBlockThreads) {
-    const bool valid = i < n;
-    const uint32_t tile = valid ? static_cast<uint32_t>(keys[i] >> 32) : 0xffffffffu;
-    uint32_t previous = __shfl_up(tile, 1, kWave);
-    uint32_t next = __shfl_down(tile, 1, kWave);
-    if (lane == 0) previous = i > 0 && valid ? static_cast<uint32_t>(keys[i - 1] >> 32) : 0xfffffffeu;
-    if (lane == kWave - 1 || i + 1 >= n) next = i + 1 < n ? static_cast<uint32_t>(keys[i + 1] >> 32) : 0xfffffffeu;
-    if (!valid || tile >= static_cast<uint64_t>(global_tiles)) continue;
-    if (i == 0 || previous != tile) tile_starts[tile] = static_cast<int32_t>(i);
-    if (i + 1 == n || next != tile) {
-      tile_ends[tile] = static_cast<int32_t>(i + 1);
-      ++ends;
+  for (int64_t base = wave_id * kWave * kPer; base < n; base += wave_count * kWave * kPer) {
+    const int64_t first = base + static_cast<int64_t>(lane) * kPer;
+    uint32_t tile[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) tile[k] = first + k < n ? static_cast<uint32_t>(keys[first + k] >> 32) : kOutside;
+    uint32_t previous = __shfl_up(tile[kPer - 1], 1, kWave);
+    uint32_t next = __shfl_down(tile[0], 1, kWave);
+    if (lane == 0) previous = first > 0 && first < n ? static_cast<uint32_t>(keys[first - 1] >> 32) : kOutside;
+    if (lane == kWave - 1) next = first + kPer < n ? static_cast<uint32_t>(keys[first + kPer] >> 32) : kOutside;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int64_t i = first + k;
+      const uint32_t t = tile[k];
+      if (i >= n || t >= static_cast<uint64_t>(global_tiles)) continue;
+      const uint32_t before = k == 0 ? previous : tile[k - 1];
+      const uint32_t after = k == kPer - 1 ? next : tile[k + 1];
+      if (i == 0 || before != t) tile_starts[t] = static_cast<int32_t>(i);
+      if (i + 1 == n || after != t) {
+        tile_ends[t] = static_cast<int32_t>(i + 1);
+        ++ends;
+      }
     }
   }
   for (int offset = 32; offset > 0; offset >>= 1) ends += __shfl_xor(ends, offset, kWave);
-  if ((threadIdx.x & (kWave - 1)) == 0 && ends != 0) {
+  if (lane == 0 && ends != 0) {
     atomicAdd(reinterpret_cast<unsigned long long*>(counters + VGR_COUNTER_ACTIVE_TILES),
               static_cast<unsigned long long>(ends));
   }
@@ -585,7 +611,7 @@ int launch_ranges(const vgr_workspace& ws, int64_t global_tiles, bool sorted_in_
     launch_fill_u32(reinterpret_cast<uint32_t*>(ws.tile_ends), global_tiles, 0u, stream);
   }
   const uint64_t* keys = sorted_in_out ? ws.keys_out : ws.keys_in;
-  int64_t blocks = (ws.intersection_capacity + kBlockThreads * 8 - 1) / (kBlockThreads * 8);
+  int64_t blocks = (ws.intersection_capacity + kBlockThreads * 16 - 1) / (kBlockThreads * 16);
   blocks = blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks);
   hipLaunchKernelGGL(ranges_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0, stream, keys,
                      ws.counters, ws.intersection_capacity, global_tiles, ws.tile_starts, ws.tile_ends, ws.counters);
