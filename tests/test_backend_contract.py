@@ -270,7 +270,7 @@ def test_unsupported_reference_modes_are_rejected():
 @pytest.mark.parametrize("kwargs", [
     {"capacity_growth_factor": 1.0}, {"capacity_headroom": 0.9}, {"max_capacity_retries": -1},
     {"max_capacity_retries": True}, {"max_workspace_bytes": 0}, {"semantic_min_alpha": 1.5},
-    {"covariance_epsilon": float("nan")}, {"near_plane": 0.0}, {"sort_digit_bits": 9}, {"raster_batch": 96},
+    {"covariance_epsilon": float("nan")}, {"near_plane": 0.0}, {"sort_digit_bits": 10}, {"sort_digit_bits": 3}, {"raster_batch": 96},
     {"cameras_per_block": 65}, {"sh_degree": 4},
 ])
 def test_parameters_are_strictly_validated(kwargs):

@@ -42,9 +42,9 @@ def test_sh3_multi_camera_deterministic_matches_oracle():
     compare_ordering(backend, expected)
 
 
-@pytest.mark.parametrize("items,digit_bits", [(8, 8), (12, 8), (16, 8), (16, 6), (12, 7)])
+@pytest.mark.parametrize("items,digit_bits", [(8, 8), (12, 8), (16, 8), (16, 6), (12, 7), (16, 9), (8, 9)])
 def test_sort_tile_sizes_and_digit_widths_keep_order_bit_exact(items, digit_bits):
-    """Sort tiles of 512 x {8, 12, 16} keys with 6-8-bit digits: several tiles and a partial
+    """Sort tiles of 512 x {8, 12, 16} keys with 6-9-bit digits: several tiles and a partial
     last one (140,757 intersections), ordering bit-exact against the oracle."""
     scene = synthetic.gaussian_scene(50_000, seed=31, sh_degree=0)
     rig = synthetic.orbit_rig(2, 320, 240, 277.0, seed=31, radius=(9.0, 12.0))

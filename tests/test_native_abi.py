@@ -51,6 +51,16 @@ def test_scratch_query_grows_with_capacity():
     assert 0 < small < large
 
 
+def test_scratch_query_is_monotonic_in_tile_count_with_default_digits():
+    """The default digit width switches between 8 and 9 bits with the key width; scratch sized for
+    the largest tile count must still cover every smaller render (the batched path relies on it)."""
+    module = native.NativeRenderer()
+    sizes = [module.scratch_bytes(max_views=views, height=480, width=640, gaussians=100_000,
+                                  intersection_capacity=5_000_000, tile_size=16, near_plane=0.01, far_plane=100.0)
+             for views in (1, 2, 3, 5, 8, 16, 33, 64, 100, 128, 200, 256)]
+    assert all(a <= b for a, b in zip(sizes, sizes[1:])), sizes
+
+
 def test_invalid_options_are_reported_with_a_message():
     module = native.NativeRenderer()
     with pytest.raises(RuntimeError, match="tile_size"):

@@ -36,8 +36,11 @@ int64_t global_tile_count(const vgr_cameras& cameras, const vgr_options& options
   return static_cast<int64_t>(cameras.batch) * tiles_x * tiles_y;
 }
 
-int resolve_digit_bits(const vgr_options& options) {
-  return options.sort_digit_bits == 0 ? 8 : options.sort_digit_bits;
+// Default digit width: 8 bits, or 9 where that saves a radix pass (a 44-bit key:
+// 5 passes instead of 6; an 8192-key tile still averages 16 keys per digit).
+int resolve_digit_bits(const vgr_options& options, int key_bits) {
+  if (options.sort_digit_bits != 0) return options.sort_digit_bits;
+  return (key_bits + 8) / 9 < (key_bits + 7) / 8 ? 9 : 8;
 }
 
 int resolve_cameras_per_block(const vgr_options& options) {
@@ -59,11 +62,21 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   layout.depth_bits = bit_length_u64(static_cast<uint64_t>(far_bits - near_bits) + 1u);
   layout.tile_bits = global_tiles > 1 ? bit_length_u64(static_cast<uint64_t>(global_tiles - 1)) : 0;
   layout.sort_key_bits = layout.tile_bits + layout.depth_bits;
-  layout.sort_digit_bits = resolve_digit_bits(options);
+  layout.sort_digit_bits = resolve_digit_bits(options, layout.sort_key_bits);
   layout.sort_passes = (layout.sort_key_bits + layout.sort_digit_bits - 1) / layout.sort_digit_bits;
   layout.sort_tile = kSortThreads * (options.sort_tile_items == 0 ? kDefaultSortItems : options.sort_tile_items);
   layout.sort_tiles = (intersection_capacity + layout.sort_tile - 1) / layout.sort_tile;
   const int64_t radix = 1 << layout.sort_digit_bits;
+  // Per-pass sort arrays hold passes x radix slots.  With the default digit width
+  // they are sized for the larger of the 8- and 9-bit choices, which grows with the
+  // key width, so scratch sized for the largest tile count fits every smaller render.
+  int64_t pass_slots = static_cast<int64_t>(layout.sort_passes) * radix;
+  int64_t pass_count = layout.sort_passes;
+  if (options.sort_digit_bits == 0) {
+    const int64_t passes8 = (layout.sort_key_bits + 7) / 8, passes9 = (layout.sort_key_bits + 8) / 9;
+    pass_slots = passes8 * 256 > passes9 * 512 ? passes8 * 256 : passes9 * 512;
+    pass_count = passes8;
+  }
 
   size_t offset = 0;
   layout.camera_table = offset;
@@ -73,16 +86,16 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   layout.block_bases = offset;
   offset = align_up(offset + sizeof(int64_t) * 2 * static_cast<size_t>(layout.projection_blocks + scan_blocks + 1));
   layout.sort_histogram = offset;
-  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(layout.sort_passes * radix));
+  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_slots));
   layout.sort_digit_base = offset;
-  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(layout.sort_passes * radix));
+  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_slots));
   layout.sort_tickets = offset;
-  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(layout.sort_passes));
+  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_count));
   layout.sort_status = offset;
   offset = align_up(offset + sizeof(uint32_t) * 4);
   layout.sort_flags = offset;
-  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(layout.sort_passes) *
-                                 static_cast<size_t>(layout.sort_tiles) * static_cast<size_t>(radix));
+  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_slots) *
+                                 static_cast<size_t>(layout.sort_tiles));
   layout.work_counter = offset;
   offset = align_up(offset + sizeof(uint32_t) * 4);
   layout.culling = options.chunk_culling != 0;
@@ -126,9 +139,8 @@ int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
     set_error("gaussian_support_sigma must be positive and covariance_epsilon non-negative");
     return VGR_E_INVALID_ARGUMENT;
   }
-  const int digit_bits = resolve_digit_bits(options);
-  if (digit_bits < 4 || digit_bits > 8) {
-    set_error("sort_digit_bits must be in [4, 8]; got %d", options.sort_digit_bits);
+  if (options.sort_digit_bits != 0 && (options.sort_digit_bits < 4 || options.sort_digit_bits > 9)) {
+    set_error("sort_digit_bits must be 0 (default) or in [4, 9]; got %d", options.sort_digit_bits);
     return VGR_E_INVALID_ARGUMENT;
   }
   const int batch = options.raster_batch == 0 ? 256 : options.raster_batch;

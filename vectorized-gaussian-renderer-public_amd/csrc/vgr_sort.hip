@@ -28,6 +28,8 @@
 // a status word instead of hanging.
 // The element count is read from the device counter block, so the sequence
 // never synchronizes the host and can be captured in a graph.
+#include <type_traits>
+
 #include "vgr_internal.h"
 
 namespace vgr {
@@ -113,10 +115,10 @@ __global__ __launch_bounds__(kBlockThreads) void sort_histogram_kernel(const uin
 
 // Exclusive scan of every pass's histogram (one block, one digit per thread),
 // then clears the tickets and status words for the passes.
-__global__ __launch_bounds__(kBlockThreads) void sort_digit_scan_kernel(const uint32_t* __restrict__ histogram,
+__global__ __launch_bounds__(kSortThreads) void sort_digit_scan_kernel(const uint32_t* __restrict__ histogram,
                                                                         int digit_bits, int passes,
                                                                         uint32_t* __restrict__ digit_base) {
-  __shared__ uint32_t wave_totals[kBlockThreads / kWave];
+  __shared__ uint32_t wave_totals[kSortThreads / kWave];
   const int radix = 1 << digit_bits;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
@@ -177,6 +179,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
     uint32_t* __restrict__ status) {
   constexpr int kRadix = 1 << kDigitBits;
   constexpr int kSortTile = kSortThreads * kSortItems;
+  static_assert(kRadix <= kSortThreads && kSortTile <= 65535, "one digit per thread; 16-bit offsets");
   constexpr int kWaves = kSortThreads / kWave;
   constexpr int kWaveRun = kSortTile / kWaves;  // contiguous keys ranked by one wave
   constexpr uint32_t kMask = kRadix - 1;
@@ -185,7 +188,10 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
   __shared__ uint64_t s_stage[kSortTile];
   uint64_t* const s_keys = s_stage;
   int32_t* const s_values = reinterpret_cast<int32_t*>(s_stage);
-  __shared__ uint32_t s_count[kWaves][kRadix];
+  // Per-wave digit counts, then in-place per-wave offsets (at most the tile size,
+  // 8192): 16-bit entries for 9-bit digits keep two workgroups per CU in LDS.
+  using Count = typename std::conditional<(kRadix > 256), uint16_t, uint32_t>::type;
+  __shared__ Count s_count[kWaves][kRadix];
   __shared__ uint32_t s_local_start[kRadix];
   __shared__ int32_t s_global_start[kRadix];  // capacities are below 2^30
   __shared__ uint32_t s_ticket;
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
       const uint32_t below = static_cast<uint32_t>(__popcll(peers & lt));
       const uint32_t base = s_count[wave][digit];
       ranks[r] = base + below;
-      if (below == 0) s_count[wave][digit] = base + static_cast<uint32_t>(__popcll(peers));
+      if (below == 0) s_count[wave][digit] = static_cast<Count>(base + static_cast<uint32_t>(__popcll(peers)));
     }
   }
   __syncthreads();
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
       const uint32_t c = s_count[w][threadIdx.x];
-      s_count[w][threadIdx.x] = running;
+      s_count[w][threadIdx.x] = static_cast<Count>(running);
       running += c;
     }
     total = running;
@@ -255,7 +261,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
     const uint32_t flag = (tile == 0 ? kFlagPrefix : kFlagAggregate) | total;
     __hip_atomic_store(&flags[tile * kRadix + threadIdx.x], flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  // Exclusive scan of the digit totals (kRadix <= 256 = one per thread).
+  // Exclusive scan of the digit totals (kRadix <= kSortThreads: one per thread).
   {
     __shared__ uint32_t wave_totals[kWaves];
     uint32_t inclusive = total;
@@ -549,7 +555,7 @@ int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scra
   histogram_blocks = histogram_blocks > 2048 ? 2048 : (histogram_blocks < 1 ? 1 : histogram_blocks);
   hipLaunchKernelGGL(sort_histogram_kernel, dim3(static_cast<unsigned>(histogram_blocks)), dim3(kBlockThreads), 0,
                      stream, ws.keys_in, ws.counters, capacity, kp, digit_bits, passes, histogram);
-  hipLaunchKernelGGL(sort_digit_scan_kernel, dim3(1), dim3(kBlockThreads), 0, stream, histogram, digit_bits, passes,
+  hipLaunchKernelGGL(sort_digit_scan_kernel, dim3(1), dim3(kSortThreads), 0, stream, histogram, digit_bits, passes,
                      digit_base);
   const uint64_t* source_keys = ws.keys_in;
   const int32_t* source_values = ws.values_in;
@@ -574,6 +580,10 @@ int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scra
         break;
       case 7:
         launch_pass<7>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
+                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+        break;
+      case 9:
+        launch_pass<9>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
                        pass_base, pass_flags, tickets + p, status, tiles, stream);
         break;
       default:

@@ -30,7 +30,7 @@ import torch
 DEFAULT_SPACE: dict[str, tuple[int, ...]] = {
     "tile_size": (16, 8),
     "raster_batch": (256, 128, 512, 64),
-    "sort_digit_bits": (8, 7, 6),
+    "sort_digit_bits": (0, 8, 7, 6),  # 0: the library default (8, or 9 where that saves a pass)
     "cameras_per_block": (8, 16, 4, 32),
 }
 

@@ -57,6 +57,11 @@ def _emitted_sort_key_end_bit(global_tile_capacity: int) -> int:
     return 32 + (global_tile_capacity - 1).bit_length()
 
 
+def default_sort_digit_bits(key_bits: int) -> int:
+    """The library's default radix digit width: 8, or 9 where that saves a pass."""
+    return 9 if -(-key_bits // 9) < -(-key_bits // 8) else 8
+
+
 def compressed_sort_key_bits(global_tiles: int, near_plane: float, far_plane: float) -> int:
     """Significant bits of the order-preserving sort-key image used on the device."""
     tile_bits = (global_tiles - 1).bit_length() if global_tiles > 1 else 0
@@ -167,8 +172,8 @@ class CustomHipBackend:
             raise ValueError("max_workspace_bytes must be positive when set.")
         if sh_degree is not None and not 0 <= sh_degree <= 3:
             raise ValueError("sh_degree must be in [0, 3] or None (infer from the feature width).")
-        if sort_digit_bits != 0 and not 4 <= sort_digit_bits <= 8:
-            raise ValueError("sort_digit_bits must be 0 (default) or in [4, 8].")
+        if sort_digit_bits != 0 and not 4 <= sort_digit_bits <= 9:
+            raise ValueError("sort_digit_bits must be 0 (default) or in [4, 9].")
         if raster_batch not in (0, 64, 128, 256, 512):
             raise ValueError("raster_batch must be 0 (default), 64, 128, 256 or 512.")
         if not 0 <= cameras_per_block <= 64:
@@ -665,8 +670,8 @@ class CustomHipBackend:
         global_tiles = self._global_tiles(max_views)
         if global_tiles > _INT32_MAX:
             raise ValueError("Global tile capacity exceeds the int32 implementation limit.")
-        digit_bits = self.sort_digit_bits or 8
         key_bits = compressed_sort_key_bits(global_tiles, self.near_plane, self.far_plane)
+        digit_bits = self.sort_digit_bits or default_sort_digit_bits(key_bits)
         scratch_bytes = self._scratch_required(intersection_capacity)
         specs: dict[str, tuple[tuple[int, ...], torch.dtype]] = {
             "visible_records": ((visible_capacity, RECORD_FLOATS), torch.float32),
