@@ -845,15 +845,14 @@ __global__ __launch_bounds__(kBlockThreads) void projection_backward_kernel(Proj
   float grad_cov[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   float grad_opacity = 0.0f;
   bool touched = false;
-  int64_t visible_base = args.block_bases[static_cast<int64_t>(blockIdx.x) * 2 + 0];
+  int64_t visible_base =
+      args.block_bases[(static_cast<int64_t>(blockIdx.x) * kProjectionWaves + threadIdx.x / kWave) * 2 + 0];
   const int first = group * fw.cameras_per_block;
   const int last = min(fw.active_count, first + fw.cameras_per_block);
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
-  // Double-buffered by camera parity (as in the forward's write kernel): one barrier per camera.
-  __shared__ int wave_totals[2][kBlockThreads / kWave];
+  // Records are found with the forward's per-wave bases and wave scans (no block barrier).
   __shared__ float s_pose[kBlockThreads / kWave][kCameraGrads];
-  int parity = 0;
   for (int a = first; a < last; ++a) {
     if (((mask >> (a - first)) & 1ull) == 0) continue;
     const CameraParams& cam = fw.table[a];
@@ -864,22 +863,14 @@ __global__ __launch_bounds__(kBlockThreads) void projection_backward_kernel(Proj
     int inclusive_visible = visible ? 1 : 0;
     int unused = 0;
     wave_inclusive_scan2(inclusive_visible, unused);
-    if (lane == kWave - 1) wave_totals[parity][wave] = inclusive_visible;
-    __syncthreads();
-    int64_t prefix_visible = 0, total_visible = 0;
-#pragma unroll
-    for (int w = 0; w < kBlockThreads / kWave; ++w) {
-      if (w < wave) prefix_visible += wave_totals[parity][w];
-      total_visible += wave_totals[parity][w];
-    }
-    parity ^= 1;
+    const int total_visible = __shfl(inclusive_visible, kWave - 1, kWave);
     float pose[kCameraGrads];
     if (kPose) {
 #pragma unroll
       for (int k = 0; k < kCameraGrads; ++k) pose[k] = 0.0f;
     }
     if (visible) {
-      const int64_t record = visible_base + prefix_visible + inclusive_visible - 1;
+      const int64_t record = visible_base + inclusive_visible - 1;
       if (record < args.visible_capacity) {
         float r[kRecordGrads];
         const float* src = args.record_grads + record * kRecordGrads;
@@ -917,7 +908,8 @@ __global__ __launch_bounds__(kBlockThreads) void projection_backward_kernel(Proj
         }
       }
     }
-    if (kPose && total_visible > 0) {
+    // Block-uniform test (the barrier below needs every wave).
+    if (kPose && __syncthreads_or(total_visible > 0)) {
 #pragma unroll
       for (int k = 0; k < kCameraGrads; ++k) {
         const float sum = wave_sum_to_last_lane(pose[k]);

@@ -55,6 +55,9 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   layout.projection_groups = (active_count + layout.cameras_per_block - 1) / layout.cameras_per_block;
   layout.projection_blocks = layout.projection_chunks * layout.projection_groups;
   const int64_t scan_blocks = (layout.projection_blocks + kBlockThreads * 8 - 1) / (kBlockThreads * 8);
+  // Record and intersection bases are scanned per projection wave.
+  const int64_t projection_waves = layout.projection_blocks * kProjectionWaves;
+  const int64_t wave_scan_blocks = (projection_waves + kBlockThreads * 8 - 1) / (kBlockThreads * 8);
 
   const uint32_t near_bits = vgr_float_bits(options.near_plane);
   const uint32_t far_bits = vgr_float_bits(options.far_plane);
@@ -82,9 +85,9 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   layout.camera_table = offset;
   offset = align_up(offset + sizeof(CameraParams) * static_cast<size_t>(active_count > 0 ? active_count : 1));
   layout.block_counts = offset;
-  offset = align_up(offset + sizeof(int64_t) * 2 * static_cast<size_t>(layout.projection_blocks + 1));
+  offset = align_up(offset + sizeof(int64_t) * 2 * static_cast<size_t>(projection_waves + 1));
   layout.block_bases = offset;
-  offset = align_up(offset + sizeof(int64_t) * 2 * static_cast<size_t>(layout.projection_blocks + scan_blocks + 1));
+  offset = align_up(offset + sizeof(int64_t) * 2 * static_cast<size_t>(projection_waves + wave_scan_blocks + 1));
   layout.sort_histogram = offset;
   offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_slots));
   layout.sort_digit_base = offset;

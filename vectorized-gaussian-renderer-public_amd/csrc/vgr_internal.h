@@ -42,8 +42,8 @@ static_assert(sizeof(CameraParams) == 256, "camera table row must be 256 bytes")
 // Scratch layout inside workspace.scratch (all offsets 256-byte aligned).
 struct ScratchLayout {
   size_t camera_table;     // CameraParams[active_count]
-  size_t block_counts;     // int64[2 * projection_blocks]
-  size_t block_bases;      // int64[2 * projection_blocks]
+  size_t block_counts;     // int64[2 * projection_waves]: (visible, intersections) per projection wave
+  size_t block_bases;      // int64[2 * (projection_waves + scan partials)]: exclusive bases per wave
   size_t sort_histogram;   // uint32[passes * 256]
   size_t sort_digit_base;  // uint32[passes * 256]
   size_t sort_tickets;     // uint32[passes]
@@ -72,6 +72,7 @@ struct ScratchLayout {
   uint32_t depth_base;     // float bits of the near plane
 };
 
+constexpr int kProjectionWaves = kBlockThreads / kWave;  // waves of a projection workgroup
 constexpr int kSortThreads = 512;      // threads per onesweep workgroup (8 waves)
 constexpr int kDefaultSortItems = 16;  // keys per thread per sort tile (tile = kSortThreads * items keys)
 

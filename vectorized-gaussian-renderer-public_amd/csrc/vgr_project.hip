@@ -155,10 +155,11 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
                                                                          int64_t* __restrict__ block_counts) {
   int64_t block;
   uint64_t mask;
+  const int64_t wave_slot = static_cast<int64_t>(blockIdx.x) * kProjectionWaves + threadIdx.x / kWave;
   if (!resolve_block(args, block, mask)) {
-    if (threadIdx.x == 0) {
-      block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 0] = 0;
-      block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 1] = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      block_counts[wave_slot * 2 + 0] = 0;
+      block_counts[wave_slot * 2 + 1] = 0;
     }
     return;
   }
@@ -182,23 +183,13 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
       intersections += tiles;
     }
   }
-  __shared__ int64_t partial[2][kBlockThreads / kWave];
+  // Counts per wave: each wave of the write kernel then owns its output range and
+  // runs without block barriers.
   visible = wave_sum_i64(visible);
   intersections = wave_sum_i64(intersections);
-  const int wave = threadIdx.x / kWave;
   if ((threadIdx.x & (kWave - 1)) == 0) {
-    partial[0][wave] = visible;
-    partial[1][wave] = intersections;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int64_t v = 0, t = 0;
-    for (int w = 0; w < kBlockThreads / kWave; ++w) {
-      v += partial[0][w];
-      t += partial[1][w];
-    }
-    block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 0] = v;
-    block_counts[static_cast<int64_t>(blockIdx.x) * 2 + 1] = t;
+    block_counts[wave_slot * 2 + 0] = visible;
+    block_counts[wave_slot * 2 + 1] = intersections;
   }
 }
 
@@ -221,17 +212,13 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
 #pragma unroll
   for (int i = 0; i < kChannels; ++i) f[i] = 0.0f;
   if (loaded && opacity >= VGR_ALPHA_THRESHOLD) load_features<kDegree>(args.features, g, args.feature_width, f);
-  int64_t visible_base = block_bases[static_cast<int64_t>(blockIdx.x) * 2 + 0];
-  int64_t intersection_base = block_bases[static_cast<int64_t>(blockIdx.x) * 2 + 1];
+  const int64_t wave_slot = static_cast<int64_t>(blockIdx.x) * kProjectionWaves + threadIdx.x / kWave;
+  int64_t visible_base = block_bases[wave_slot * 2 + 0];
+  int64_t intersection_base = block_bases[wave_slot * 2 + 1];
   const int32_t record_id = args.gaussian_ids != nullptr && loaded ? args.gaussian_ids[g] : static_cast<int32_t>(g);
   const int first = group * args.cameras_per_block;
   const int last = min(args.active_count, first + args.cameras_per_block);
-  const int wave = threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  // Double-buffered by camera parity: a wave can only overwrite a buffer after the
-  // next camera's barrier, which every wave reaches after reading this camera's totals.
-  __shared__ int wave_totals[2][2][kBlockThreads / kWave];
-  int parity = 0;
+  // Each wave owns its scanned output range, so the waves never synchronize.
   for (int a = first; a < last; ++a) {
     if (((mask >> (a - first)) & 1ull) == 0) continue;  // uniform: whole block skips the camera
     const CameraParams& cam = args.table[a];
@@ -242,24 +229,10 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
     int inclusive_visible = visible ? 1 : 0;
     int inclusive_tiles = tiles;
     wave_inclusive_scan2(inclusive_visible, inclusive_tiles);
-    if (lane == kWave - 1) {
-      wave_totals[parity][0][wave] = inclusive_visible;
-      wave_totals[parity][1][wave] = inclusive_tiles;
-    }
-    __syncthreads();
-    int64_t prefix_visible = 0, prefix_tiles = 0, total_visible = 0, total_tiles = 0;
-#pragma unroll
-    for (int w = 0; w < kBlockThreads / kWave; ++w) {
-      if (w < wave) {
-        prefix_visible += wave_totals[parity][0][w];
-        prefix_tiles += wave_totals[parity][1][w];
-      }
-      total_visible += wave_totals[parity][0][w];
-      total_tiles += wave_totals[parity][1][w];
-    }
-    parity ^= 1;
+    const int total_visible = __shfl(inclusive_visible, kWave - 1, kWave);
+    const int total_tiles = __shfl(inclusive_tiles, kWave - 1, kWave);
     if (visible) {
-      const int64_t record = visible_base + prefix_visible + inclusive_visible - 1;
+      const int64_t record = visible_base + inclusive_visible - 1;
       const bool stored = record < visible_capacity;
       if (stored) {
         float rgb[3];
@@ -273,7 +246,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
           radii[record * 2 + 1] = p.radius_y;
         }
       }
-      int64_t slot = intersection_base + prefix_tiles + inclusive_tiles - tiles;
+      int64_t slot = intersection_base + inclusive_tiles - tiles;
       if (tiles > 0) {
         const uint64_t depth_bits = vgr_float_bits(p.depth);
         for (int ty = rect.min_y; ty < rect.max_y; ++ty) {
@@ -685,8 +658,9 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
   int64_t* block_counts = reinterpret_cast<int64_t*>(scratch + layout.block_counts);
   int64_t* block_bases = reinterpret_cast<int64_t*>(scratch + layout.block_bases);
   const int64_t blocks = layout.projection_blocks;
-  // Partials live in the tail of block_bases' region, which holds 2*(blocks + scan_blocks) entries.
-  int64_t* partials = block_bases + 2 * blocks;
+  const int64_t waves = blocks * kProjectionWaves;
+  // Partials live in the tail of block_bases' region, which holds 2*(waves + scan_blocks) entries.
+  int64_t* partials = block_bases + 2 * waves;
 
   if (culled) {
     int64_t* cull_flags = reinterpret_cast<int64_t*>(scratch + layout.cull_counts);
@@ -711,7 +685,7 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
     hipLaunchKernelGGL(projection_count_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0,
                        stream, args, block_counts);
   }
-  scan_pairs(block_counts, blocks, block_bases, partials, ws.visible_capacity, ws.intersection_capacity,
+  scan_pairs(block_counts, waves, block_bases, partials, ws.visible_capacity, ws.intersection_capacity,
              ws.counters, nullptr, stream);
   if (blocks > 0) {
     const dim3 grid(static_cast<unsigned>(blocks)), threads(kBlockThreads);
