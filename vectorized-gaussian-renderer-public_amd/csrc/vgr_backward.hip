@@ -61,6 +61,66 @@ __device__ __forceinline__ float wave_sum_to_last_lane(float v) {
   return v;
 }
 
+// Sum of a value and its copy from the neighbouring row (16 lanes) or half (32
+// lanes).  The swap instructions exchange rows between two registers.  Empty asm
+// statements keep the optimizer from relating the two operands or the two
+// results: it otherwise rewrites the second result as the first (treating the
+// exchange as a plain register swap), which drops the neighbour's rows.
+__device__ __forceinline__ float permlane_pair_sum(float v, bool across_32) {
+  unsigned first = __builtin_bit_cast(unsigned, v);
+  unsigned second = first;
+  asm volatile("" : "+v"(second));
+  unsigned out0, out1;
+  if (across_32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(first, second, false, false);
+    out0 = r[0];
+    out1 = r[1];
+  } else {
+    const auto r = __builtin_amdgcn_permlane16_swap(first, second, false, false);
+    out0 = r[0];
+    out1 = r[1];
+  }
+  asm volatile("" : "+v"(out0), "+v"(out1));
+  return __builtin_bit_cast(float, out0) + __builtin_bit_cast(float, out1);
+}
+
+// Sums ten per-lane values over the wave, leaving value i's total in the lanes
+// whose in-row position L (lane % 16) maps to i, in every row.  Inside a row,
+// three exchange steps each halve the values a lane carries (it keeps one half
+// and sends the other to a partner that holds the same value indices): xor 1
+// and xor 2 (quad permutes), then rotations by 4 and by 8 lanes, which pair
+// the four quads so that every quad's partial reaches every kept value once.
+// The rows are then summed with the permlane swaps (rows 0+1 and 2+3, then the
+// two halves).  37 instructions against 60 for ten separate 6-step sums.
+// L = b + 2c + 4d + 8e (bits) holds value 5b + 3c + 2d + e when d == 0 or
+// (c == 0 and e == 0); its *index is -1 otherwise.
+__device__ __forceinline__ float wave_transpose_sum10(const float (&v)[10], int lane, int* index) {
+  const int l = lane & 15;
+  const bool b = (l & 1) != 0, c = (l & 2) != 0, d = (l & 4) != 0, e = (l & 8) != 0;
+  float w[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) w[j] = (b ? v[j + 5] : v[j]) + dpp_term<0xB1, 0xF>(b ? v[j] : v[j + 5]);
+  float x[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const float high = j < 2 ? w[3 + j] : 0.0f;
+    x[j] = (c ? high : w[j]) + dpp_term<0x4E, 0xF>(c ? w[j] : high);
+  }
+  float y[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const float high = j == 0 ? x[2] : 0.0f;
+    y[j] = (d ? high : x[j]) + dpp_term<0x124, 0xF>(d ? x[j] : high);  // row_ror:4
+  }
+  float z = (e ? y[1] : y[0]) + dpp_term<0x128, 0xF>(e ? y[0] : y[1]);  // row_ror:8
+  z = permlane_pair_sum(z, false);
+  z = permlane_pair_sum(z, true);
+  *index = (!d || (!c && !e)) ? 5 * static_cast<int>(b) + 3 * static_cast<int>(c) + 2 * static_cast<int>(d) +
+                                    static_cast<int>(e)
+                              : -1;
+  return z;
+}
+
 __device__ __forceinline__ void atomic_add_f32(float* address, float value) {
   unsafeAtomicAdd(address, value);
 }
@@ -281,10 +341,11 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_backward_kerne
 // lane four pixels (column lane % 16, rows lane / 16 + 4k).  Pixel k of every
 // lane lies in the tile's k-th band of four rows, so the per-band reach mask of
 // a splat (the same conservative ellipse box as the forward's per-wave lists)
-// is a uniform branch.  Each lane sums its four pixels' gradients of a splat;
-// DPP then sums the wave into lane 63, which parks the sums in LDS, and after
-// the batch every lane issues one slot's atomics, so the atomics of a batch
-// are 10 vector instructions.
+// is a uniform branch.  Each lane sums its four pixels' gradient terms of a
+// splat; a transposing butterfly (DPP and permlane swaps) sums them over the
+// wave into ten lanes, which park them in LDS, and after the batch every lane
+// issues one slot's atomics, so the atomics of a batch are 10 vector
+// instructions.
 // Five waves per SIMD: at most 102 VGPRs (the 6.6 KB of LDS per wave would allow 24 per CU);
 // the batch is gathered at its start rather than prefetched, to stay within them.
 template <int kBatch>
@@ -298,9 +359,10 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
   __shared__ f32x4 s_colour[kBatch];
   __shared__ int32_t s_record[kBatch];
   __shared__ uint16_t s_list[kBatch];
-  // Per batch slot: the wave sums of its ten gradients (padded to 12 floats for
-  // 16-byte stores) and whether any pixel blended it.
-  __shared__ f32x4 s_partial[kBatch][3];
+  // Per batch slot: the wave sums of its ten gradient terms (m_x, m_y, m_xx, m_xy,
+  // m_yy, opacity, depth, r, g, b; padded to 12 floats for 16-byte reads) and
+  // whether any pixel blended it.
+  __shared__ __align__(16) float s_partial[kBatch][12];
   __shared__ uint32_t s_blended[kBatch];
 
   const int64_t block = blockIdx.x;
@@ -488,28 +550,12 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
         }
       }
       if (__ballot(any) == 0ull) continue;
-      // Wave sums land in lane 63, which parks them in LDS; after the batch every
-      // lane owns one slot and issues its atomics (one vector atomic per value).
-      m_x = wave_sum_to_last_lane(m_x);
-      m_y = wave_sum_to_last_lane(m_y);
-      m_xx = wave_sum_to_last_lane(m_xx);
-      m_xy = wave_sum_to_last_lane(m_xy);
-      m_yy = wave_sum_to_last_lane(m_yy);
-      d_o = wave_sum_to_last_lane(d_o);
-      d_z = wave_sum_to_last_lane(d_z);
-      d_r = wave_sum_to_last_lane(d_r);
-      d_g = wave_sum_to_last_lane(d_g);
-      d_b = wave_sum_to_last_lane(d_b);
-      if (lane == kWave - 1) {
-#pragma clang fp contract(fast)
-        // power = g.z dx^2 + g.w dx dy + sh.x dy^2 with dx = px - mean_x, dy = py - mean_y.
-        const float o = sh.y;
-        const float d_mx = -o * (2.0f * g.z * m_x + g.w * m_y);
-        const float d_my = -o * (g.w * m_x + 2.0f * sh.x * m_y);
-        s_partial[item][0] = f32x4{d_mx, d_my, -0.5f * o * m_xx, -o * m_xy};
-        s_partial[item][1] = f32x4{-0.5f * o * m_yy, d_o, d_z, d_r};
-        s_partial[item][2] = f32x4{d_g, d_b, 0.0f, 0.0f};
-      }
+      // The wave sums land in ten lanes of row 0, which park them in LDS; after the
+      // batch every lane owns one slot and issues its atomics (one vector atomic per value).
+      const float terms[10] = {m_x, m_y, m_xx, m_xy, m_yy, d_o, d_z, d_r, d_g, d_b};
+      int index;
+      const float total = wave_transpose_sum10(terms, lane, &index);
+      if (lane < 16 && index >= 0) s_partial[item][index] = total;
       if (lane == 0) s_blended[item] = 1u;
     }
     __syncthreads();
@@ -519,15 +565,30 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
       if (slot >= count || s_blended[slot] == 0u) continue;
       const int32_t record = s_record[slot];
       if (record < 0) continue;
-      const f32x4 p0 = s_partial[slot][0];
-      const f32x4 p1 = s_partial[slot][1];
-      const f32x4 p2 = s_partial[slot][2];
+      const f32x4* partial = reinterpret_cast<const f32x4*>(s_partial[slot]);
+      const f32x4 p0 = partial[0];  // m_x, m_y, m_xx, m_xy
+      const f32x4 p1 = partial[1];  // m_yy, opacity, depth, r
+      const f32x4 p2 = partial[2];  // g, b
+      const f32x4 g = s_geometry[slot];
+      const f32x4 sh = s_shading[slot];
+      float d_mx, d_my, d_c0, d_c1, d_c2;
+      {
+#pragma clang fp contract(fast)
+        // power = g.z dx^2 + g.w dx dy + sh.x dy^2 with dx = px - mean_x, dy = py - mean_y;
+        // dL/d power = q * opacity, and the moments of q were summed above.
+        const float o = sh.y;
+        d_mx = -o * (2.0f * g.z * p0.x + g.w * p0.y);
+        d_my = -o * (g.w * p0.x + 2.0f * sh.x * p0.y);
+        d_c0 = -0.5f * o * p0.z;
+        d_c1 = -o * p0.w;
+        d_c2 = -0.5f * o * p1.x;
+      }
       float* out = args.record_grads + static_cast<int64_t>(record) * kRecordGrads;
-      atomic_add_f32(out + 0, p0.x);
-      atomic_add_f32(out + 1, p0.y);
-      atomic_add_f32(out + 2, p0.z);
-      atomic_add_f32(out + 3, p0.w);
-      atomic_add_f32(out + 4, p1.x);
+      atomic_add_f32(out + 0, d_mx);
+      atomic_add_f32(out + 1, d_my);
+      atomic_add_f32(out + 2, d_c0);
+      atomic_add_f32(out + 3, d_c1);
+      atomic_add_f32(out + 4, d_c2);
       atomic_add_f32(out + 5, p1.y);
       atomic_add_f32(out + 6, p1.z);
       atomic_add_f32(out + 7, p1.w);
