@@ -406,10 +406,19 @@ __global__ __launch_bounds__(kBlockThreads) void ranges_kernel(const uint64_t* _
       }
     }
   }
+  // One atomic per workgroup on the shared counter (per-wave atomics on one
+  // address serialize: thousands of waves would queue on it).
+  __shared__ int64_t s_ends[kBlockThreads / kWave];
   for (int offset = 32; offset > 0; offset >>= 1) ends += __shfl_xor(ends, offset, kWave);
-  if (lane == 0 && ends != 0) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(counters + VGR_COUNTER_ACTIVE_TILES),
-              static_cast<unsigned long long>(ends));
+  if (lane == 0) s_ends[threadIdx.x / kWave] = ends;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t total = 0;
+    for (int w = 0; w < kBlockThreads / kWave; ++w) total += s_ends[w];
+    if (total != 0) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(counters + VGR_COUNTER_ACTIVE_TILES),
+                static_cast<unsigned long long>(total));
+    }
   }
 }
 
