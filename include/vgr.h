@@ -156,7 +156,7 @@ typedef struct vgr_options {
   int32_t fixed_capacity_sort;      /* accepted; the sort never reads the count on the host */
   int32_t sort_digit_bits;          /* 0 = default (8, or 9 where that saves a pass); 4..9 */
   int32_t raster_batch;             /* Gaussians staged per LDS batch: 0 = default (256); 64..512 */
-  int32_t cameras_per_block;        /* projection camera group: 0 = default (8); 1..64 */
+  int32_t cameras_per_block;        /* projection camera group: 0 = default (32); 1..64 */
   int32_t reuse_projection;         /* 1: composite only, reusing the sorted intersections and
                                        tile ranges of the previous render into this workspace
                                        (reference projection cache, cuda_backend.py:607-619) */

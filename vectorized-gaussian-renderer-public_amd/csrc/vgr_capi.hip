@@ -43,8 +43,12 @@ int resolve_digit_bits(const vgr_options& options, int key_bits) {
   return (key_bits + 8) / 9 < (key_bits + 7) / 8 ? 9 : 8;
 }
 
+// Default camera group: 32 cameras per projection workgroup.  Each group re-reads
+// the Gaussians' means, covariances and SH coefficients, so larger groups read
+// less; 64 cameras of 640x480 over 1M SH3 Gaussians project in 1.79 ms with 32
+// per group against 2.00 ms with 8 (64 gives no further gain).
 int resolve_cameras_per_block(const vgr_options& options) {
-  return options.cameras_per_block == 0 ? 8 : options.cameras_per_block;
+  return options.cameras_per_block == 0 ? 32 : options.cameras_per_block;
 }
 
 ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const vgr_options& options,

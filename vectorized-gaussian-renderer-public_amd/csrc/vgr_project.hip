@@ -204,6 +204,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
   const int group = static_cast<int>(block / args.chunks);
   const int64_t chunk = block - static_cast<int64_t>(group) * args.chunks;
   const int64_t g = chunk * kBlockThreads + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   bool loaded;
   float m0, m1, m2, opacity, cov[6];
   load_gaussian(args, g, loaded, m0, m1, m2, opacity, cov);
@@ -246,21 +247,48 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
           radii[record * 2 + 1] = p.radius_y;
         }
       }
-      int64_t slot = intersection_base + inclusive_tiles - tiles;
-      if (tiles > 0) {
-        const uint64_t depth_bits = vgr_float_bits(p.depth);
-        for (int ty = rect.min_y; ty < rect.max_y; ++ty) {
-          for (int tx = rect.min_x; tx < rect.max_x; ++tx, ++slot) {
-            if (slot >= intersection_capacity) continue;
-            if (stored) {
-              const uint32_t tile = cam.tile_base + static_cast<uint32_t>(ty * cam.tiles_x + tx);
-              keys[slot] = (static_cast<uint64_t>(tile) << 32) | depth_bits;
-              values[slot] = static_cast<int32_t>(record);
-            } else {
-              keys[slot] = kSentinelKey;
-              values[slot] = -1;
-            }
-          }
+    }
+    // Tile intersections of the wave: the wave writes its total_tiles slots as
+    // contiguous 64-slot runs (coalesced stores).  Output slot j belongs to the
+    // lane whose [exclusive, inclusive) tile range holds j, found by binary
+    // search over the lanes' inclusive counts; that lane's rectangle gives the
+    // tile in row-major order, as a per-lane loop over the rectangle would.
+    const int exclusive_tiles = inclusive_tiles - tiles;
+    const int owner_min_x = visible ? rect.min_x : 0;
+    const int owner_min_y = visible ? rect.min_y : 0;
+    const int owner_width = visible ? rect.max_x - rect.min_x : 1;
+    const int32_t owner_record =
+        visible && visible_base + inclusive_visible - 1 < visible_capacity
+            ? static_cast<int32_t>(visible_base + inclusive_visible - 1)
+            : -1;
+    const uint32_t owner_depth = visible ? vgr_float_bits(p.depth) : 0u;
+    for (int run = 0; run < total_tiles; run += kWave) {
+      const int j = run + lane;
+      int owner = 0;
+#pragma unroll
+      for (int step = kWave / 2; step > 0; step >>= 1) {
+        if (__shfl(inclusive_tiles, owner + step - 1, kWave) <= j) owner += step;
+      }
+      const int local = j - __shfl(exclusive_tiles, owner, kWave);
+      const int min_x = __shfl(owner_min_x, owner, kWave);
+      const int min_y = __shfl(owner_min_y, owner, kWave);
+      const int width = __shfl(owner_width, owner, kWave);
+      const int32_t value = __shfl(owner_record, owner, kWave);
+      const uint32_t depth_bits = __shfl(owner_depth, owner, kWave);
+      const int64_t slot = intersection_base + j;
+      if (j < total_tiles && slot < intersection_capacity) {
+        // Row within the rectangle: float quotient, corrected to the exact integer one.
+        int dy = static_cast<int>(static_cast<float>(local) / static_cast<float>(width));
+        if (dy * width > local) --dy;
+        if ((dy + 1) * width <= local) ++dy;
+        const int dx = local - dy * width;
+        if (value >= 0) {
+          const uint32_t tile = cam.tile_base + static_cast<uint32_t>((min_y + dy) * cam.tiles_x + min_x + dx);
+          keys[slot] = (static_cast<uint64_t>(tile) << 32) | depth_bits;
+          values[slot] = value;
+        } else {
+          keys[slot] = kSentinelKey;
+          values[slot] = -1;
         }
       }
     }

@@ -31,7 +31,7 @@ DEFAULT_SPACE: dict[str, tuple[int, ...]] = {
     "tile_size": (16, 8),
     "raster_batch": (256, 128, 512, 64),
     "sort_digit_bits": (0, 8, 7, 6),  # 0: the library default (8, or 9 where that saves a pass)
-    "cameras_per_block": (8, 16, 4, 32),
+    "cameras_per_block": (32, 16, 8, 64),
 }
 
 DEFAULT_CACHE = Path(os.environ.get("VGR_AUTOTUNE_CACHE", Path.home() / ".cache" / "vgr_amd" / "autotune.json"))
