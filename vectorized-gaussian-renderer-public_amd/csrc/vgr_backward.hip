@@ -140,10 +140,81 @@ struct CompositeBackwardArgs {
   const float* grad_depth;
   const float* grad_alpha;
   float* record_grads;
+  const int32_t* tile_order;  // optional launch order of the (camera, tile) blocks
   int64_t visible_capacity;
   int active_count;
   int uniform_tiles;
 };
+
+// (active camera, image tile) of a compositing block; false for a block past the
+// last tile of its camera (uneven resolutions).
+__device__ __forceinline__ bool backward_block_tile(const CompositeBackwardArgs& args, int64_t block, int& active,
+                                                    int64_t& image_tile) {
+  if (args.uniform_tiles > 0) {
+    active = static_cast<int>(block / args.uniform_tiles);
+    if (active >= args.active_count) return false;
+    image_tile = block - static_cast<int64_t>(active) * args.uniform_tiles;
+    return true;
+  }
+  active = find_active_camera(args.table, args.active_count, block);
+  image_tile = block - args.table[active].active_tile_begin;
+  return image_tile < static_cast<int64_t>(args.table[active].tiles_x) * args.table[active].tiles_y;
+}
+
+// Launch order for the wave-per-tile backward: blocks sorted by descending list
+// length (counting sort on length / 8, capped), so the longest tiles start first
+// and the short ones fill in behind them.  One workgroup; the order inside a
+// length bucket is unspecified (it changes only the schedule: every tile adds its
+// gradients with atomics either way).
+constexpr int kOrderBuckets = 1024;
+constexpr int kOrderThreads = 1024;
+
+__device__ __forceinline__ int backward_order_bucket(const CompositeBackwardArgs& args, int64_t block) {
+  int active;
+  int64_t image_tile;
+  if (!backward_block_tile(args, block, active, image_tile)) return 0;
+  const CameraParams& cam = args.table[active];
+  if (cam.camera < 0) return 0;
+  const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
+  const int64_t start = args.tile_starts[global_tile];
+  const int64_t end = args.tile_ends[global_tile];
+  const int64_t length = start >= 0 && end > start ? end - start : 0;
+  const int64_t bucket = length >> 3;
+  return static_cast<int>(bucket < kOrderBuckets - 1 ? bucket : kOrderBuckets - 1);
+}
+
+__global__ __launch_bounds__(kOrderThreads) void backward_tile_order_kernel(CompositeBackwardArgs args,
+                                                                            int64_t blocks,
+                                                                            int32_t* __restrict__ order) {
+  __shared__ uint32_t s_next[kOrderBuckets];
+  __shared__ uint32_t s_wave_totals[kOrderThreads / kWave];
+  for (int i = threadIdx.x; i < kOrderBuckets; i += kOrderThreads) s_next[i] = 0;
+  __syncthreads();
+  for (int64_t b = threadIdx.x; b < blocks; b += kOrderThreads) atomicAdd(&s_next[backward_order_bucket(args, b)], 1u);
+  __syncthreads();
+  // Exclusive scan in descending bucket order: thread t owns bucket kOrderBuckets-1-t.
+  static_assert(kOrderBuckets == kOrderThreads, "one bucket per thread");
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int bucket = kOrderBuckets - 1 - static_cast<int>(threadIdx.x);
+  const uint32_t count = s_next[bucket];
+  uint32_t inclusive = count;
+#pragma unroll
+  for (int offset = 1; offset < kWave; offset <<= 1) {
+    const uint32_t t = __shfl_up(inclusive, offset, kWave);
+    if (lane >= offset) inclusive += t;
+  }
+  if (lane == kWave - 1) s_wave_totals[wave] = inclusive;
+  __syncthreads();
+  uint32_t prefix = 0;
+  for (int w = 0; w < wave; ++w) prefix += s_wave_totals[w];
+  s_next[bucket] = prefix + inclusive - count;
+  __syncthreads();
+  for (int64_t b = threadIdx.x; b < blocks; b += kOrderThreads) {
+    const uint32_t position = atomicAdd(&s_next[backward_order_bucket(args, b)], 1u);
+    order[position] = static_cast<int32_t>(b);
+  }
+}
 
 template <int kThreads, int kBatch, int kPerThread>
 __device__ __forceinline__ void gather_batch_ids(const int32_t* __restrict__ sorted_values, int64_t visible_capacity,
@@ -365,18 +436,10 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
   __shared__ __align__(16) float s_partial[kBatch][12];
   __shared__ uint32_t s_blended[kBatch];
 
-  const int64_t block = blockIdx.x;
+  const int64_t block = args.tile_order != nullptr ? args.tile_order[blockIdx.x] : blockIdx.x;
   int active;
   int64_t image_tile;
-  if (args.uniform_tiles > 0) {
-    active = static_cast<int>(block / args.uniform_tiles);
-    if (active >= args.active_count) return;
-    image_tile = block - static_cast<int64_t>(active) * args.uniform_tiles;
-  } else {
-    active = find_active_camera(args.table, args.active_count, block);
-    image_tile = block - args.table[active].active_tile_begin;
-    if (image_tile >= static_cast<int64_t>(args.table[active].tiles_x) * args.table[active].tiles_y) return;
-  }
+  if (!backward_block_tile(args, block, active, image_tile)) return;
   const CameraParams& cam = args.table[active];
   if (cam.camera < 0) return;
   const int tile_x = static_cast<int>(image_tile % cam.tiles_x);
@@ -1197,6 +1260,7 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
   cargs.grad_depth = grads.grad_depth;
   cargs.grad_alpha = grads.grad_alpha;
   cargs.record_grads = grads.record_grads;
+  cargs.tile_order = nullptr;
   cargs.visible_capacity = ws.visible_capacity;
   cargs.active_count = cameras.active_count;
   int64_t tile_blocks;
@@ -1219,6 +1283,12 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
     } else {
       // The backward's batch is independent of the forward's (decisions are per pair);
       // 64 keeps the prefetch registers and LDS small enough for five waves per SIMD.
+      // One wave walks a whole tile list, so the longest list bounds the kernel:
+      // tiles are launched longest first.
+      int32_t* order = reinterpret_cast<int32_t*>(const_cast<char*>(scratch) + layout.tile_order);
+      hipLaunchKernelGGL(backward_tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, stream, cargs, tile_blocks,
+                         order);
+      cargs.tile_order = order;
       launch_composite_backward_wave<64>(cargs, tile_blocks, stream);
     }
   }

@@ -105,6 +105,8 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
                                  static_cast<size_t>(layout.sort_tiles));
   layout.work_counter = offset;
   offset = align_up(offset + sizeof(uint32_t) * 4);
+  layout.tile_order = offset;
+  offset = align_up(offset + sizeof(int32_t) * static_cast<size_t>(global_tiles > 0 ? global_tiles : 1));
   layout.culling = options.chunk_culling != 0;
   const size_t blocks = static_cast<size_t>(layout.projection_blocks);
   const size_t cull_blocks = layout.culling ? blocks : 0;

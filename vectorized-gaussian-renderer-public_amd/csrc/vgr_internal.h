@@ -56,6 +56,7 @@ struct ScratchLayout {
   size_t item_masks;       // uint64[projection_blocks]
   size_t item_totals;      // int64[2]: kept item count
   size_t work_counter;     // uint32[4]: persistent compositor queue
+  size_t tile_order;       // int32[global_tiles]: backward tile launch order (longest lists first)
   size_t total;
   bool culling;
   int64_t projection_blocks;
