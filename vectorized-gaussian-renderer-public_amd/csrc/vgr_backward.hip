@@ -161,61 +161,6 @@ __device__ __forceinline__ bool backward_block_tile(const CompositeBackwardArgs&
   return image_tile < static_cast<int64_t>(args.table[active].tiles_x) * args.table[active].tiles_y;
 }
 
-// Launch order for the wave-per-tile backward: blocks sorted by descending list
-// length (counting sort on length / 8, capped), so the longest tiles start first
-// and the short ones fill in behind them.  One workgroup; the order inside a
-// length bucket is unspecified (it changes only the schedule: every tile adds its
-// gradients with atomics either way).
-constexpr int kOrderBuckets = 1024;
-constexpr int kOrderThreads = 1024;
-
-__device__ __forceinline__ int backward_order_bucket(const CompositeBackwardArgs& args, int64_t block) {
-  int active;
-  int64_t image_tile;
-  if (!backward_block_tile(args, block, active, image_tile)) return 0;
-  const CameraParams& cam = args.table[active];
-  if (cam.camera < 0) return 0;
-  const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
-  const int64_t start = args.tile_starts[global_tile];
-  const int64_t end = args.tile_ends[global_tile];
-  const int64_t length = start >= 0 && end > start ? end - start : 0;
-  const int64_t bucket = length >> 3;
-  return static_cast<int>(bucket < kOrderBuckets - 1 ? bucket : kOrderBuckets - 1);
-}
-
-__global__ __launch_bounds__(kOrderThreads) void backward_tile_order_kernel(CompositeBackwardArgs args,
-                                                                            int64_t blocks,
-                                                                            int32_t* __restrict__ order) {
-  __shared__ uint32_t s_next[kOrderBuckets];
-  __shared__ uint32_t s_wave_totals[kOrderThreads / kWave];
-  for (int i = threadIdx.x; i < kOrderBuckets; i += kOrderThreads) s_next[i] = 0;
-  __syncthreads();
-  for (int64_t b = threadIdx.x; b < blocks; b += kOrderThreads) atomicAdd(&s_next[backward_order_bucket(args, b)], 1u);
-  __syncthreads();
-  // Exclusive scan in descending bucket order: thread t owns bucket kOrderBuckets-1-t.
-  static_assert(kOrderBuckets == kOrderThreads, "one bucket per thread");
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int bucket = kOrderBuckets - 1 - static_cast<int>(threadIdx.x);
-  const uint32_t count = s_next[bucket];
-  uint32_t inclusive = count;
-#pragma unroll
-  for (int offset = 1; offset < kWave; offset <<= 1) {
-    const uint32_t t = __shfl_up(inclusive, offset, kWave);
-    if (lane >= offset) inclusive += t;
-  }
-  if (lane == kWave - 1) s_wave_totals[wave] = inclusive;
-  __syncthreads();
-  uint32_t prefix = 0;
-  for (int w = 0; w < wave; ++w) prefix += s_wave_totals[w];
-  s_next[bucket] = prefix + inclusive - count;
-  __syncthreads();
-  for (int64_t b = threadIdx.x; b < blocks; b += kOrderThreads) {
-    const uint32_t position = atomicAdd(&s_next[backward_order_bucket(args, b)], 1u);
-    order[position] = static_cast<int32_t>(b);
-  }
-}
-
 template <int kThreads, int kBatch, int kPerThread>
 __device__ __forceinline__ void gather_batch_ids(const int32_t* __restrict__ sorted_values, int64_t visible_capacity,
                                                  int64_t batch_start, int64_t range_end, int lane,
@@ -1286,8 +1231,9 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
       // One wave walks a whole tile list, so the longest list bounds the kernel:
       // tiles are launched longest first.
       int32_t* order = reinterpret_cast<int32_t*>(const_cast<char*>(scratch) + layout.tile_order);
-      hipLaunchKernelGGL(backward_tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, stream, cargs, tile_blocks,
-                         order);
+      const int status = launch_tile_order(table, ws.tile_starts, ws.tile_ends, cameras.active_count,
+                                           cargs.uniform_tiles, tile_blocks, order, stream);
+      if (status != 0) return status;
       cargs.tile_order = order;
       launch_composite_backward_wave<64>(cargs, tile_blocks, stream);
     }

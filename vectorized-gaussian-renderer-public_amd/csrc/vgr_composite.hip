@@ -19,6 +19,78 @@ namespace {
 
 using namespace dev;
 
+// ---- tile launch order -------------------------------------------------------
+// Blocks sorted by descending list length (counting sort on length / 8, capped):
+// the longest tiles start first and the short ones fill in behind them, instead of
+// a long tile starting late and running alone at the end of the grid.  One
+// workgroup; the order inside a length bucket is unspecified (it changes only the
+// schedule: every tile's outputs are its own).  On 16 cameras of 512x512 (lists
+// of ~950 entries on average, up to 5k) the forward goes 2.85 -> 2.24 ms and
+// forward+backward 9.6 -> 8.0 ms.
+constexpr int kOrderBuckets = 1024;
+constexpr int kOrderThreads = 1024;
+constexpr int64_t kOrderedCompositeBlocks = 32768;  // forward grids up to this many tiles are ordered
+
+__device__ __forceinline__ int order_bucket(const CameraParams* __restrict__ table,
+                                            const int32_t* __restrict__ tile_starts,
+                                            const int32_t* __restrict__ tile_ends, int active_count,
+                                            int uniform_tiles, int64_t block) {
+  int active;
+  int64_t image_tile;
+  if (uniform_tiles > 0) {
+    active = static_cast<int>(block / uniform_tiles);
+    if (active >= active_count) return 0;
+    image_tile = block - static_cast<int64_t>(active) * uniform_tiles;
+  } else {
+    active = find_active_camera(table, active_count, block);
+    image_tile = block - table[active].active_tile_begin;
+    if (image_tile >= static_cast<int64_t>(table[active].tiles_x) * table[active].tiles_y) return 0;
+  }
+  if (table[active].camera < 0) return 0;
+  const uint32_t global_tile = table[active].tile_base + static_cast<uint32_t>(image_tile);
+  const int64_t start = tile_starts[global_tile];
+  const int64_t end = tile_ends[global_tile];
+  const int64_t bucket = (start >= 0 && end > start ? end - start : 0) >> 3;
+  return static_cast<int>(bucket < kOrderBuckets - 1 ? bucket : kOrderBuckets - 1);
+}
+
+__global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const CameraParams* __restrict__ table,
+                                                                   const int32_t* __restrict__ tile_starts,
+                                                                   const int32_t* __restrict__ tile_ends,
+                                                                   int active_count, int uniform_tiles, int64_t blocks,
+                                                                   int32_t* __restrict__ order) {
+  __shared__ uint32_t s_next[kOrderBuckets];
+  __shared__ uint32_t s_wave_totals[kOrderThreads / kWave];
+  for (int i = threadIdx.x; i < kOrderBuckets; i += kOrderThreads) s_next[i] = 0;
+  __syncthreads();
+  for (int64_t b = threadIdx.x; b < blocks; b += kOrderThreads) {
+    atomicAdd(&s_next[order_bucket(table, tile_starts, tile_ends, active_count, uniform_tiles, b)], 1u);
+  }
+  __syncthreads();
+  // Exclusive scan in descending bucket order: thread t owns bucket kOrderBuckets-1-t.
+  static_assert(kOrderBuckets == kOrderThreads, "one bucket per thread");
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int bucket = kOrderBuckets - 1 - static_cast<int>(threadIdx.x);
+  const uint32_t count = s_next[bucket];
+  uint32_t inclusive = count;
+#pragma unroll
+  for (int offset = 1; offset < kWave; offset <<= 1) {
+    const uint32_t t = __shfl_up(inclusive, offset, kWave);
+    if (lane >= offset) inclusive += t;
+  }
+  if (lane == kWave - 1) s_wave_totals[wave] = inclusive;
+  __syncthreads();
+  uint32_t prefix = 0;
+  for (int w = 0; w < wave; ++w) prefix += s_wave_totals[w];
+  s_next[bucket] = prefix + inclusive - count;
+  __syncthreads();
+  for (int64_t b = threadIdx.x; b < blocks; b += kOrderThreads) {
+    const int k = order_bucket(table, tile_starts, tile_ends, active_count, uniform_tiles, b);
+    order[atomicAdd(&s_next[k], 1u)] = static_cast<int32_t>(b);
+  }
+}
+
 struct CompositeArgs {
   const CameraParams* table;
   const int32_t* tile_starts;
@@ -34,6 +106,7 @@ struct CompositeArgs {
   int64_t visible_capacity;
   int64_t gaussian_count;
   uint32_t* work_counter;  // persistent queue counter (zeroed before the launch)
+  const int32_t* tile_order;  // launch order of the (camera, tile) items
   int64_t total_items;     // (camera, tile) items in the launch
   int active_count;
   int uniform_tiles;     // tiles per camera when every camera has the same resolution, else 0
@@ -216,7 +289,8 @@ constexpr int kCompositeWavesPerSimd =
 template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint>
 __global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTileSize, kBatch, kCount>)) void composite_kernel(
     CompositeArgs args) {
-  composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint>(args, blockIdx.x);
+  const int64_t block = args.tile_order != nullptr ? args.tile_order[blockIdx.x] : blockIdx.x;
+  composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint>(args, block);
 }
 
 // Persistent form: a fixed grid whose workgroups pull (camera, tile) items from
@@ -232,7 +306,9 @@ composite_persistent_kernel(CompositeArgs args) {
     const int64_t item = s_item;
     __syncthreads();
     if (item >= args.total_items) break;
-    composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint>(args, item);
+    composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint>(args,
+                                                                  args.tile_order != nullptr ? args.tile_order[item]
+                                                                                             : item);
   }
 }
 
@@ -287,6 +363,19 @@ void launch_batch(int batch, bool full, const CompositeArgs& args, int64_t block
 
 }  // namespace
 
+int launch_tile_order(const CameraParams* table, const int32_t* tile_starts, const int32_t* tile_ends,
+                      int active_count, int uniform_tiles, int64_t blocks, int32_t* order, hipStream_t stream) {
+  if (blocks <= 0 || active_count <= 0) return 0;
+  hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(kOrderThreads), 0, stream, table, tile_starts, tile_ends,
+                     active_count, uniform_tiles, blocks, order);
+  const hipError_t error = hipGetLastError();
+  if (error != hipSuccess) {
+    set_error("tile order launch failed: %s", hipGetErrorString(error));
+    return VGR_E_LAUNCH;
+  }
+  return 0;
+}
+
 int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_outputs& outputs,
                      const vgr_options& options, const vgr_workspace& ws, const ScratchLayout& layout,
                      const char* scratch, bool sorted_in_out, hipStream_t stream) {
@@ -333,6 +422,16 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const v
   const bool full = options.full_sensor_output != 0;
   args.total_items = blocks;
   args.work_counter = reinterpret_cast<uint32_t*>(const_cast<char*>(scratch) + layout.work_counter);
+  // Longest tiles first pays when the grid is a few waves deep per SIMD; on deep
+  // grids the tail is short already.
+  args.tile_order = nullptr;
+  if (blocks <= kOrderedCompositeBlocks) {
+    int32_t* order = reinterpret_cast<int32_t*>(const_cast<char*>(scratch) + layout.tile_order);
+    const int order_status = launch_tile_order(args.table, args.tile_starts, args.tile_ends, args.active_count,
+                                               args.uniform_tiles, blocks, order, stream);
+    if (order_status != 0) return order_status;
+    args.tile_order = order;
+  }
   const int persistent = options.persistent_composite;
   if (persistent > 0) launch_fill_u32(args.work_counter, 1, 0u, stream);
   if (options.tile_size == 8) {

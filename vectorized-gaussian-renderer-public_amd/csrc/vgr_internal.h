@@ -101,6 +101,10 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
                     const char* scratch, const vgr_gradients& grads, hipStream_t stream);
 int launch_covariance_backward(const float* scales, const float* rotations, const float* grad_covariances,
                                int64_t count, float* grad_scales, float* grad_rotations, hipStream_t stream);
+// Writes the (camera, tile) compositing blocks [0, blocks) into `order` sorted by
+// descending tile-list length, so the longest tiles are launched first.
+int launch_tile_order(const CameraParams* table, const int32_t* tile_starts, const int32_t* tile_ends,
+                      int active_count, int uniform_tiles, int64_t blocks, int32_t* order, hipStream_t stream);
 int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream);
 int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream);
 int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream);
