@@ -1228,8 +1228,8 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
     } else {
       // The backward's batch is independent of the forward's (decisions are per pair);
       // 64 keeps the prefetch registers and LDS small enough for five waves per SIMD.
-      // One wave walks a whole tile list, so the longest list bounds the kernel:
-      // tiles are launched longest first.
+      // One wave walks a whole tile list, so a long tile that starts late runs
+      // alone at the end of the grid: tiles are launched longest first.
       int32_t* order = reinterpret_cast<int32_t*>(const_cast<char*>(scratch) + layout.tile_order);
       const int status = launch_tile_order(table, ws.tile_starts, ws.tile_ends, cameras.active_count,
                                            cargs.uniform_tiles, tile_blocks, order, stream);
