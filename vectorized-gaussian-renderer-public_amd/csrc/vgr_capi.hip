@@ -92,6 +92,8 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   offset = align_up(offset + sizeof(int64_t) * 2 * static_cast<size_t>(projection_waves + 1));
   layout.block_bases = offset;
   offset = align_up(offset + sizeof(int64_t) * 2 * static_cast<size_t>(projection_waves + wave_scan_blocks + 1));
+  layout.wave_masks = offset;
+  offset = align_up(offset + sizeof(uint64_t) * static_cast<size_t>(projection_waves + 1));
   layout.sort_histogram = offset;
   offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_slots));
   layout.sort_digit_base = offset;

@@ -44,6 +44,7 @@ struct ScratchLayout {
   size_t camera_table;     // CameraParams[active_count]
   size_t block_counts;     // int64[2 * projection_waves]: (visible, intersections) per projection wave
   size_t block_bases;      // int64[2 * (projection_waves + scan partials)]: exclusive bases per wave
+  size_t wave_masks;       // uint64[projection_waves]: group cameras with a visible pair in the wave
   size_t sort_histogram;   // uint32[passes * 256]
   size_t sort_digit_base;  // uint32[passes * 256]
   size_t sort_tickets;     // uint32[passes]

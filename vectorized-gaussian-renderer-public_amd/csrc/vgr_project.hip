@@ -152,7 +152,8 @@ __global__ void build_camera_table_kernel(
 
 
 __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(ProjectionArgs args,
-                                                                         int64_t* __restrict__ block_counts) {
+                                                                         int64_t* __restrict__ block_counts,
+                                                                         uint64_t* __restrict__ wave_masks) {
   int64_t block;
   uint64_t mask;
   const int64_t wave_slot = static_cast<int64_t>(blockIdx.x) * kProjectionWaves + threadIdx.x / kWave;
@@ -160,6 +161,7 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
     if ((threadIdx.x & (kWave - 1)) == 0) {
       block_counts[wave_slot * 2 + 0] = 0;
       block_counts[wave_slot * 2 + 1] = 0;
+      wave_masks[wave_slot] = 0;
     }
     return;
   }
@@ -170,6 +172,7 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
   float m0, m1, m2, opacity, cov[6];
   load_gaussian(args, g, loaded, m0, m1, m2, opacity, cov);
   int64_t visible = 0, intersections = 0;
+  uint64_t wave_mask = 0;  // cameras with at least one visible pair among the wave's Gaussians
   const int first = group * args.cameras_per_block;
   const int last = min(args.active_count, first + args.cameras_per_block);
   for (int a = first; a < last; ++a) {
@@ -178,10 +181,12 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
     Projected p;
     TileRect rect;
     int tiles;
-    if (evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles)) {
+    const bool seen = evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    if (seen) {
       visible += 1;
       intersections += tiles;
     }
+    if (__ballot(seen) != 0ull) wave_mask |= 1ull << (a - first);
   }
   // Counts per wave: each wave of the write kernel then owns its output range and
   // runs without block barriers.
@@ -190,14 +195,15 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
   if ((threadIdx.x & (kWave - 1)) == 0) {
     block_counts[wave_slot * 2 + 0] = visible;
     block_counts[wave_slot * 2 + 1] = intersections;
+    wave_masks[wave_slot] = wave_mask;
   }
 }
 
 template <int kDegree>
 __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
-    ProjectionArgs args, const int64_t* __restrict__ block_bases, vgr_record* __restrict__ records,
-    int32_t* __restrict__ radii, uint64_t* __restrict__ keys, int32_t* __restrict__ values,
-    int64_t visible_capacity, int64_t intersection_capacity) {
+    ProjectionArgs args, const int64_t* __restrict__ block_bases, const uint64_t* __restrict__ wave_masks,
+    vgr_record* __restrict__ records, int32_t* __restrict__ radii, uint64_t* __restrict__ keys,
+    int32_t* __restrict__ values, int64_t visible_capacity, int64_t intersection_capacity) {
   int64_t block;
   uint64_t mask;
   if (!resolve_block(args, block, mask)) return;
@@ -216,12 +222,16 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
   const int64_t wave_slot = static_cast<int64_t>(blockIdx.x) * kProjectionWaves + threadIdx.x / kWave;
   int64_t visible_base = block_bases[wave_slot * 2 + 0];
   int64_t intersection_base = block_bases[wave_slot * 2 + 1];
+  // Cameras that see none of the wave's Gaussians (per the count pass) are skipped
+  // whole: they add no records and no intersections.
+  const uint64_t wave_mask = wave_masks[wave_slot];
   const int32_t record_id = args.gaussian_ids != nullptr && loaded ? args.gaussian_ids[g] : static_cast<int32_t>(g);
   const int first = group * args.cameras_per_block;
   const int last = min(args.active_count, first + args.cameras_per_block);
   // Each wave owns its scanned output range, so the waves never synchronize.
   for (int a = first; a < last; ++a) {
-    if (((mask >> (a - first)) & 1ull) == 0) continue;  // uniform: whole block skips the camera
+    if (((mask >> (a - first)) & 1ull) == 0) continue;       // uniform: whole block skips the camera
+    if (((wave_mask >> (a - first)) & 1ull) == 0) continue;  // uniform: the wave skips the camera
     const CameraParams& cam = args.table[a];
     Projected p;
     TileRect rect;
@@ -685,6 +695,7 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
 
   int64_t* block_counts = reinterpret_cast<int64_t*>(scratch + layout.block_counts);
   int64_t* block_bases = reinterpret_cast<int64_t*>(scratch + layout.block_bases);
+  uint64_t* wave_masks = reinterpret_cast<uint64_t*>(scratch + layout.wave_masks);
   const int64_t blocks = layout.projection_blocks;
   const int64_t waves = blocks * kProjectionWaves;
   // Partials live in the tail of block_bases' region, which holds 2*(waves + scan_blocks) entries.
@@ -711,7 +722,7 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
 
   if (blocks > 0) {
     hipLaunchKernelGGL(projection_count_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0,
-                       stream, args, block_counts);
+                       stream, args, block_counts, wave_masks);
   }
   scan_pairs(block_counts, waves, block_bases, partials, ws.visible_capacity, ws.intersection_capacity,
              ws.counters, nullptr, stream);
@@ -719,19 +730,23 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
     const dim3 grid(static_cast<unsigned>(blocks)), threads(kBlockThreads);
     switch (args.sh_degree) {
       case 1:
-        hipLaunchKernelGGL(projection_write_kernel<1>, grid, threads, 0, stream, args, block_bases, ws.records,
+        hipLaunchKernelGGL(projection_write_kernel<1>, grid, threads, 0, stream, args, block_bases, wave_masks,
+                           ws.records,
                            ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
         break;
       case 2:
-        hipLaunchKernelGGL(projection_write_kernel<2>, grid, threads, 0, stream, args, block_bases, ws.records,
+        hipLaunchKernelGGL(projection_write_kernel<2>, grid, threads, 0, stream, args, block_bases, wave_masks,
+                           ws.records,
                            ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
         break;
       case 3:
-        hipLaunchKernelGGL(projection_write_kernel<3>, grid, threads, 0, stream, args, block_bases, ws.records,
+        hipLaunchKernelGGL(projection_write_kernel<3>, grid, threads, 0, stream, args, block_bases, wave_masks,
+                           ws.records,
                            ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
         break;
       default:
-        hipLaunchKernelGGL(projection_write_kernel<0>, grid, threads, 0, stream, args, block_bases, ws.records,
+        hipLaunchKernelGGL(projection_write_kernel<0>, grid, threads, 0, stream, args, block_bases, wave_masks,
+                           ws.records,
                            ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
         break;
     }
