@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define VGR_ABI_VERSION 2
+#define VGR_ABI_VERSION 3
 
 enum vgr_status {
   VGR_OK = 0,
@@ -36,15 +36,21 @@ enum vgr_status {
   VGR_E_CAPACITY = 3,
 };
 
-/* Counter block written by vgr_render (int64[5]); same order and meaning as the
- * reference's COUNTER_NAMES (cuda_backend.py:22-28). */
+/* Counter block written by vgr_render (int64[6]).  Entries 0-4 have the order
+ * and meaning of the reference's COUNTER_NAMES (cuda_backend.py:22-28; the
+ * reference requires int64[>=5]).  Entry 5 counts radix-sort look-back waits
+ * that gave up (bounded spin): nonzero means the intersection order of this
+ * render is wrong and its images must not be used; callers treat it like an
+ * overflow (re-render or fail closed).  No reference counterpart (CUB's sort
+ * has no such bound). */
 enum vgr_counter {
   VGR_COUNTER_VISIBLE = 0,
   VGR_COUNTER_INTERSECTIONS = 1,
   VGR_COUNTER_VISIBLE_OVERFLOW = 2,
   VGR_COUNTER_INTERSECTION_OVERFLOW = 3,
   VGR_COUNTER_ACTIVE_TILES = 4,
-  VGR_COUNTER_COUNT = 5,
+  VGR_COUNTER_SORT_FAILURES = 5,
+  VGR_COUNTER_COUNT = 6,
 };
 
 /* Packed scene, one entry per Gaussian over all registered scenes
@@ -131,7 +137,7 @@ typedef struct vgr_workspace {
   int32_t* values_out;
   int32_t* tile_starts;             /* [tile_capacity] */
   int32_t* tile_ends;
-  int64_t* counters;                /* [5] */
+  int64_t* counters;                /* [VGR_COUNTER_COUNT] */
   void* scratch;                    /* [scratch_bytes] */
   int64_t visible_capacity;
   int64_t intersection_capacity;
@@ -166,12 +172,16 @@ typedef struct vgr_options {
   int32_t stage_event_count;        /* 0 or 5 */
   int32_t sort_tile_items;          /* keys per thread of a 512-thread sort tile: 0 = default (16); 8, 12 or 16 */
   int64_t* statistics;              /* int64[4], zeroed then filled: [0] compositor (pixel, Gaussian)
-                                       evaluations; [1] sort look-back spin overflows; [2] projection
-                                       work items kept by chunk culling */
+                                       evaluations; [1] sort look-back give-ups (as counters[5]);
+                                       [2] projection work items kept by chunk culling */
   int32_t persistent_composite;     /* 0: one workgroup per (camera, tile); N > 0: N workgroups pull
                                        (camera, tile) items from a device queue */
   int32_t chunk_culling;            /* 1: cull (camera group x chunk) items against camera frusta using
                                        scene.chunk_bounds (scratch sized for it) */
+  int32_t lookback_spin_limit;      /* radix-sort look-back polls of an unpublished predecessor before the
+                                       wait gives up (counted in counters[VGR_COUNTER_SORT_FAILURES]):
+                                       0 = default (2^22); small values make give-ups likely and -1 makes
+                                       every wait give up at once (both for tests of the fail-closed path) */
 } vgr_options;
 
 typedef struct vgr_workspace_sizes {

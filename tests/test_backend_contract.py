@@ -89,7 +89,8 @@ class FakeNativeModule:
         values = self.counter_sequence.pop(0) if self.counter_sequence else {
             "visible_gaussians": 3, "tile_intersections": 5, "visible_overflow": 0, "intersection_overflow": 0,
             "active_tiles": 1}
-        counters.copy_(torch.tensor([values[name] for name in COUNTER_NAMES], dtype=torch.int64))
+        block = [values[name] for name in COUNTER_NAMES] + [values.get("sort_lookback_failures", 0)]
+        counters.copy_(torch.tensor(block, dtype=torch.int64))
         return self.selector
 
     def prepare_chunked_active_ids(self, active_ids, mask, chunk_ids, logical, physical):
@@ -369,10 +370,12 @@ def test_allocator_failure_restores_released_workspace(monkeypatch):
     service, backend = make_service(native, max_visible_records=4, max_intersections=8)
     service.load_scene(17, **scene_tensors(3))
     real_empty = torch.empty
-    state = {"fail": True}
+    state = {"fail": True, "installed": backend.workspace_bytes}
 
     def failing_empty(shape, *args, **kwargs):
-        if state["fail"] and isinstance(shape, tuple) and shape and shape[0] == 15:
+        # The grown workspace arena is the first allocation larger than the installed one.
+        if (state["fail"] and isinstance(shape, tuple) and len(shape) == 1 and kwargs.get("dtype") == torch.uint8
+                and shape[0] > state["installed"]):
             state["fail"] = False
             raise torch.OutOfMemoryError("injected")
         return real_empty(shape, *args, **kwargs)
@@ -416,3 +419,65 @@ def test_chunked_logical_batch_preserves_order_and_active_ids():
     assert backend.capacity_stats["execution_schedule"]["maximum_logical_chunks"] == 3
     with pytest.raises(ValueError):
         CustomHipBackend(max_physical_views=2, fixed_capacity_sort=True)  # adaptive capacity still on
+
+
+# ------------------------------------------------------------ sort look-back give-ups
+
+def sort_failure(count):
+    return {"visible_gaussians": 3, "tile_intersections": 5, "visible_overflow": 0, "intersection_overflow": 0,
+            "active_tiles": 1, "sort_lookback_failures": count}
+
+
+def test_sort_lookback_failure_is_rendered_again_without_growth():
+    native = FakeNativeModule([sort_failure(2), sort_failure(0)])
+    service, backend = make_service(native, max_visible_records=4, max_intersections=8)
+    service.load_scene(17, **scene_tensors(3))
+    view, k = cameras(1)
+    service.render(view, k, torch.tensor([17]))
+    assert native.render_calls == 2
+    assert native.capacities == [(4, 8), (4, 8)]  # re-rendered, not grown
+    stats = backend.capacity_stats
+    assert stats["totals"]["growth_events"] == 0 and stats["totals"]["retries"] == 1
+    assert stats["last_render"]["status"] == "success"
+    assert stats["last_render"]["attempts"][0]["sort_lookback_failures"] == 2
+
+
+def test_persistent_sort_lookback_failure_fails_closed():
+    native = FakeNativeModule([sort_failure(1)] * 10)
+    service, backend = make_service(native, max_visible_records=4, max_intersections=8, max_capacity_retries=2)
+    service.load_scene(17, **scene_tensors(3))
+    view, k = cameras(1)
+    outputs = service.prepare_outputs(1)
+    with pytest.raises(RendererCapacityError, match="look-back") as error:
+        service.render(view, k, torch.tensor([17]), outputs=outputs)
+    assert native.render_calls == 3
+    assert error.value.telemetry["reason"] == "sort-lookback-timeout"
+    assert torch.all(torch.isnan(outputs["rgb"])) and torch.all(torch.isnan(outputs["alpha"]))
+    assert torch.all(outputs["semantic_id"] == -1)
+
+
+def test_manual_capacity_check_reports_sort_lookback_failure():
+    native = FakeNativeModule([sort_failure(1)])
+    service, backend = make_service(native, adaptive_capacity=False)
+    service.load_scene(17, **scene_tensors(3))
+    view, k = cameras(1)
+    service.render(view, k, torch.tensor([17]))
+    assert backend.read_counters()["sort_lookback_failures"] == 1
+    with pytest.raises(RendererCapacityError, match="sort_lookback_failures=1"):
+        backend.check_capacity()
+
+
+def test_workspace_is_one_arena_of_aligned_views():
+    native = FakeNativeModule()
+    service, backend = make_service(native, batch=2, max_visible_records=7, max_intersections=33)
+    arena = backend._arena
+    base = arena.storage.data_ptr()
+    for name, spec in arena.plan.buffers.items():
+        view = backend.workspace[name]
+        offset = view.data_ptr() - base if view.numel() else arena.plan.offsets[name]
+        assert offset % 256 == 0 and offset + spec.nbytes <= arena.nbytes, name
+    assert backend.workspace_bytes == arena.storage.numel()
+    spans = sorted((arena.plan.offsets[n], arena.plan.offsets[n] + s.nbytes) for n, s in arena.plan.buffers.items()
+                   if s.nbytes)
+    assert all(a_end <= b_start for (_, a_end), (b_start, _) in zip(spans, spans[1:]))  # no overlaps
+    assert backend.workspace["counters"].numel() == 6 and int(backend.workspace["counters"].abs().sum()) == 0

@@ -105,14 +105,17 @@ def reference(params, views, intrinsics, size, sh_degree, weights, antialiased=F
     return [leaf.grad.detach() for leaf in leaves], structure, mask, opt
 
 
+# Tile size 16 runs the wave-per-tile backward compositor, tile size 8 the
+# one-pixel-per-lane kernel: both are compared with the same references.
+@pytest.mark.parametrize("tile_size", [16, 8])
 @pytest.mark.parametrize("shift", [0.0, 1.5], ids=["c4-subset", "dense-overlap"])
-def test_gradients_match_dense_fp64_autograd(shift):
+def test_gradients_match_dense_fp64_autograd(shift, tile_size):
     size, degree = 64, 3
     params, views, intrinsics = setup(shift=shift)
     weights = loss_weights(1, size, size, seed=7)
     ref, _, mask, _ = reference(params, views, intrinsics, size, degree, weights)
-    hip, _ = hip_gradients(params, views, intrinsics, size, size, degree, weights, mask)
-    report = compare(hip, ref, f"vs FP64 autograd (log-scale shift {shift})")
+    hip, _ = hip_gradients(params, views, intrinsics, size, size, degree, weights, mask, tile_size=tile_size)
+    report = compare(hip, ref, f"vs FP64 autograd (log-scale shift {shift}, tile {tile_size})")
     for name, (max_rel, l2) in report.items():
         assert max_rel < 2e-3 and l2 < 1e-3, (name, max_rel, l2)
 
@@ -138,13 +141,15 @@ def test_gradients_match_fp64_central_differences():
         assert abs(analytic - fd) <= 1e-3 * abs(fd) + 1e-6, (name, analytic, fd)
 
 
-def test_pose_and_scene_gradients_over_several_cameras():
+@pytest.mark.parametrize("tile_size", [16, 8])
+def test_pose_and_scene_gradients_over_several_cameras(tile_size):
     size, degree = 48, 1
     params, views, intrinsics = setup(count=1500, shift=1.0, sh_degree=degree, cameras=5, size=size, seed=31)
     weights = loss_weights(5, size, size, seed=13)
     ref, _, mask, _ = reference(params, views, intrinsics, size, degree, weights)
-    hip, _ = hip_gradients(params, views, intrinsics, size, size, degree, weights, mask, cameras_per_block=2)
-    report = compare(hip, ref, "5 cameras, 2 per block")
+    hip, _ = hip_gradients(params, views, intrinsics, size, size, degree, weights, mask, cameras_per_block=2,
+                           tile_size=tile_size)
+    report = compare(hip, ref, f"5 cameras, 2 per block, tile {tile_size}")
     for name, (max_rel, l2) in report.items():
         assert max_rel < 2e-3 and l2 < 1e-3, (name, max_rel, l2)
 

@@ -186,3 +186,49 @@ def test_projection_cache_and_chunked_batches_match_plain():
     for name in plain:
         np.testing.assert_array_equal(out2[name].cpu().numpy(), plain[name])
     assert chunked.read_counters()["tile_intersections"] == counters["tile_intersections"]
+
+
+def test_forced_sort_lookback_give_up_fails_closed():
+    """Every look-back wait gives up (host flag -1): the counter reports it, the
+    backend re-renders, then raises with poisoned outputs; kernels exit normally."""
+    from vgr_amd import RendererCapacityError
+
+    scene = synthetic.gaussian_scene(50_000, seed=31, sh_degree=0)
+    rig = synthetic.orbit_rig(2, 320, 240, 277.0, seed=31, radius=(9.0, 12.0))
+    dev = device()
+    backend = CustomHipBackend(output_srgb=False, lookback_spin_limit=-1, max_capacity_retries=1)
+    service = RendererService(backend, height=240, width=320, max_views=2)
+    service.initialize(stage=None, device=dev)
+    service.load_scene(0, **scene.to(dev).as_kwargs())
+    outputs = service.prepare_outputs(2)
+    with pytest.raises(RendererCapacityError, match="look-back") as error:
+        service.render(rig.viewmats.to(dev), rig.intrinsics.to(dev), torch.zeros(2, dtype=torch.int64, device=dev),
+                       outputs=outputs)
+    counters = backend.read_counters()
+    _print("forced give-up", counters)
+    assert counters["sort_lookback_failures"] > 0
+    assert error.value.telemetry["reason"] == "sort-lookback-timeout"
+    assert bool(torch.isnan(outputs["rgb"]).all()) and bool((outputs["semantic_id"] == -1).all())
+
+
+def test_small_lookback_spin_limit_is_correct_or_fails_closed():
+    """A 1-poll bound on the look-back waits: each render either keeps the exact
+    order (no give-up) or reports the give-up and fails closed, never silently wrong."""
+    from vgr_amd import RendererCapacityError
+
+    scene = synthetic.gaussian_scene(400_000, seed=20261202, sh_degree=0)
+    rig = synthetic.orbit_rig(8, 640, 480, 554.0, seed=20261202, radius=(10.0, 15.0))
+    expected = render_oracle([(0, scene)], rig, [0] * 8, want_sorted=False)
+    outcomes = []
+    for attempt in range(3):
+        try:
+            gpu, backend = render_service([(0, scene)], rig, [0] * 8, lookback_spin_limit=1, max_capacity_retries=0,
+                                          max_visible_records=4_000_000, max_intersections=40_000_000)
+        except RendererCapacityError as error:
+            assert error.telemetry["reason"] == "sort-lookback-timeout"
+            outcomes.append("failed-closed")
+            continue
+        assert backend.read_counters()["sort_lookback_failures"] == 0
+        compare_images(gpu, expected)
+        outcomes.append("exact")
+    _print("spin limit 1", outcomes)

@@ -1065,11 +1065,14 @@ __global__ void camera_grads_finalize_kernel(const CameraParams* __restrict__ ta
 
 // Zeroes the gradient rows of the stored visible records (count read on the device).
 // Device-side guard: when the forward overflowed a buffer (its lists are then
-// incomplete) every record gradient starts as NaN, so the scene and pose
-// gradients come out NaN instead of silently wrong.
+// incomplete) or its sort gave up a look-back wait (its order is then wrong),
+// every record gradient starts as NaN, so the scene and pose gradients come out
+// NaN instead of silently wrong.
 __global__ void zero_record_grads_kernel(float* __restrict__ grads, const int64_t* __restrict__ counters,
                                          int64_t capacity, int64_t intersection_capacity) {
-  const bool overflow = counters[0] > capacity || counters[1] > intersection_capacity;
+  const bool overflow = counters[VGR_COUNTER_VISIBLE] > capacity ||
+                        counters[VGR_COUNTER_INTERSECTIONS] > intersection_capacity ||
+                        counters[VGR_COUNTER_SORT_FAILURES] != 0;
   const int64_t visible = min(counters[0], capacity);
   const int64_t n = visible * kRecordGrads;
   const float value = overflow ? __int_as_float(0x7fc00000) : 0.0f;

@@ -100,8 +100,6 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_slots));
   layout.sort_tickets = offset;
   offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_count));
-  layout.sort_status = offset;
-  offset = align_up(offset + sizeof(uint32_t) * 4);
   layout.sort_flags = offset;
   offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_slots) *
                                  static_cast<size_t>(layout.sort_tiles));
@@ -161,6 +159,11 @@ int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
   }
   if (options.persistent_composite < 0 || options.persistent_composite > (1 << 20)) {
     set_error("persistent_composite must be in [0, 2^20]; got %d", options.persistent_composite);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (options.lookback_spin_limit < -1) {
+    set_error("lookback_spin_limit must be -1 (forced give-up, tests), 0 (default) or positive; got %d",
+              options.lookback_spin_limit);
     return VGR_E_INVALID_ARGUMENT;
   }
   if (options.sort_tile_items != 0 && options.sort_tile_items != 8 && options.sort_tile_items != 12 &&
@@ -370,7 +373,8 @@ extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras
   if (!opt.reuse_projection) {
     if ((status = launch_projection(sc, cams, resolved, ws, layout, scratch, stream)) != 0) return status;
     if ((status = mark(1)) != 0) return status;
-    if ((status = launch_sort(ws, layout, scratch, stream, &sorted_in_out, sc.gaussian_ids != nullptr)) != 0)
+    if ((status = launch_sort(ws, layout, scratch, static_cast<uint32_t>(opt.lookback_spin_limit), stream,
+                              &sorted_in_out, sc.gaussian_ids != nullptr)) != 0)
       return status;
     if ((status = mark(2)) != 0) return status;
     if ((status = launch_ranges(ws, tiles, sorted_in_out != 0, stream)) != 0) return status;
@@ -382,8 +386,7 @@ extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras
     return status;
   if ((status = mark(4)) != 0) return status;
   if (opt.statistics != nullptr && !opt.reuse_projection) {
-    if ((status = launch_copy_u32_to_i64(reinterpret_cast<const uint32_t*>(scratch + layout.sort_status),
-                                         opt.statistics + 1, stream)) != 0)
+    if ((status = launch_copy_i64(ws.counters + VGR_COUNTER_SORT_FAILURES, opt.statistics + 1, stream)) != 0)
       return status;
   }
   if (selector_out != nullptr) *selector_out = sorted_in_out;

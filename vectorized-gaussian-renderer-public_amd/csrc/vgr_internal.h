@@ -48,7 +48,6 @@ struct ScratchLayout {
   size_t sort_histogram;   // uint32[passes * 256]
   size_t sort_digit_base;  // uint32[passes * 256]
   size_t sort_tickets;     // uint32[passes]
-  size_t sort_status;      // uint32[4]: [0] lookback spin overflow
   size_t sort_flags;       // uint32[passes * sort_tiles * 256]
   size_t cull_counts;      // int64[2 * (projection_blocks + 1)]   (chunk culling only)
   size_t cull_bases;       // int64[2 * (projection_blocks + scan_blocks + 1)]
@@ -89,7 +88,7 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras,
                       const vgr_options& options, const vgr_workspace& ws,
                       const ScratchLayout& layout, char* scratch,
                       hipStream_t stream);
-int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scratch,
+int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, uint32_t spin_limit,
                 hipStream_t stream, int* result_in_out_buffer, bool restore_id_order);
 int launch_ranges(const vgr_workspace& ws, int64_t global_tiles, bool sorted_in_out,
                   hipStream_t stream);
@@ -109,7 +108,6 @@ int launch_tile_order(const CameraParams* table, const int32_t* tile_starts, con
 int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream);
 int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream);
 int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream);
-int launch_copy_u32_to_i64(const uint32_t* source, int64_t* target, hipStream_t stream);
 int launch_copy_i64(const int64_t* source, int64_t* target, hipStream_t stream);
 
 }  // namespace vgr

@@ -414,6 +414,7 @@ __global__ __launch_bounds__(kBlockThreads) void scan_partials_kernel(int64_t* _
       counters[VGR_COUNTER_INTERSECTION_OVERFLOW] =
           carry_b > intersection_capacity ? carry_b - intersection_capacity : 0;
       counters[VGR_COUNTER_ACTIVE_TILES] = 0;
+      counters[VGR_COUNTER_SORT_FAILURES] = 0;
     }
   }
 }

@@ -24,8 +24,9 @@
 //      tile is as large as two resident workgroups per CU allow (74 KB of LDS
 //      each): 8192 keys sort about 15% faster than 4096.
 // Ticket order guarantees every predecessor a workgroup waits on is already
-// resident; the look-back spin is additionally bounded and reports overflow in
-// a status word instead of hanging.
+// resident; the look-back spin is additionally bounded: a wait that gives up
+// adds one to counters[VGR_COUNTER_SORT_FAILURES] instead of hanging, and every
+// caller treats a nonzero count like an overflow (the order is then wrong).
 // The element count is read from the device counter block, so the sequence
 // never synchronizes the host and can be captured in a graph.
 #include <type_traits>
@@ -38,7 +39,8 @@ namespace {
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kFlagValueMask = (1u << 30) - 1u;
-constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr uint32_t kDefaultSpinLimit = 1u << 22;
+constexpr uint32_t kForceGiveUp = 0xffffffffu;  // test hook: every look-back gives up at once
 
 struct SortKeyParams {
   int depth_bits;
@@ -114,7 +116,7 @@ __global__ __launch_bounds__(kBlockThreads) void sort_histogram_kernel(const uin
 }
 
 // Exclusive scan of every pass's histogram (one block, one digit per thread),
-// then clears the tickets and status words for the passes.
+// for the per-pass digit offsets.
 __global__ __launch_bounds__(kSortThreads) void sort_digit_scan_kernel(const uint32_t* __restrict__ histogram,
                                                                         int digit_bits, int passes,
                                                                         uint32_t* __restrict__ digit_base) {
@@ -139,14 +141,13 @@ __global__ __launch_bounds__(kSortThreads) void sort_digit_scan_kernel(const uin
   }
 }
 
-// Clears histogram, tickets, status and the look-back flags of the tiles the
-// device count needs (graph-safe replacement for hipMemsetAsync).
+// Clears histogram, tickets and the look-back flags of the tiles the device
+// count needs (graph-safe replacement for hipMemsetAsync).
 __global__ __launch_bounds__(kBlockThreads) void sort_clear_kernel(const int64_t* __restrict__ counters,
                                                                    int64_t capacity, int digit_bits, int passes,
                                                                    int64_t max_tiles, int tile_keys,
                                                                    uint32_t* __restrict__ histogram,
                                                                    uint32_t* __restrict__ tickets,
-                                                                   uint32_t* __restrict__ status,
                                                                    uint32_t* __restrict__ flags) {
   const int radix = 1 << digit_bits;
   const int64_t n = element_count(counters, capacity);
@@ -158,7 +159,6 @@ __global__ __launch_bounds__(kBlockThreads) void sort_clear_kernel(const int64_t
     for (int64_t i = first; i < static_cast<int64_t>(passes) * radix; i += stride) histogram[i] = 0;
   }
   if (first < passes) tickets[first] = 0;
-  if (first < 4) status[first] = 0;
   const int64_t per_pass = tiles * radix;
   for (int p = 0; p < passes; ++p) {
     uint32_t* pass_flags = flags + static_cast<int64_t>(p) * max_tiles * radix;
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
     const uint64_t* __restrict__ keys_in, const int32_t* __restrict__ values_in, uint64_t* __restrict__ keys_out,
     int32_t* __restrict__ values_out, const int64_t* __restrict__ counters, int64_t capacity, SortKeyParams kp,
     int shift, const uint32_t* __restrict__ digit_base, uint32_t* __restrict__ flags, uint32_t* __restrict__ ticket,
-    uint32_t* __restrict__ status) {
+    int64_t* __restrict__ failures, uint32_t spin_limit) {
   constexpr int kRadix = 1 << kDigitBits;
   constexpr int kSortTile = kSortThreads * kSortItems;
   static_assert(kRadix <= kSortThreads && kSortTile <= 65535, "one digit per thread; 16-bit offsets");
@@ -298,6 +298,10 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
       int64_t predecessor = tile - 1;
       uint32_t spins = 0;
       bool found = false;
+      if (spin_limit == kForceGiveUp) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(failures), 1ull);
+        found = true;  // skip the wait; the exclusive prefix stays incomplete
+      }
       while (!found) {
         uint32_t window[kWindow];
 #pragma unroll
@@ -320,8 +324,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
         if (found) break;
         predecessor -= consumed;
         if (consumed == 0) {
-          if (++spins > kSpinLimit) {
-            atomicAdd(&status[0], 1u);
+          if (++spins > spin_limit) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(failures), 1ull);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
@@ -475,10 +479,6 @@ __global__ void fill_f32_kernel(float* __restrict__ data, int64_t n, float value
   }
 }
 
-__global__ void copy_u32_to_i64_kernel(const uint32_t* __restrict__ source, int64_t* __restrict__ target) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *target = static_cast<int64_t>(*source);
-}
-
 __global__ void copy_i64_kernel(const int64_t* __restrict__ source, int64_t* __restrict__ target) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *target = *source;
 }
@@ -493,20 +493,20 @@ unsigned fill_grid(int64_t n) {
 template <int kDigitBits>
 void launch_pass(int items, const uint64_t* keys_in, const int32_t* values_in, uint64_t* keys_out,
                  int32_t* values_out, const int64_t* counters, int64_t capacity, SortKeyParams kp, int shift,
-                 const uint32_t* digit_base, uint32_t* flags, uint32_t* ticket, uint32_t* status, int64_t tiles,
-                 hipStream_t stream) {
+                 const uint32_t* digit_base, uint32_t* flags, uint32_t* ticket, int64_t* failures,
+                 uint32_t spin_limit, int64_t tiles, hipStream_t stream) {
   if (items == 8) {
     hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 8>), dim3(static_cast<unsigned>(tiles)),
                        dim3(kSortThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
-                       kp, shift, digit_base, flags, ticket, status);
+                       kp, shift, digit_base, flags, ticket, failures, spin_limit);
   } else if (items == 12) {
     hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 12>), dim3(static_cast<unsigned>(tiles)),
                        dim3(kSortThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
-                       kp, shift, digit_base, flags, ticket, status);
+                       kp, shift, digit_base, flags, ticket, failures, spin_limit);
   } else {
     hipLaunchKernelGGL((sort_onesweep_pass_kernel<kDigitBits, 16>), dim3(static_cast<unsigned>(tiles)),
                        dim3(kSortThreads), 0, stream, keys_in, values_in, keys_out, values_out, counters, capacity,
-                       kp, shift, digit_base, flags, ticket, status);
+                       kp, shift, digit_base, flags, ticket, failures, spin_limit);
   }
 }
 
@@ -515,11 +515,6 @@ void launch_pass(int items, const uint64_t* keys_in, const int32_t* values_in, u
 int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(fill_grid(n)), dim3(kBlockThreads), 0, stream, data, n, value);
-  return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
-}
-
-int launch_copy_u32_to_i64(const uint32_t* source, int64_t* target, hipStream_t stream) {
-  hipLaunchKernelGGL(copy_u32_to_i64_kernel, dim3(1), dim3(64), 0, stream, source, target);
   return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
 }
 
@@ -540,15 +535,14 @@ int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream) {
   return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
 }
 
-int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, hipStream_t stream,
-                int* result_in_out_buffer, bool restore_id_order) {
+int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, uint32_t spin_limit,
+                hipStream_t stream, int* result_in_out_buffer, bool restore_id_order) {
   const int digit_bits = layout.sort_digit_bits;
   const int passes = layout.sort_passes;
   const int radix = 1 << digit_bits;
   uint32_t* histogram = reinterpret_cast<uint32_t*>(scratch + layout.sort_histogram);
   uint32_t* digit_base = reinterpret_cast<uint32_t*>(scratch + layout.sort_digit_base);
   uint32_t* tickets = reinterpret_cast<uint32_t*>(scratch + layout.sort_tickets);
-  uint32_t* status = reinterpret_cast<uint32_t*>(scratch + layout.sort_status);
   uint32_t* flags = reinterpret_cast<uint32_t*>(scratch + layout.sort_flags);
   SortKeyParams kp;
   kp.depth_bits = layout.depth_bits;
@@ -556,9 +550,11 @@ int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scra
   kp.key_bits = layout.sort_key_bits;
   const int64_t capacity = ws.intersection_capacity;
   const int64_t tiles = layout.sort_tiles;
+  int64_t* failures = ws.counters + VGR_COUNTER_SORT_FAILURES;
+  const uint32_t limit = spin_limit == 0 ? kDefaultSpinLimit : spin_limit;
 
   hipLaunchKernelGGL(sort_clear_kernel, dim3(1024), dim3(kBlockThreads), 0, stream, ws.counters, capacity,
-                     digit_bits, passes, tiles, layout.sort_tile, histogram, tickets, status, flags);
+                     digit_bits, passes, tiles, layout.sort_tile, histogram, tickets, flags);
   const int items = layout.sort_tile / kSortThreads;
   int64_t histogram_blocks = (capacity + kBlockThreads * 16 - 1) / (kBlockThreads * 16);
   histogram_blocks = histogram_blocks > 2048 ? 2048 : (histogram_blocks < 1 ? 1 : histogram_blocks);
@@ -577,27 +573,27 @@ int launch_sort(const vgr_workspace& ws, const ScratchLayout& layout, char* scra
     switch (digit_bits) {
       case 4:
         launch_pass<4>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
-                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+                       pass_base, pass_flags, tickets + p, failures, limit, tiles, stream);
         break;
       case 5:
         launch_pass<5>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
-                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+                       pass_base, pass_flags, tickets + p, failures, limit, tiles, stream);
         break;
       case 6:
         launch_pass<6>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
-                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+                       pass_base, pass_flags, tickets + p, failures, limit, tiles, stream);
         break;
       case 7:
         launch_pass<7>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
-                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+                       pass_base, pass_flags, tickets + p, failures, limit, tiles, stream);
         break;
       case 9:
         launch_pass<9>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
-                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+                       pass_base, pass_flags, tickets + p, failures, limit, tiles, stream);
         break;
       default:
         launch_pass<8>(items, source_keys, source_values, target_keys, target_values, ws.counters, capacity, kp, shift,
-                       pass_base, pass_flags, tickets + p, status, tiles, stream);
+                       pass_base, pass_flags, tickets + p, failures, limit, tiles, stream);
         break;
     }
     const uint64_t* next_keys = target_keys;

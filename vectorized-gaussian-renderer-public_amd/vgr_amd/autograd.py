@@ -28,7 +28,8 @@ from dataclasses import dataclass
 import torch
 
 from . import native as native_module
-from .native import CAMERA_GRAD_FLOATS, RECORD_FLOATS, RECORD_GRAD_FLOATS, infer_sh_degree
+from .native import (CAMERA_GRAD_FLOATS, COUNTER_SLOTS, RECORD_FLOATS, RECORD_GRAD_FLOATS, SORT_FAILURE_SLOT,
+                     infer_sh_degree)
 
 _GROWTH = 1.25
 
@@ -157,7 +158,7 @@ class DifferentiableRenderer:
             values_out=torch.empty(intersections, dtype=torch.int32, device=device),
             tile_starts=torch.empty(tiles, dtype=torch.int32, device=device),
             tile_ends=torch.empty(tiles, dtype=torch.int32, device=device),
-            counters=torch.zeros(5, dtype=torch.int64, device=device),
+            counters=torch.zeros(COUNTER_SLOTS, dtype=torch.int64, device=device),
             scratch=torch.empty(scratch_bytes, dtype=torch.uint8, device=device))
 
     # -- forward ---------------------------------------------------------------------------
@@ -202,14 +203,16 @@ class DifferentiableRenderer:
                 checkpoints=(transmittance, last_blended))
             call = self.native.last_call
             if self.capacity_check == "deferred" and self._calibrated:
-                pinned = torch.empty(2, dtype=torch.int64, pin_memory=True)
-                pinned.copy_(workspace.counters[:2], non_blocking=True)
+                pinned = torch.empty(COUNTER_SLOTS, dtype=torch.int64, pin_memory=True)
+                pinned.copy_(workspace.counters, non_blocking=True)
                 event = torch.cuda.Event()
                 event.record()
                 self._pending.append((event, pinned, int(workspace.records.shape[0]), int(workspace.keys_in.numel())))
                 break
-            visible, intersections = (int(v) for v in workspace.counters[:2].tolist())
-            self.last_counters = {"visible_gaussians": visible, "tile_intersections": intersections}
+            block = workspace.counters.tolist()
+            visible, intersections, sort_failures = int(block[0]), int(block[1]), int(block[SORT_FAILURE_SLOT])
+            self.last_counters = {"visible_gaussians": visible, "tile_intersections": intersections,
+                                  "sort_lookback_failures": sort_failures}
             if visible > self.max_visible or intersections > self.max_intersections:
                 factor = max(visible / self.max_visible, intersections / self.max_intersections)
                 raise _SplitCameras(math.ceil(factor * _GROWTH))
@@ -222,11 +225,16 @@ class DifferentiableRenderer:
             if intersections > workspace.keys_in.numel():
                 self.intersection_capacity = min(int(math.ceil(intersections * _GROWTH)), (1 << 30) - 1)
                 grow = True
+            if sort_failures and not grow:
+                continue  # a sort look-back wait gave up: the order is wrong, render again
             if not grow:
                 self._calibrated = True
                 break
         else:
-            raise RuntimeError("capacity growth did not converge")
+            from .hip_backend import RendererCapacityError
+
+            raise RendererCapacityError("capacity growth or the intersection sort did not converge",
+                                        telemetry={"counters": dict(self.last_counters)})
         state = {"call": call, "workspace": workspace, "covariances": covariances, "transmittance": transmittance,
                  "last_blended": last_blended, "scene": scene, "cameras": cameras, "outputs": outputs}
         return outputs, state
@@ -241,8 +249,10 @@ class DifferentiableRenderer:
                 continue
             event.synchronize()
             visible, intersections = int(pinned[0]), int(pinned[1])
-            self.last_counters = {"visible_gaussians": visible, "tile_intersections": intersections}
-            if visible > visible_capacity or intersections > intersection_capacity:
+            sort_failures = int(pinned[SORT_FAILURE_SLOT])
+            self.last_counters = {"visible_gaussians": visible, "tile_intersections": intersections,
+                                  "sort_lookback_failures": sort_failures}
+            if visible > visible_capacity or intersections > intersection_capacity or sort_failures:
                 overflow = (visible, intersections, visible_capacity, intersection_capacity)
         self._pending = keep
         if overflow is None:

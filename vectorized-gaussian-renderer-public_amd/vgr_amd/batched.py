@@ -31,7 +31,7 @@ from typing import Any
 import torch
 
 from .hip_backend import COUNTER_NAMES, RendererCapacityError
-from .native import RECORD_FLOATS, infer_sh_degree, load_native_module
+from .native import COUNTER_SLOTS, RECORD_FLOATS, SORT_FAILURE_SLOT, infer_sh_degree, load_native_module
 
 
 @dataclass(frozen=True)
@@ -261,7 +261,7 @@ class BatchedRenderer:
         for start, stop in zip(bounds[:-1], bounds[1:]):
             tiles = sum(math.ceil(c.width / ts) * math.ceil(c.height / ts) for c in self.cameras[start:stop])
             self.chunks.append((start, stop, tiles))
-        self.chunk_counters = torch.zeros((count, 5), dtype=torch.int64, device=self.device)
+        self.chunk_counters = torch.zeros((count, COUNTER_SLOTS), dtype=torch.int64, device=self.device)
         self.graph = None
 
     def _scratch_bytes(self, intersection_capacity: int) -> int:
@@ -298,7 +298,7 @@ class BatchedRenderer:
             "values_out": torch.empty(intersection_capacity, device=dev, dtype=torch.int32),
             "tile_starts": torch.empty(tiles, device=dev, dtype=torch.int32),
             "tile_ends": torch.empty(tiles, device=dev, dtype=torch.int32),
-            "counters": torch.zeros(5, device=dev, dtype=torch.int64),
+            "counters": torch.zeros(COUNTER_SLOTS, device=dev, dtype=torch.int64),
             "scratch": torch.empty(self._scratch_bytes(intersection_capacity), device=dev, dtype=torch.uint8),
         }
         self.visible_capacity = visible_capacity
@@ -349,6 +349,7 @@ class BatchedRenderer:
         torch.cuda.synchronize(self.device)
         rows = self.chunk_counters.cpu().tolist()
         totals = dict(zip(COUNTER_NAMES, (int(sum(column)) for column in zip(*rows))))
+        totals["sort_lookback_failures"] = int(sum(r[SORT_FAILURE_SLOT] for r in rows))
         totals["max_visible_gaussians"] = max(int(r[0]) for r in rows)
         totals["max_tile_intersections"] = max(int(r[1]) for r in rows)
         totals["submissions"] = len(rows)
@@ -356,6 +357,12 @@ class BatchedRenderer:
 
     def check_capacity(self) -> dict[str, int]:
         counters = self.read_counters()
+        if counters["sort_lookback_failures"]:
+            raise RendererCapacityError(
+                f"Batched renderer sort gave up {counters['sort_lookback_failures']} look-back wait(s); "
+                "the intersection order and the images of this render are invalid.",
+                telemetry={"schema_version": "renderer-capacity-error/v1", "reason": "sort-lookback-timeout",
+                           "counters": counters})
         if counters["visible_overflow"] or counters["intersection_overflow"]:
             raise RendererCapacityError(
                 "Batched renderer workspace overflow: "
@@ -389,8 +396,11 @@ class BatchedRenderer:
             counters = self.read_counters()
             over_limit = (counters["max_visible_gaussians"] > self.max_visible
                           or counters["max_tile_intersections"] > self.max_intersections)
-            if not (over_limit or counters["visible_overflow"] or counters["intersection_overflow"]):
+            overflow = over_limit or counters["visible_overflow"] or counters["intersection_overflow"]
+            if not overflow and not counters["sort_lookback_failures"]:
                 return counters
+            if not overflow:
+                continue  # a sort look-back wait gave up: render again with the same buffers
             visible = math.ceil(counters["max_visible_gaussians"] * self.capacity_headroom)
             intersections = math.ceil(counters["max_tile_intersections"] * self.capacity_headroom)
             if over_limit or visible > self.max_visible or intersections > self.max_intersections:

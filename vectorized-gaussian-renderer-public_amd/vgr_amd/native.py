@@ -44,7 +44,11 @@ WORKSPACE_TENSORS = (
 RECORD_FLOATS = 12  # 48-byte visible record, see vgr_record in include/vgr.h
 RECORD_GRAD_FLOATS = 10  # dL/d (mean_x, mean_y, conic_0..2, opacity, depth, r, g, b) per visible record
 CAMERA_GRAD_FLOATS = 24  # per-camera pose partials of the backward pass
-ABI_VERSION = 2
+ABI_VERSION = 3
+# Device counter block: the reference's five counters (cuda_backend.py:22-28) plus
+# one slot counting radix-sort look-back waits that gave up (the order is then wrong).
+COUNTER_SLOTS = 6
+SORT_FAILURE_SLOT = 5
 
 
 class _Scene(ctypes.Structure):
@@ -154,6 +158,7 @@ class _Options(ctypes.Structure):
         ("statistics", ctypes.c_void_p),
         ("persistent_composite", ctypes.c_int32),
         ("chunk_culling", ctypes.c_int32),
+        ("lookback_spin_limit", ctypes.c_int32),
     ]
 
 
@@ -376,6 +381,7 @@ class NativeRenderer:
         stage_events: Sequence[torch.cuda.Event] | None = None,
         statistics: torch.Tensor | None = None,
         checkpoints: tuple[torch.Tensor, torch.Tensor] | None = None,
+        lookback_spin_limit: int = 0,
     ) -> int:
         """One render submission; returns the sorted-buffer selector (1: *_out holds the result).
 
@@ -383,7 +389,9 @@ class NativeRenderer:
         boundaries on the render stream; ``statistics`` (int64[4] on the device)
         receives the compositor's evaluation count.  ``checkpoints`` (float32
         final transmittance and int32 last-blended position, one per pixel) makes
-        the forward store what :meth:`render_backward` needs.
+        the forward store what :meth:`render_backward` needs.  ``lookback_spin_limit``
+        (0 = the library default) bounds the sort's look-back waits; a wait that
+        gives up is counted in ``counters[SORT_FAILURE_SLOT]``.
         """
         if ray_gaussian_evaluation:
             raise ValueError("ray_gaussian_evaluation is not available on the gfx950 tile path.")
@@ -418,8 +426,8 @@ class NativeRenderer:
                 raise ValueError("all render tensors must be contiguous.")
         if device.type != "cuda":
             raise ValueError(f"render tensors must be on a GPU device; got {device}.")
-        if int(counters.numel()) < 5:
-            raise ValueError("counters workspace must contain at least five int64 values")
+        if int(counters.numel()) < COUNTER_SLOTS:
+            raise ValueError(f"counters workspace must contain at least {COUNTER_SLOTS} int64 values")
         if env_xforms.shape[0] < viewmats.shape[0]:
             raise ValueError("env_xforms must provide one transform per camera")
 
@@ -475,7 +483,7 @@ class NativeRenderer:
             sort_digit_bits=int(sort_digit_bits), raster_batch=int(raster_batch),
             cameras_per_block=int(cameras_per_block), reuse_projection=int(bool(reuse_projection)),
             sort_tile_items=int(sort_tile_items), persistent_composite=int(persistent_composite),
-            chunk_culling=int(chunk_bounds is not None))
+            chunk_culling=int(chunk_bounds is not None), lookback_spin_limit=int(lookback_spin_limit))
         event_array = None
         if stage_events is not None:
             if len(stage_events) != 5:

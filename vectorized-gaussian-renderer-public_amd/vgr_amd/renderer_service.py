@@ -1,10 +1,12 @@
 """Process-local renderer service: the public call a simulator makes.
 
-Same constructor, lifecycle and ``render`` contract as the reference service
-(``src/isaacsim_gaussian_renderer/renderer_service.py:15-237``): strict tensor
-validation, owned output sets initialised to rgb 0 / depth 0 / alpha 0 /
-semantic -1, optional caller-provided outputs, active-camera subsets, and a
-render cadence that returns the previous outputs on skipped frames.
+Public surface of the reference service (``src/isaacsim_gaussian_renderer/
+renderer_service.py:15-237``, Apache-2.0): constructor arguments,
+``initialize / load_scene / update_scene_transforms / prepare_outputs /
+render / synchronize / shutdown``, strict tensor validation, service-owned
+output sets initialised to rgb 0 / depth 0 / alpha 0 / semantic -1,
+caller-provided outputs, active-camera subsets, and a render cadence that
+returns the previous outputs on skipped frames.
 """
 
 from __future__ import annotations
@@ -17,12 +19,52 @@ import torch
 from .backend import GaussianRendererBackend, RenderRequest
 from .contracts import (
     OUTPUT_SPECS,
-    ensure_tensor,
+    TensorSpec,
     validate_active_camera_ids,
     validate_camera_batch,
     validate_outputs,
 )
 from .scene import GaussianScene, SceneRegistry
+
+# Value every owned output starts with (semantic -1 = "no label").
+_OUTPUT_FILL = {"rgb": 0.0, "depth": 0.0, "alpha": 0.0, "semantic_id": -1}
+
+
+class _OutputPool:
+    """Service-owned output sets, one per batch size, allocated once and reused."""
+
+    def __init__(self, names: tuple[str, ...], height: int, width: int) -> None:
+        self.names = names
+        self.height = height
+        self.width = width
+        self._sets: dict[int, dict[str, torch.Tensor]] = {}
+
+    def get(self, batch: int, device: torch.device) -> dict[str, torch.Tensor]:
+        existing = self._sets.get(batch)
+        if existing is not None and tuple(existing) == self.names:
+            return existing
+        fresh = {}
+        for name in self.names:
+            channels, dtype = OUTPUT_SPECS[name]
+            fresh[name] = torch.full((batch, self.height, self.width, channels), _OUTPUT_FILL[name], dtype=dtype,
+                                     device=device)
+        self._sets[batch] = fresh
+        return fresh
+
+    def clear(self) -> None:
+        self._sets = {}
+
+
+class _Cadence:
+    """Admits the 1st, (n+1)th, (2n+1)th ... render request."""
+
+    def __init__(self, every_n: int) -> None:
+        self.every_n = every_n
+        self.requests = 0
+
+    def admit(self) -> bool:
+        self.requests += 1
+        return self.every_n == 1 or self.requests % self.every_n == 1
 
 
 class RendererService:
@@ -39,17 +81,18 @@ class RendererService:
         render_every_n: int = 1,
         allow_cpu_for_tests: bool = False,
     ) -> None:
-        if height <= 0 or width <= 0:
-            raise ValueError("height and width must be positive.")
-        if render_every_n <= 0:
-            raise ValueError("render_every_n must be positive.")
-        unknown = sorted(set(outputs) - set(OUTPUT_SPECS))
-        if unknown:
-            raise ValueError(f"Unsupported output name(s): {', '.join(unknown)}.")
+        if min(height, width) <= 0:
+            raise ValueError(f"height and width must be positive; got {height}x{width}.")
+        if render_every_n < 1:
+            raise ValueError(f"render_every_n must be positive; got {render_every_n}.")
+        names = tuple(outputs)
+        unsupported = [name for name in names if name not in OUTPUT_SPECS]
+        if unsupported:
+            raise ValueError(f"Unsupported output name(s): {', '.join(sorted(unsupported))}.")
         self.backend = backend
         self.height = height
         self.width = width
-        self.output_names = tuple(outputs)
+        self.output_names = names
         self.max_views = max_views
         self.render_every_n = render_every_n
         self.allow_cpu_for_tests = allow_cpu_for_tests
@@ -57,59 +100,62 @@ class RendererService:
         self.stage: Any | None = None
         self.registry = SceneRegistry()
         self.workspace: dict[str, torch.Tensor] = {}
-        self._owned_outputs: dict[int, dict[str, torch.Tensor]] = {}
-        self._last_outputs: dict[str, torch.Tensor] | None = None
-        self._render_request_count = 0
-        self._initialized = False
+        self._pool = _OutputPool(names, height, width)
+        self._cadence = _Cadence(render_every_n)
+        self._previous: dict[str, torch.Tensor] | None = None
+
+    # ------------------------------------------------------------------ lifecycle
+    @property
+    def _require_cuda(self) -> bool:
+        return not self.allow_cpu_for_tests
+
+    def _device(self) -> torch.device:
+        if self.device is None:
+            raise RuntimeError("RendererService.initialize(stage, device) must be called first.")
+        return self.device
 
     def initialize(self, stage: Any, device: torch.device | str) -> None:
-        device = torch.device(device)
-        if device.type == "cuda" and device.index is None:
-            device = torch.device("cuda", torch.cuda.current_device())
-        if device.type != "cuda" and not self.allow_cpu_for_tests:
-            raise ValueError(f"RendererService requires a CUDA device; got {device}.")
+        resolved = torch.device(device)
+        if resolved.type == "cuda" and resolved.index is None:
+            resolved = torch.device("cuda", torch.cuda.current_device())
+        if resolved.type != "cuda" and self._require_cuda:
+            raise ValueError(f"RendererService requires a CUDA device; got {resolved}.")
+        self.backend.initialize(stage, resolved)
         self.stage = stage
-        self.device = device
-        self.backend.initialize(stage, device)
+        self.device = resolved
         if self.max_views is not None:
-            self.workspace = self.backend.allocate_workspace(
-                max_views=self.max_views, height=self.height, width=self.width, device=device, dtype=torch.float32)
-        self._initialized = True
+            self.workspace = self.backend.allocate_workspace(max_views=self.max_views, height=self.height,
+                                                             width=self.width, device=resolved,
+                                                             dtype=torch.float32)
 
-    def load_scene(
-        self,
-        scene_id: int,
-        *,
-        means: torch.Tensor,
-        scales: torch.Tensor,
-        rotations: torch.Tensor,
-        opacities: torch.Tensor,
-        features: torch.Tensor,
-        semantic_ids: torch.Tensor,
-    ) -> None:
-        self._require_initialized()
-        assert self.device is not None
-        scene = GaussianScene(means=means, scales=scales, rotations=rotations, opacities=opacities,
-                              features=features, semantic_ids=semantic_ids).validate(
-            device=self.device, require_cuda=not self.allow_cpu_for_tests)
+    def load_scene(self, scene_id: int, *, means: torch.Tensor, scales: torch.Tensor, rotations: torch.Tensor,
+                   opacities: torch.Tensor, features: torch.Tensor, semantic_ids: torch.Tensor) -> None:
+        device = self._device()
+        scene = GaussianScene(means, scales, rotations, opacities, features, semantic_ids)
+        scene.validate(device=device, require_cuda=self._require_cuda)
         offset = self.registry.register(scene_id, scene)
         self.backend.register_scene(scene_id, scene, offset)
-        # Materialize the packed offsets now so render() only reads a resident tensor.
-        self.registry.offsets_tensor(device=self.device, require_cuda=not self.allow_cpu_for_tests)
+        # Build the device offsets now so render() only reads a resident tensor.
+        self.registry.offsets_tensor(device=device, require_cuda=self._require_cuda)
 
     def update_scene_transforms(self, scene_ids: torch.Tensor, transforms: torch.Tensor) -> None:
-        self._require_initialized()
-        assert self.device is not None
-        require_cuda = not self.allow_cpu_for_tests
-        ensure_tensor(transforms, name="transforms", shape=(None, 4, 4), dtype=torch.float32, device=self.device,
-                      require_cuda=require_cuda)
-        if scene_ids.shape != (transforms.shape[0],):
-            raise ValueError("scene_ids must have one entry per transform.")
-        ensure_tensor(scene_ids, name="scene_ids", shape=(transforms.shape[0],), dtype=torch.int64,
-                      device=self.device, require_cuda=require_cuda)
-        if not require_cuda:
+        device = self._device()
+        TensorSpec((None, 4, 4), torch.float32).check(transforms, "transforms", device=device,
+                                                       require_cuda=self._require_cuda)
+        count = int(transforms.shape[0])
+        if tuple(scene_ids.shape) != (count,):
+            raise ValueError(f"scene_ids must have one entry per transform ({count}); "
+                             f"got shape {tuple(scene_ids.shape)}.")
+        TensorSpec((count,), torch.int64).check(scene_ids, "scene_ids", device=device,
+                                               require_cuda=self._require_cuda)
+        if not self._require_cuda:
             self.registry.require_scene_ids(scene_ids)
         self.backend.update_scene_transforms(scene_ids, transforms)
+
+    # --------------------------------------------------------------------- render
+    def _check_batch_size(self, batch: int) -> None:
+        if self.max_views is not None and batch > self.max_views:
+            raise ValueError(f"Batch has {batch} cameras but service max_views is {self.max_views}.")
 
     def render(
         self,
@@ -119,73 +165,43 @@ class RendererService:
         outputs: dict[str, torch.Tensor] | None = None,
         active_camera_ids: torch.Tensor | None = None,
     ) -> dict[str, torch.Tensor]:
-        self._require_initialized()
-        assert self.device is not None
-        require_cuda = not self.allow_cpu_for_tests
-        batch = validate_camera_batch(camera_transforms, intrinsics, scene_ids, device=self.device,
-                                      require_cuda=require_cuda)
-        if self.max_views is not None and batch > self.max_views:
-            raise ValueError(f"Batch has {batch} cameras but service max_views is {self.max_views}.")
-        active_camera_ids = validate_active_camera_ids(active_camera_ids, batch=batch, device=self.device,
-                                                       require_cuda=require_cuda)
-        if not require_cuda:
+        device = self._device()
+        strict = self._require_cuda
+        batch = validate_camera_batch(camera_transforms, intrinsics, scene_ids, device=device, require_cuda=strict)
+        self._check_batch_size(batch)
+        active = validate_active_camera_ids(active_camera_ids, batch=batch, device=device, require_cuda=strict)
+        if not strict:
             self.registry.require_scene_ids(scene_ids)
-        target = outputs if outputs is not None else self._owned_output_set(batch)
-        validate_outputs(target, batch=batch, height=self.height, width=self.width, device=self.device,
-                         require_cuda=require_cuda)
-        self._render_request_count += 1
-        if self.render_every_n > 1 and (self._render_request_count - 1) % self.render_every_n != 0:
-            if self._last_outputs is None:
+        target = self._pool.get(batch, device) if outputs is None else outputs
+        validate_outputs(target, batch=batch, height=self.height, width=self.width, device=device,
+                         require_cuda=strict)
+        if not self._cadence.admit():
+            if self._previous is None:
                 raise RuntimeError("Render cadence skipped before any output exists.")
-            return self._last_outputs
-        request = RenderRequest(
-            camera_transforms=camera_transforms,
-            intrinsics=intrinsics,
-            scene_ids=scene_ids,
-            scene_offsets=self.registry.offsets_tensor(device=self.device, require_cuda=require_cuda),
-            outputs=target,
-            active_camera_ids=active_camera_ids,
-            workspace=self.workspace,
-        )
-        self.backend.render(request)
-        self._last_outputs = target
+            return self._previous
+        self.backend.render(RenderRequest(
+            camera_transforms=camera_transforms, intrinsics=intrinsics, scene_ids=scene_ids,
+            scene_offsets=self.registry.offsets_tensor(device=device, require_cuda=strict),
+            outputs=target, active_camera_ids=active, workspace=self.workspace))
+        self._previous = target
         return target
 
     def prepare_outputs(self, batch: int) -> dict[str, torch.Tensor]:
-        """Allocate and keep one output set outside the measured render loop."""
-        self._require_initialized()
-        if batch <= 0:
-            raise ValueError("batch must be positive.")
-        if self.max_views is not None and batch > self.max_views:
-            raise ValueError(f"Batch has {batch} cameras but service max_views is {self.max_views}.")
-        return self._owned_output_set(batch)
+        """Allocate (once) the service-owned output set of ``batch`` cameras outside the render loop."""
+        device = self._device()
+        if batch < 1:
+            raise ValueError(f"batch must be positive; got {batch}.")
+        self._check_batch_size(batch)
+        return self._pool.get(batch, device)
 
     def synchronize(self) -> None:
-        self._require_initialized()
+        self._device()
         self.backend.synchronize()
 
     def shutdown(self) -> None:
-        if self._initialized:
+        if self.device is not None:
             self.backend.shutdown()
+        self.device = None
         self.workspace = {}
-        self._owned_outputs = {}
-        self._last_outputs = None
-        self._initialized = False
-
-    def _owned_output_set(self, batch: int) -> dict[str, torch.Tensor]:
-        assert self.device is not None
-        cached = self._owned_outputs.get(batch)
-        if cached is not None and tuple(cached) == self.output_names:
-            return cached
-        created: dict[str, torch.Tensor] = {}
-        for name in self.output_names:
-            channels, dtype = OUTPUT_SPECS[name]
-            created[name] = torch.zeros((batch, self.height, self.width, channels), device=self.device, dtype=dtype)
-            if name == "semantic_id":
-                created[name].fill_(-1)
-        self._owned_outputs[batch] = created
-        return created
-
-    def _require_initialized(self) -> None:
-        if not self._initialized:
-            raise RuntimeError("RendererService.initialize(stage, device) must be called first.")
+        self._pool.clear()
+        self._previous = None
