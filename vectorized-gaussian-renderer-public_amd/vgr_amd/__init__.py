@@ -15,6 +15,7 @@ from .hip_backend import (
     CustomHipBackend,
     RendererCapacityError,
 )
+from .ply import canonicalize_3dgs_scene, load_ply_to_canonical_gaussians, load_ply_to_gaussians
 from .renderer_service import RendererService
 from .scene import GaussianScene, SceneRegistry
 
@@ -32,4 +33,7 @@ __all__ = [
     "RendererCapacityError",
     "RendererService",
     "SceneRegistry",
+    "canonicalize_3dgs_scene",
+    "load_ply_to_canonical_gaussians",
+    "load_ply_to_gaussians",
 ]

@@ -182,3 +182,55 @@ def tensor_sha256(*tensors: torch.Tensor) -> str:
     for tensor in tensors:
         digest.update(tensor.detach().cpu().contiguous().numpy().tobytes())
     return digest.hexdigest()
+
+
+# ---------------------------------------------------------------- benchmark workloads
+# The five workloads of the benchmark, built from the generators above with fixed
+# seeds so the benchmark, the parity tests and the CPU baseline share their bytes.
+
+ORBIT_SEED, UNEVEN_SEED, TRAINING_SEED, ROBOTICS_SEED = 20261202, 20261203, 20261204, 20261205
+UNEVEN_SIZES = (200_000, 500_000, 1_000_000, 2_000_000)
+UNEVEN_RESOLUTIONS = ((128, 128), (320, 240), (640, 480), (1280, 720))
+
+
+@dataclass
+class UnevenBatch:
+    """Cameras of several resolutions over several scenes (scene id = camera index mod scenes)."""
+
+    scenes: list[tuple[int, SyntheticScene]]
+    scene_ids: list[int]
+    resolutions: list[tuple[int, int]]
+    viewmats: torch.Tensor
+    intrinsics: torch.Tensor
+
+
+def single_camera_workload() -> tuple[SyntheticScene, CameraRig]:
+    """1 camera 640x480, fx = fy = 554, 100k Gaussians SH0 (the CPU-runnable workload)."""
+    return gaussian_scene(100_000, seed=20261201, sh_degree=0), single_camera(640, 480, 554.0)
+
+
+def orbit_workload(cameras: int = 64, gaussians: int = 1_000_000) -> tuple[SyntheticScene, CameraRig]:
+    """64 cameras 640x480 (fx = fy = 554) at radius 10-15 around 1M Gaussians SH3."""
+    return (gaussian_scene(gaussians, seed=ORBIT_SEED, sh_degree=3),
+            orbit_rig(cameras, 640, 480, 554.0, seed=ORBIT_SEED, radius=(10.0, 15.0)))
+
+
+def uneven_workload(cameras: int = 256, sizes: tuple[int, ...] = UNEVEN_SIZES) -> UnevenBatch:
+    """256 cameras, resolutions U{128^2, 320x240, 640x480, 1280x720} with a 60 degree horizontal
+    FOV, at radius 10-15 around 4 scenes of 200k/500k/1M/2M Gaussians SH3, round-robin."""
+    g = torch.Generator(device="cpu").manual_seed(UNEVEN_SEED)
+    choice = torch.randint(0, len(UNEVEN_RESOLUTIONS), (cameras,), generator=g).tolist()
+    scenes = [(k, gaussian_scene(count, seed=UNEVEN_SEED + k, sh_degree=3)) for k, count in enumerate(sizes)]
+    rig = orbit_rig(cameras, 1, 1, 1.0, seed=UNEVEN_SEED, radius=(10.0, 15.0))
+    resolutions = [UNEVEN_RESOLUTIONS[c] for c in choice]
+    intrinsics = torch.stack([pinhole(1, w, h, (w / 2) / math.tan(math.radians(30)))[0] for w, h in resolutions])
+    return UnevenBatch(scenes, [i % len(sizes) for i in range(cameras)], resolutions, rig.viewmats,
+                       intrinsics.contiguous())
+
+
+def robotics_workload(trajectories: int = 32, samples: int = 32,
+                      gaussians: int = 5_000_000) -> tuple[SyntheticScene, CameraRig]:
+    """1024 cameras 160x120 (fx = fy = 139) along 32 Catmull-Rom trajectories in a 5M-Gaussian
+    SH1 room with 256 semantic classes."""
+    scene = gaussian_scene(gaussians, seed=ROBOTICS_SEED, sh_degree=1, semantic_classes=256, layout="room")
+    return scene, trajectory_rig(trajectories, samples, 160, 120, 139.0, seed=ROBOTICS_SEED)
