@@ -141,7 +141,7 @@ def test_far_scene_renders_empty_with_zero_counters():
     gpu, backend = render_service([(0, scene)], rig, [0])
     counters = backend.read_counters()
     assert counters == {"visible_gaussians": 0, "tile_intersections": 0, "visible_overflow": 0,
-                        "intersection_overflow": 0, "active_tiles": 0}
+                        "intersection_overflow": 0, "active_tiles": 0, "sort_lookback_failures": 0}
     assert np.all(gpu["alpha"] == 0) and np.all(gpu["rgb"] == 0)
     assert np.all(np.isinf(gpu["depth"])) and np.all(gpu["semantic_id"] == -1)
 
