@@ -232,6 +232,24 @@ int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras,
                    const vgr_outputs* outputs, const vgr_workspace* workspace,
                    const vgr_options* options, void* stream, int64_t* selector_out);
 
+/* Allocation-free variant of vgr_render.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Identical
+ * to vgr_render except that workspace->scratch and workspace->scratch_bytes are
+ * ignored: the scratch comes from a device array owned by the library,
+ * VGR_STATIC_SCRATCH_BYTES per device, held from the first call on a device
+ * until the process ends.  A render whose scratch requirement
+ * (vgr_workspace_sizes_query) exceeds it fails with VGR_E_CAPACITY.  The
+ * caller still passes the inputs, outputs and the other workspace buffers.
+ * Calls on one device share the array: calls that overlap in time on one
+ * device (two streams, two host threads) race and must be serialized by the
+ * caller.  Calls nothing that allocates. */
+#define VGR_STATIC_SCRATCH_BYTES (64LL << 20)
+int64_t vgr_static_scratch_bytes(void);
+int32_t vgr_render_static_scratch(const vgr_scene* scene, const vgr_cameras* cameras,
+                                  const vgr_outputs* outputs, const vgr_workspace* workspace,
+                                  const vgr_options* options, void* stream, int64_t* selector_out);
+
 /* Gradients of one checkpointed forward pass.  Pixel-indexed inputs use the
  * forward's output layout; grad_depth / grad_alpha may be NULL (zero).  Scene
  * gradients are indexed like the packed scene arrays and are ADDED to (the

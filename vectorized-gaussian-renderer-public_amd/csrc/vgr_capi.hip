@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "vgr_internal.h"
 
@@ -331,24 +332,14 @@ int prepare_render(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outpu
 }  // namespace
 }  // namespace vgr
 
-extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras, const vgr_outputs* outputs,
-                              const vgr_workspace* workspace, const vgr_options* options, void* stream_handle,
-                              int64_t* selector_out) {
-  if (scene == nullptr || cameras == nullptr || outputs == nullptr || workspace == nullptr || options == nullptr) {
-    set_error("vgr_render: null argument struct");
-    return VGR_E_INVALID_ARGUMENT;
-  }
-  const vgr_scene& sc = *scene;
-  const vgr_cameras& cams = *cameras;
-  const vgr_outputs& out = *outputs;
-  const vgr_workspace& ws = *workspace;
-  const vgr_options& opt = *options;
-  hipStream_t stream = static_cast<hipStream_t>(stream_handle);
-  ScratchLayout layout;
-  int64_t tiles = 0;
-  vgr_options resolved;
-  int status = prepare_render(sc, cams, out, ws, opt, layout, tiles, resolved);
-  if (status != 0) return status;
+namespace vgr {
+namespace {
+
+// The launch sequence of one forward submission (validated arguments).
+int render_submission(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outputs& out, const vgr_workspace& ws,
+                      const vgr_options& opt, const ScratchLayout& layout, int64_t tiles,
+                      const vgr_options& resolved, hipStream_t stream, int64_t* selector_out) {
+  int status = 0;
 
   if (opt.stage_event_count != 0 && (opt.stage_event_count != 5 || opt.stage_events == nullptr)) {
     set_error("stage_events must hold exactly 5 events");
@@ -364,16 +355,16 @@ extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras
     return 0;
   };
 
-  char* scratch = static_cast<char*>(ws.scratch);
-  CameraParams* table = reinterpret_cast<CameraParams*>(scratch + layout.camera_table);
+  char* scratch_base = static_cast<char*>(ws.scratch);
+  CameraParams* table = reinterpret_cast<CameraParams*>(scratch_base + layout.camera_table);
   if ((status = mark(0)) != 0) return status;
   if (opt.statistics != nullptr && (status = launch_fill_i64(opt.statistics, 4, 0, stream)) != 0) return status;
   if ((status = launch_camera_table(sc, cams, resolved, table, stream)) != 0) return status;
   int sorted_in_out = 1;
   if (!opt.reuse_projection) {
-    if ((status = launch_projection(sc, cams, resolved, ws, layout, scratch, stream)) != 0) return status;
+    if ((status = launch_projection(sc, cams, resolved, ws, layout, scratch_base, stream)) != 0) return status;
     if ((status = mark(1)) != 0) return status;
-    if ((status = launch_sort(ws, layout, scratch, static_cast<uint32_t>(opt.lookback_spin_limit), stream,
+    if ((status = launch_sort(ws, layout, scratch_base, static_cast<uint32_t>(opt.lookback_spin_limit), stream,
                               &sorted_in_out, sc.gaussian_ids != nullptr)) != 0)
       return status;
     if ((status = mark(2)) != 0) return status;
@@ -382,7 +373,7 @@ extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras
   } else {
     if ((status = mark(1)) != 0 || (status = mark(2)) != 0 || (status = mark(3)) != 0) return status;
   }
-  if ((status = launch_composite(sc, cams, out, resolved, ws, layout, scratch, sorted_in_out != 0, stream)) != 0)
+  if ((status = launch_composite(sc, cams, out, resolved, ws, layout, scratch_base, sorted_in_out != 0, stream)) != 0)
     return status;
   if ((status = mark(4)) != 0) return status;
   if (opt.statistics != nullptr && !opt.reuse_projection) {
@@ -392,6 +383,109 @@ extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras
   if (selector_out != nullptr) *selector_out = sorted_in_out;
   return 0;
 }
+
+}  // namespace
+}  // namespace vgr
+
+extern "C" int32_t vgr_render(const vgr_scene* scene, const vgr_cameras* cameras, const vgr_outputs* outputs,
+                              const vgr_workspace* workspace, const vgr_options* options, void* stream_handle,
+                              int64_t* selector_out) {
+  if (scene == nullptr || cameras == nullptr || outputs == nullptr || workspace == nullptr || options == nullptr) {
+    set_error("vgr_render: null argument struct");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  ScratchLayout layout;
+  int64_t tiles = 0;
+  vgr_options resolved;
+  const int status = prepare_render(*scene, *cameras, *outputs, *workspace, *options, layout, tiles, resolved);
+  if (status != 0) return status;
+  return render_submission(*scene, *cameras, *outputs, *workspace, *options, layout, tiles, resolved,
+                           static_cast<hipStream_t>(stream_handle), selector_out);
+}
+
+// ---- allocation-free variant: scratch in a library-owned device array ----
+//
+// vgr_render_static_scratch renders like vgr_render but takes its scratch
+// (camera table, projection counts and bases, sort histograms and look-back
+// flags, queue counter, tile order) from `scratch` below instead of
+// workspace->scratch, for callers that cannot allocate device memory.  Each
+// device holds its own copy of the array for the life of the process; host
+// code reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name is a host-side stand-in).  Every scratch
+// word a render reads is written or cleared by a kernel of the same render, so
+// leftovers of an earlier call are harmless; calls that overlap on one device
+// share the array and race.
+static __device__ __align__(256) unsigned char scratch[VGR_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_static_scratch_mutex;
+void* g_static_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int static_scratch_address(void** address) {
+  int device = 0;
+  hipError_t error = hipGetDevice(&device);
+  if (error != hipSuccess) {
+    set_error("vgr_render_static_scratch: hipGetDevice failed: %s", hipGetErrorString(error));
+    return VGR_E_LAUNCH;
+  }
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("vgr_render_static_scratch: device %d is beyond the %d devices supported", device, kMaxDevices);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> lock(g_static_scratch_mutex);
+  if (g_static_scratch[device] == nullptr) {
+    void* pointer = nullptr;
+    error = hipGetSymbolAddress(&pointer, HIP_SYMBOL(scratch));
+    if (error != hipSuccess || pointer == nullptr) {
+      set_error("vgr_render_static_scratch: hipGetSymbolAddress failed on device %d: %s", device,
+                hipGetErrorString(error));
+      return VGR_E_LAUNCH;
+    }
+    g_static_scratch[device] = pointer;
+  }
+  *address = g_static_scratch[device];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int64_t vgr_static_scratch_bytes(void) { return VGR_STATIC_SCRATCH_BYTES; }
+
+extern "C" int32_t vgr_render_static_scratch(const vgr_scene* scene, const vgr_cameras* cameras,
+                                             const vgr_outputs* outputs, const vgr_workspace* workspace,
+                                             const vgr_options* options, void* stream_handle,
+                                             int64_t* selector_out) {
+  if (scene == nullptr || cameras == nullptr || outputs == nullptr || workspace == nullptr || options == nullptr) {
+    set_error("vgr_render_static_scratch: null argument struct");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  void* address = nullptr;
+  int status = static_scratch_address(&address);
+  if (status != 0) return status;
+  vgr_workspace ws = *workspace;
+  ws.scratch = address;
+  ws.scratch_bytes = VGR_STATIC_SCRATCH_BYTES;
+  vgr_workspace_sizes sizes{};
+  status = vgr_workspace_sizes_query(cameras, options, scene->gaussian_count, ws.intersection_capacity, &sizes);
+  if (status != 0) return status;
+  if (sizes.scratch_bytes > VGR_STATIC_SCRATCH_BYTES) {
+    set_error("vgr_render_static_scratch: this render needs %lld scratch bytes; the static scratch holds %lld "
+              "(use vgr_render with a caller-owned scratch buffer)",
+              static_cast<long long>(sizes.scratch_bytes), static_cast<long long>(VGR_STATIC_SCRATCH_BYTES));
+    return VGR_E_CAPACITY;
+  }
+  ScratchLayout layout;
+  int64_t tiles = 0;
+  vgr_options resolved;
+  status = prepare_render(*scene, *cameras, *outputs, ws, *options, layout, tiles, resolved);
+  if (status != 0) return status;
+  return render_submission(*scene, *cameras, *outputs, ws, *options, layout, tiles, resolved,
+                           static_cast<hipStream_t>(stream_handle), selector_out);
+}
+
 
 extern "C" int32_t vgr_render_backward(const vgr_scene* scene, const vgr_cameras* cameras,
                                        const vgr_outputs* outputs, const vgr_workspace* workspace,

@@ -189,6 +189,12 @@ EXPORTED_SYMBOLS: dict[str, tuple[type, tuple]] = {
         (ctypes.POINTER(_Scene), ctypes.POINTER(_Cameras), ctypes.POINTER(_Outputs), ctypes.POINTER(_Workspace),
          ctypes.POINTER(_Options), _P, ctypes.POINTER(_I64)),
     ),
+    "vgr_static_scratch_bytes": (_I64, ()),
+    "vgr_render_static_scratch": (
+        _I32,
+        (ctypes.POINTER(_Scene), ctypes.POINTER(_Cameras), ctypes.POINTER(_Outputs), ctypes.POINTER(_Workspace),
+         ctypes.POINTER(_Options), _P, ctypes.POINTER(_I64)),
+    ),
     "vgr_render_backward": (
         _I32,
         (ctypes.POINTER(_Scene), ctypes.POINTER(_Cameras), ctypes.POINTER(_Outputs), ctypes.POINTER(_Workspace),
@@ -307,6 +313,10 @@ class NativeRenderer:
                     "vgr_workspace_sizes_query")
         return int(sizes.scratch_bytes)
 
+    def static_scratch_bytes(self) -> int:
+        """Bytes of the per-device scratch array behind ``render(..., static_scratch=True)``."""
+        return int(self.lib.vgr_static_scratch_bytes())
+
     def precompute_covariances(self, scales: torch.Tensor, rotations: torch.Tensor, out: torch.Tensor) -> None:
         count = int(scales.shape[0])
         if count == 0:
@@ -382,6 +392,7 @@ class NativeRenderer:
         statistics: torch.Tensor | None = None,
         checkpoints: tuple[torch.Tensor, torch.Tensor] | None = None,
         lookback_spin_limit: int = 0,
+        static_scratch: bool = False,
     ) -> int:
         """One render submission; returns the sorted-buffer selector (1: *_out holds the result).
 
@@ -391,7 +402,9 @@ class NativeRenderer:
         final transmittance and int32 last-blended position, one per pixel) makes
         the forward store what :meth:`render_backward` needs.  ``lookback_spin_limit``
         (0 = the library default) bounds the sort's look-back waits; a wait that
-        gives up is counted in ``counters[SORT_FAILURE_SLOT]``.
+        gives up is counted in ``counters[SORT_FAILURE_SLOT]``.  ``static_scratch`` calls
+        ``vgr_render_static_scratch``: the library's own per-device scratch array replaces the
+        ``scratch`` workspace tensor (which may then be empty), and no device memory is allocated.
         """
         if ray_gaussian_evaluation:
             raise ValueError("ray_gaussian_evaluation is not available on the gfx950 tile path.")
@@ -497,9 +510,11 @@ class NativeRenderer:
             c_options.statistics = statistics.data_ptr()
         selector = _I64(1)
         with torch.cuda.device(device):
-            self._check(self.lib.vgr_render(ctypes.byref(c_scene), ctypes.byref(c_cameras), ctypes.byref(c_outputs),
-                                            ctypes.byref(c_workspace), ctypes.byref(c_options), _stream(device),
-                                            ctypes.byref(selector)), "vgr_render")
+            entry = self.lib.vgr_render_static_scratch if static_scratch else self.lib.vgr_render
+            self._check(entry(ctypes.byref(c_scene), ctypes.byref(c_cameras), ctypes.byref(c_outputs),
+                              ctypes.byref(c_workspace), ctypes.byref(c_options), _stream(device),
+                              ctypes.byref(selector)),
+                        "vgr_render_static_scratch" if static_scratch else "vgr_render")
         self.last_call = RenderCall(c_scene, c_cameras, c_outputs, c_workspace, c_options, int(selector.value),
                                     device, event_array)
         return int(selector.value)
