@@ -210,15 +210,25 @@ def main() -> int:
         from oracle import oracle
 
         sample = list(range(min(args.cpu_cameras, args.cameras)))
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         cpu_start = time.perf_counter()
         expected = oracle.render([scene.as_kwargs()], [0], rig.viewmats[sample], rig.intrinsics[sample],
                                  [0] * len(sample), height=height, width=width, sh_degree=3, threads=threads)
         cpu_seconds = time.perf_counter() - cpu_start
+        one = sample[:2]
+        one_start = time.perf_counter()
+        oracle.render([scene.as_kwargs()], [0], rig.viewmats[one], rig.intrinsics[one], [0] * len(one),
+                      height=height, width=width, sh_degree=3, threads=1)
+        one_seconds = time.perf_counter() - one_start
         cpu_baseline = {"value": round(len(sample) * width * height / 1e6 / cpu_seconds, 4), "unit": "Mpix/s",
                         "cores": threads, "kind": "port",
                         "sample": f"first {len(sample)} of {args.cameras} cameras over the full "
-                                  f"{args.gaussians}-Gaussian SH3 scene ({cpu_seconds:.1f} s)"}
+                                  f"{args.gaussians}-Gaussian SH3 scene ({cpu_seconds:.1f} s)",
+                        "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+                        "threads_note": "16 threads: the host-CPU share of a one-GPU box",
+                        "one_core": {"value": round(len(one) * width * height / 1e6 / one_seconds, 4),
+                                     "sample": f"first {len(one)} cameras, 1 thread ({one_seconds:.1f} s)"},
+                        "config1_full": cpu_config1(threads)}
         got = {name: outputs[name][sample].cpu().numpy() for name in outputs}
         finite = np.isfinite(expected["depth"])
         depth_rel = np.abs(got["depth"][finite] - expected["depth"][finite]) / np.abs(expected["depth"][finite])
@@ -273,6 +283,73 @@ def main() -> int:
     return 0
 
 
+def oracle_parity(scene, rig, cameras: list[int], width: int, height: int, sh_degree: int, gpu_image) -> dict:
+    """Worst deviation of the GPU images of ``cameras`` from the CPU oracle; ``gpu_image(i)`` gives
+    camera i's outputs as numpy [H,W,C] arrays.  Tolerances: RGB/alpha max-abs 1e-4, depth relative
+    1e-4, semantic IDs exact."""
+    import numpy as np
+    from oracle import oracle
+
+    expected = oracle.render([scene.as_kwargs()], [0], rig.viewmats[cameras], rig.intrinsics[cameras],
+                             [0] * len(cameras), height=height, width=width, sh_degree=sh_degree,
+                             threads=cpu_threads())
+    report = {"cameras_checked": cameras, "rgb_max_abs": 0.0, "alpha_max_abs": 0.0, "depth_max_rel": 0.0,
+              "depth_background_equal": True, "semantic_mismatches": 0}
+    for local, camera in enumerate(cameras):
+        got = gpu_image(camera)
+        want = {name: expected[name][local] for name in ("rgb", "depth", "alpha", "semantic_id")}
+        report["rgb_max_abs"] = max(report["rgb_max_abs"], float(np.abs(got["rgb"] - want["rgb"]).max()))
+        report["alpha_max_abs"] = max(report["alpha_max_abs"], float(np.abs(got["alpha"] - want["alpha"]).max()))
+        finite = np.isfinite(want["depth"])
+        report["depth_background_equal"] &= bool(np.array_equal(np.isinf(got["depth"]), np.isinf(want["depth"])))
+        if finite.any():
+            rel = np.abs(got["depth"][finite] - want["depth"][finite]) / np.abs(want["depth"][finite])
+            report["depth_max_rel"] = max(report["depth_max_rel"], float(rel.max()))
+        report["semantic_mismatches"] += int((got["semantic_id"] != want["semantic_id"]).sum())
+    report["pass"] = bool(report["rgb_max_abs"] <= 1e-4 and report["alpha_max_abs"] <= 1e-4
+                          and report["depth_max_rel"] <= 1e-4 and report["depth_background_equal"]
+                          and report["semantic_mismatches"] == 0)
+    return report
+
+
+def autotune_cache() -> Path:
+    """Per-run autotuner cache (the measurement itself is what the bench reports)."""
+    return ROOT / "gpurun_out" / "autotune_bench.json"
+
+
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: the box's share for one GPU (16) or fewer if absent."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_config1(threads: int) -> dict:
+    """1 camera 640x480 (fx = fy = 554) at radius 12 around 100k SH0 Gaussians, all outputs,
+    rendered in full by the CPU oracle with ``threads`` threads and with one."""
+    from oracle import oracle
+    from vgr_amd import synthetic
+
+    scene, rig = synthetic.single_camera_workload()
+    result = {"workload": "1 camera 640x480, 100k Gaussians SH0, all four outputs (in full)"}
+    for label, count in (("threads", threads), ("one_core", 1)):
+        start = time.perf_counter()
+        oracle.render([scene.as_kwargs()], [0], rig.viewmats, rig.intrinsics, [0], height=480, width=640,
+                      sh_degree=0, threads=count)
+        seconds = time.perf_counter() - start
+        result[label] = {"threads": count, "seconds": round(seconds, 3),
+                         "mpix_s": round(640 * 480 / 1e6 / seconds, 4)}
+    return result
+
+
 def timed(torch, fn, steps: int, warmup: int) -> tuple[float, float]:
     """(mean GPU ms per call from events on the current stream, wall ms per call)."""
     for _ in range(warmup):
@@ -294,7 +371,7 @@ def bench_c4_fwd_bwd(args, torch, device) -> dict:
     """16 cameras 512x512 around 500k SH3 Gaussians; L1 + depth-L1 loss against targets
     rendered from a perturbed copy; one step = forward + loss + backward of every parameter."""
     from vgr_amd import synthetic
-    from vgr_amd.autograd import DifferentiableRenderer
+    from vgr_amd.autograd import DifferentiableRenderer, se3_exp
 
     seed, size, focal, cameras = 20261204, 512, 443.4, 16
     scene = synthetic.gaussian_scene(500_000, seed=seed, sh_degree=3)
@@ -313,6 +390,8 @@ def bench_c4_fwd_bwd(args, torch, device) -> dict:
         target = {k: v.clone() for k, v in target.items()}
     mask = (target["alpha"] > 0.5) & torch.isfinite(target["depth"])
     leaves = [p.clone().requires_grad_(True) for p in params]
+    # Camera pose as a 6-DoF twist on the base views: the step also differentiates the pose.
+    xi = torch.zeros((cameras, 6), device=device, requires_grad=True)
 
     def loss_of(out):
         depth = torch.where(mask, out["depth"], torch.zeros_like(out["depth"]))
@@ -327,20 +406,30 @@ def bench_c4_fwd_bwd(args, torch, device) -> dict:
     def step():
         for leaf in leaves:
             leaf.grad = None
-        loss_of(renderer.render(*leaves, viewmats, intrinsics)).backward()
+        xi.grad = None
+        views = (se3_exp(xi) @ viewmats).contiguous()
+        loss_of(renderer.render(*leaves, views, intrinsics)).backward()
 
     fwd_ms, fwd_wall = timed(torch, forward_only, args.extra_steps, 2)
     step_ms, step_wall = timed(torch, step, args.extra_steps, 2)
     pixels = cameras * size * size
     fwd = max(fwd_ms, fwd_wall)
     both = max(step_ms, step_wall)
-    finite = all(bool(torch.isfinite(leaf.grad).all()) for leaf in leaves)
+    finite = all(bool(torch.isfinite(leaf.grad).all()) for leaf in leaves) and bool(torch.isfinite(xi.grad).all())
+    # Forward images of the training path against the CPU oracle on two cameras.
+    with torch.no_grad():
+        images = renderer.render(*params, viewmats, intrinsics, semantic_ids=scene.semantic_ids.to(device))
+        # The oracle takes the activated parameters exactly as the renderer computed them.
+        used = synthetic.SyntheticScene(params[0].cpu(), torch.exp(params[1]).cpu(), params[2].cpu(),
+                                        torch.sigmoid(params[3]).cpu(), params[4].cpu(), scene.semantic_ids)
+    parity = oracle_parity(used, rig, [0, cameras - 1], size, size, 3,
+                           lambda i: {k: v[i].cpu().numpy() for k, v in images.items()})
     return {"workload": f"{cameras} cameras {size}x{size}, 500k Gaussians SH3 FP32, L1 + depth-L1 loss, grads of "
-                        "means/log-scales/quats/opacity logits/SH",
+                        "means/log-scales/quats/opacity logits/SH and the 6-DoF camera pose (se3_exp twist)",
             "fwd_ms": round(fwd, 3), "fwd_bwd_ms": round(both, 3), "fwd_bwd_over_fwd": round(both / fwd, 3),
             "fwd_mpix_s": round(pixels / 1e6 / (fwd / 1e3), 1),
             "fwd_bwd_mpix_s": round(pixels / 1e6 / (both / 1e3), 1),
-            "counters": renderer.last_counters, "grads_finite": finite,
+            "counters": renderer.last_counters, "grads_finite": finite, "oracle_parity": parity,
             "note": "capacity_check='deferred': counters are verified one call later without a host "
                     "synchronization; an overflowed render would give NaN gradients and raise at the next call"}
 
@@ -377,6 +466,8 @@ def bench_c5_robotics(args, torch, device) -> dict:
     renderer._submit(statistics=statistics)
     torch.cuda.synchronize()
     kept = int(statistics[2])
+    parity = oracle_parity(scene, rig, [0, rig.batch - 1], width, height, 1,
+                           lambda i: {k: v.cpu().numpy() for k, v in renderer.camera_outputs(i).items()})
     pixels = rig.batch * width * height
     batched = max(batched_ms, batched_wall)
     plain = max(plain_ms, plain_wall)
@@ -386,7 +477,8 @@ def bench_c5_robotics(args, torch, device) -> dict:
               "batched_mpix_s": round(pixels / 1e6 / (batched / 1e3), 1),
               "plain_mpix_s": round(pixels / 1e6 / (plain / 1e3), 1),
               "batched_over_plain": round(plain / batched, 3), "outputs_bitwise_equal": equal,
-              "counters": counters, "kept_cull_items": kept}
+              "batched_ordering": renderer.ordering, "fully_sorted_fallback_tiles": int(statistics[3]),
+              "oracle_parity": parity, "counters": counters, "kept_cull_items": kept}
     del renderer
     torch.cuda.empty_cache()
     # Pose-only backward on the plain training path.
@@ -418,7 +510,11 @@ def bench_c5_robotics(args, torch, device) -> dict:
 def bench_c3_uneven(args, torch, device) -> dict:
     """256 cameras with resolutions from {128^2, 320x240, 640x480, 1280x720} over 4 scenes
     (200k/500k/1M/2M SH3), round-robin: batched-persistent graph replay vs plain per-group path."""
+    import numpy as np
+
+    from oracle import oracle
     from vgr_amd import synthetic
+    from vgr_amd.autotune import Autotuner, tune_batched
     from vgr_amd.batched import BatchedRenderer, CameraSpec
 
     seed = 20261203
@@ -441,6 +537,13 @@ def bench_c3_uneven(args, torch, device) -> dict:
     renderer.plan(specs)
     counters = renderer.calibrate(viewmats, intrinsics)
     renderer.capture()
+    default_ms, default_wall = timed(torch, renderer.render, max(3, args.extra_steps // 2), 1)
+    default_config = dict(renderer.tuning, tile_size=renderer.options["tile_size"], ordering=renderer.ordering)
+    # Autotuned launch parameters (tile size, LDS batch, digit width, camera group, ordering),
+    # measured on this frame; the tuned plan is then timed like the default one.
+    tuned = tune_batched(renderer, viewmats, intrinsics, tuner=Autotuner(repeats=1, cache_path=autotune_cache()),
+                         force=True)
+    counters = renderer.read_counters()
     batched_ms, batched_wall = timed(torch, renderer.render, max(3, args.extra_steps // 2), 1)
     plain_ms, plain_wall = timed(torch, renderer.render_plain, max(3, args.extra_steps // 2), 1)
     renderer.render()
@@ -453,13 +556,41 @@ def bench_c3_uneven(args, torch, device) -> dict:
                 equal = equal and torch.equal(view[name], group["outputs"][name][local])
     pixels = sum(s.width * s.height for s in specs)
     batched = max(batched_ms, batched_wall)
+    default = max(default_ms, default_wall)
     plain = max(plain_ms, plain_wall)
+    # Oracle check on one camera of every resolution (the scenes cycle with the cameras).
+    subset = sorted({(specs[i].width, specs[i].height): i for i in range(len(specs))}.values())
+    expected = oracle.render([synthetic.gaussian_scene(count, seed=seed + k, sh_degree=3).as_kwargs()
+                              for k, count in enumerate(sizes)], list(range(4)), viewmats[subset].cpu(),
+                             intrinsics[subset].cpu(), [specs[i].scene_id for i in subset], height=1, width=1,
+                             resolutions=[(specs[i].width, specs[i].height) for i in subset], sh_degree=3,
+                             threads=cpu_threads())
+    parity = {"cameras_checked": subset, "rgb_max_abs": 0.0, "alpha_max_abs": 0.0, "depth_max_rel": 0.0,
+              "semantic_mismatches": 0}
+    offset = 0
+    for i in subset:
+        w, h = specs[i].width, specs[i].height
+        got = {k: v.reshape(h * w, -1).cpu().numpy() for k, v in renderer.camera_outputs(i).items()}
+        want = {k: expected[k][offset:offset + w * h] for k in got}
+        offset += w * h
+        parity["rgb_max_abs"] = max(parity["rgb_max_abs"], float(np.abs(got["rgb"] - want["rgb"]).max()))
+        parity["alpha_max_abs"] = max(parity["alpha_max_abs"], float(np.abs(got["alpha"] - want["alpha"]).max()))
+        finite = np.isfinite(want["depth"])
+        if finite.any():
+            rel = np.abs(got["depth"][finite] - want["depth"][finite]) / np.abs(want["depth"][finite])
+            parity["depth_max_rel"] = max(parity["depth_max_rel"], float(rel.max()))
+        parity["semantic_mismatches"] += int((got["semantic_id"] != want["semantic_id"]).sum())
+    parity["pass"] = bool(parity["rgb_max_abs"] <= 1e-4 and parity["alpha_max_abs"] <= 1e-4
+                          and parity["depth_max_rel"] <= 1e-4 and parity["semantic_mismatches"] == 0)
     return {"workload": "256 cameras, resolutions U{128^2, 320x240, 640x480, 1280x720}, 4 scenes 200k/500k/1M/2M "
                         "SH3 round-robin",
             "batched_graph_ms": round(batched, 3), "plain_ms": round(plain, 3),
             "batched_mpix_s": round(pixels / 1e6 / (batched / 1e3), 1),
             "plain_mpix_s": round(pixels / 1e6 / (plain / 1e3), 1),
-            "batched_over_plain": round(plain / batched, 3), "outputs_bitwise_equal": equal, "counters": counters,
+            "batched_over_plain": round(plain / batched, 3),
+            "default_config": default_config, "default_graph_ms": round(default, 3),
+            "tuned_config": tuned, "tuned_over_default": round(default / batched, 3),
+            "outputs_bitwise_equal": equal, "oracle_parity": parity, "counters": counters,
             "plain_groups": len(renderer.plain_groups)}
 
 

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define VGR_ABI_VERSION 3
+#define VGR_ABI_VERSION 4
 
 enum vgr_status {
   VGR_OK = 0,
@@ -95,6 +95,10 @@ typedef struct vgr_cameras {
   /* Sum of tile counts over all B cameras at options.tile_size; required when
    * widths/heights are given (the host cannot read device resolutions). */
   int64_t max_global_tiles;
+  /* Largest tile count of one camera (0: derived from width/height, or
+   * max_global_tiles with per-camera resolutions); sizes the LDS tile tables
+   * of the tile-front ordering. */
+  int64_t max_camera_tiles;
 } vgr_cameras;
 
 /* Outputs (reference output tensor list, renderer_cuda.cu:66-71).  With
@@ -143,6 +147,13 @@ typedef struct vgr_workspace {
   int64_t intersection_capacity;
   int64_t tile_capacity;            /* global tiles the tile arrays can hold */
   int64_t scratch_bytes;
+  /* Training forwards (outputs.final_transmittance set) only, else NULL:
+   * [intersection_capacity] map from intersection slot to visible record.  The
+   * sorted values are then the slots (the emission position of each
+   * intersection) instead of record indices, so the backward can store each
+   * (record, tile) pair's gradient terms in its own row without atomics.  No
+   * reference counterpart (the reference is forward-only). */
+  int32_t* slot_records;
 } vgr_workspace;
 
 /* Render options (reference scalar arguments, renderer.cpp:214-238). */
@@ -173,7 +184,8 @@ typedef struct vgr_options {
   int32_t sort_tile_items;          /* keys per thread of a 512-thread sort tile: 0 = default (16); 8, 12 or 16 */
   int64_t* statistics;              /* int64[4], zeroed then filled: [0] compositor (pixel, Gaussian)
                                        evaluations; [1] sort look-back give-ups (as counters[5]);
-                                       [2] projection work items kept by chunk culling */
+                                       [2] projection work items kept by chunk culling; [3] tiles the
+                                       tile-front ordering had to sort in full */
   int32_t persistent_composite;     /* 0: one workgroup per (camera, tile); N > 0: N workgroups pull
                                        (camera, tile) items from a device queue */
   int32_t chunk_culling;            /* 1: cull (camera group x chunk) items against camera frusta using
@@ -182,7 +194,28 @@ typedef struct vgr_options {
                                        wait gives up (counted in counters[VGR_COUNTER_SORT_FAILURES]):
                                        0 = default (2^22); small values make give-ups likely and -1 makes
                                        every wait give up at once (both for tests of the fail-closed path) */
+  int32_t ordering;                 /* VGR_ORDERING_GLOBAL_RADIX (default) or VGR_ORDERING_TILE_FRONT */
 } vgr_options;
+
+/* Intersection ordering of a render.
+ * GLOBAL_RADIX: every intersection of the batch is sorted by (tile, depth bits)
+ *   with the onesweep radix sort (the reference's DeviceRadixSort call,
+ *   renderer_cuda.cu:3215-3269); intersection capacity below 2^30.
+ * TILE_FRONT: no reference counterpart.  Intersections are bucketed per tile
+ *   (key = depth bits relative to near, value = packed Gaussian index), each
+ *   tile sorts only the nearest entries of its bucket (at most 4096, chosen by
+ *   a depth threshold) in LDS, compositing recomputes the records of the
+ *   entries it reads, and tiles where a pixel is still open at the end of a
+ *   truncated list are sorted in full and composited again.  Images are
+ *   identical to GLOBAL_RADIX; it pays when pixels saturate early (closed
+ *   scenes).  Needs all cameras active, 16x16 tiles, at most 2048 tiles per
+ *   camera, no training checkpoints; capacity up to INT32_MAX; counters[4]
+ *   counts non-empty tiles; records are not written (visible_capacity may be 1);
+ *   sorted lists are in values_out (selector 1) and keys are not kept. */
+enum vgr_ordering {
+  VGR_ORDERING_GLOBAL_RADIX = 0,
+  VGR_ORDERING_TILE_FRONT = 1,
+};
 
 typedef struct vgr_workspace_sizes {
   int64_t scratch_bytes;            /* bytes vgr_render needs in workspace.scratch */
@@ -254,13 +287,14 @@ int32_t vgr_render_static_scratch(const vgr_scene* scene, const vgr_cameras* cam
  * forward's output layout; grad_depth / grad_alpha may be NULL (zero).  Scene
  * gradients are indexed like the packed scene arrays and are ADDED to (the
  * caller zero-fills them); grad_viewmats [B,4,4] may be NULL (no pose
- * gradient).  record_grads is workspace of 10 floats per visible record
+ * gradient).  record_grads is workspace of 10 floats per intersection slot
  * (dL/d mean_x, mean_y, conic_0..2, opacity, depth, r, g, b). */
 typedef struct vgr_gradients {
   const float* grad_rgb;            /* [.,3] dL/d rgb */
   const float* grad_depth;          /* [.]   dL/d depth (ignored where depth is +inf) */
   const float* grad_alpha;          /* [.]   dL/d alpha */
-  float* record_grads;              /* [visible_capacity, 10] workspace */
+  float* record_grads;              /* [intersection_capacity, 10] workspace (one row per slot, every row
+                                       of a listed slot written by the pass: no zero fill needed) */
   float* grad_means;                /* [G,3] */
   float* grad_covariances;          /* [G,6] upper triangle, as scene.covariances */
   float* grad_opacities;            /* [G] */
@@ -274,9 +308,12 @@ typedef struct vgr_gradients {
  * the same scene, cameras, options and workspace, untouched in between: it
  * re-uses the visible records, the sorted tile lists and the projection block
  * layout left there.  Compositing runs back to front from the checkpointed
- * final transmittance (no per-intersection state is stored); projection,
- * EWA, SH and camera-pose gradients are recomputed per visible pair.  Float
- * atomics merge contributions, so results are order-dependent to rounding.
+ * final transmittance (no per-intersection state is stored) and stores each
+ * (record, tile) pair's terms in the row of its intersection slot; projection,
+ * EWA, SH and camera-pose gradients are recomputed per visible pair, which sums
+ * its slots' rows (per-pair results are deterministic).  Scene gradients of a
+ * Gaussian seen by several camera groups, and pose gradients, are merged with
+ * float atomics (order-dependent to rounding).  Needs workspace.slot_records.
  * No reference counterpart (the reference is forward-only,
  * KERNEL_ARCHITECTURE.md:454-460). */
 int32_t vgr_render_backward(const vgr_scene* scene, const vgr_cameras* cameras,

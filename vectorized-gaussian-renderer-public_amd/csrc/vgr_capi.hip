@@ -52,10 +52,35 @@ int resolve_cameras_per_block(const vgr_options& options) {
   return options.cameras_per_block == 0 ? 32 : options.cameras_per_block;
 }
 
+// Tile-front ordering: LDS budgets of the bucketed emission (count kernel
+// histogram over a camera group's tiles; write kernel tables over one camera's
+// tiles).
+constexpr int64_t kFrontMaxGroupTiles = 8192;
+constexpr int64_t kFrontMaxCameraTiles = 2048;
+
+int64_t max_camera_tiles(const vgr_cameras& cameras, const vgr_options& options) {
+  if (cameras.max_camera_tiles > 0) return cameras.max_camera_tiles;
+  if (cameras.widths != nullptr || cameras.heights != nullptr) return cameras.max_global_tiles;
+  const int64_t tiles_x = (options.width + options.tile_size - 1) / options.tile_size;
+  const int64_t tiles_y = (options.height + options.tile_size - 1) / options.tile_size;
+  return tiles_x * tiles_y;
+}
+
 ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const vgr_options& options,
-                             int64_t global_tiles, int64_t intersection_capacity) {
+                             int64_t global_tiles, int64_t intersection_capacity, int64_t camera_tiles) {
   ScratchLayout layout{};
   layout.cameras_per_block = resolve_cameras_per_block(options);
+  layout.front = options.ordering == VGR_ORDERING_TILE_FRONT;
+  if (layout.front) {
+    // A camera group's tiles must fit the count kernel's LDS histogram.
+    const int64_t per_camera = camera_tiles > 0 ? camera_tiles : 1;
+    int64_t group = kFrontMaxGroupTiles / per_camera;
+    group = group < 1 ? 1 : group;
+    if (layout.cameras_per_block > group) layout.cameras_per_block = static_cast<int32_t>(group);
+    layout.front_camera_tiles = per_camera < kFrontMaxCameraTiles ? per_camera : kFrontMaxCameraTiles;
+    const int64_t group_tiles = per_camera * layout.cameras_per_block;
+    layout.front_group_tiles = group_tiles < kFrontMaxGroupTiles ? group_tiles : kFrontMaxGroupTiles;
+  }
   layout.projection_chunks = (gaussian_count + kBlockThreads - 1) / kBlockThreads;
   layout.projection_groups = (active_count + layout.cameras_per_block - 1) / layout.cameras_per_block;
   layout.projection_blocks = layout.projection_chunks * layout.projection_groups;
@@ -102,8 +127,8 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   layout.sort_tickets = offset;
   offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_count));
   layout.sort_flags = offset;
-  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_slots) *
-                                 static_cast<size_t>(layout.sort_tiles));
+  const size_t flag_tiles = layout.front ? 0u : static_cast<size_t>(layout.sort_tiles);  // no radix pass
+  offset = align_up(offset + sizeof(uint32_t) * static_cast<size_t>(pass_slots) * flag_tiles);
   layout.work_counter = offset;
   offset = align_up(offset + sizeof(uint32_t) * 4);
   layout.tile_order = offset;
@@ -123,6 +148,25 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   offset = align_up(offset + sizeof(uint64_t) * (cull_blocks + 1));
   layout.item_totals = offset;
   offset = align_up(offset + sizeof(int64_t) * 2);
+  const size_t front_tiles = layout.front ? static_cast<size_t>(global_tiles > 0 ? global_tiles : 1) : 0u;
+  const size_t front_scan = layout.front ? static_cast<size_t>((global_tiles + kBlockThreads * 8 - 1) /
+                                                               (kBlockThreads * 8)) : 0u;
+  layout.front_counts = offset;
+  offset = align_up(offset + sizeof(int64_t) * 2 * front_tiles);
+  layout.front_bases = offset;
+  offset = align_up(offset + sizeof(int64_t) * 2 * (front_tiles + front_scan + 1));
+  layout.front_totals = offset;
+  offset = align_up(offset + sizeof(int64_t) * 2);
+  layout.front_cursors = offset;
+  offset = align_up(offset + sizeof(uint64_t) * front_tiles);
+  layout.front_lengths = offset;
+  offset = align_up(offset + sizeof(int32_t) * front_tiles);
+  layout.front_block_masks = offset;
+  offset = align_up(offset + sizeof(uint64_t) * (layout.front ? blocks + 1 : 0u));
+  layout.front_open_state = offset;
+  offset = align_up(offset + sizeof(uint32_t) * 4);
+  layout.front_open_tiles = offset;
+  offset = align_up(offset + sizeof(int32_t) * front_tiles);
   layout.total = offset;
   return layout;
 }
@@ -177,6 +221,27 @@ int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
     set_error("cameras_per_block must be in [1, 64]; got %d", options.cameras_per_block);
     return VGR_E_INVALID_ARGUMENT;
   }
+  if (options.ordering != VGR_ORDERING_GLOBAL_RADIX && options.ordering != VGR_ORDERING_TILE_FRONT) {
+    set_error("ordering must be VGR_ORDERING_GLOBAL_RADIX (0) or VGR_ORDERING_TILE_FRONT (1); got %d",
+              options.ordering);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (options.ordering == VGR_ORDERING_TILE_FRONT) {
+    if (options.tile_size != 16) {
+      set_error("tile-front ordering needs 16x16 tiles; got %d", options.tile_size);
+      return VGR_E_INVALID_ARGUMENT;
+    }
+    if (cameras.active_ids != nullptr) {
+      set_error("tile-front ordering renders every camera of the batch (active_ids must be NULL)");
+      return VGR_E_INVALID_ARGUMENT;
+    }
+    const int64_t camera_tiles = max_camera_tiles(cameras, options);
+    if (camera_tiles > kFrontMaxCameraTiles) {
+      set_error("tile-front ordering supports cameras of at most %lld tiles; this batch has %lld",
+                static_cast<long long>(kFrontMaxCameraTiles), static_cast<long long>(camera_tiles));
+      return VGR_E_INVALID_ARGUMENT;
+    }
+  }
   return 0;
 }
 
@@ -213,8 +278,8 @@ extern "C" int32_t vgr_workspace_sizes_query(const vgr_cameras* cameras, const v
   const int status = validate_options(*options, *cameras);
   if (status != 0) return status;
   const int64_t tiles = global_tile_count(*cameras, *options);
-  const ScratchLayout layout =
-      compute_layout(cameras->active_count, scene_gaussians, *options, tiles, intersection_capacity);
+  const ScratchLayout layout = compute_layout(cameras->active_count, scene_gaussians, *options, tiles,
+                                              intersection_capacity, max_camera_tiles(*cameras, *options));
   out->scratch_bytes = static_cast<int64_t>(layout.total);
   out->tile_capacity = tiles;
   out->sort_key_bits = layout.sort_key_bits;
@@ -228,7 +293,7 @@ extern "C" int64_t vgr_sort_temp_bytes(int64_t n_items, int64_t n_segments) {
   options.tile_size = 16;
   options.near_plane = 0.01f;
   options.far_plane = 100.0f;
-  const ScratchLayout layout = compute_layout(0, 0, options, n_segments, n_items);
+  const ScratchLayout layout = compute_layout(0, 0, options, n_segments, n_items, 0);
   return static_cast<int64_t>(layout.total - layout.sort_histogram);
 }
 
@@ -292,8 +357,11 @@ int prepare_render(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outpu
     set_error("visible capacity must be in [1, INT32_MAX]; got %lld", static_cast<long long>(ws.visible_capacity));
     return VGR_E_INVALID_ARGUMENT;
   }
-  if (ws.intersection_capacity <= 0 || ws.intersection_capacity >= (1LL << 30)) {
-    set_error("intersection capacity must be in [1, 2^30); got %lld",
+  // The radix sort's look-back flags carry 30-bit prefixes; the tile-front
+  // buckets are addressed with int32 tile starts.
+  const int64_t intersection_limit = opt.ordering == VGR_ORDERING_TILE_FRONT ? 0x7fffffffLL : (1LL << 30) - 1;
+  if (ws.intersection_capacity <= 0 || ws.intersection_capacity > intersection_limit) {
+    set_error("intersection capacity must be in [1, %lld]; got %lld", static_cast<long long>(intersection_limit),
               static_cast<long long>(ws.intersection_capacity));
     return VGR_E_INVALID_ARGUMENT;
   }
@@ -302,13 +370,17 @@ int prepare_render(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outpu
     set_error("global tile count %lld exceeds the int32/uint32 key range", static_cast<long long>(tiles));
     return VGR_E_INVALID_ARGUMENT;
   }
+  if (ws.slot_records != nullptr && sc.gaussian_ids != nullptr) {
+    set_error("slot-sorted (training) forwards take the scene in registration order (gaussian_ids must be NULL)");
+    return VGR_E_INVALID_ARGUMENT;
+  }
   if (ws.tile_capacity < tiles) {
     set_error("tile range workspace holds %lld tiles; the batch needs %lld", static_cast<long long>(ws.tile_capacity),
               static_cast<long long>(tiles));
     return VGR_E_INVALID_ARGUMENT;
   }
-  const ScratchLayout layout =
-      compute_layout(cams.active_count, sc.gaussian_count, opt, tiles, ws.intersection_capacity);
+  const ScratchLayout layout = compute_layout(cams.active_count, sc.gaussian_count, opt, tiles,
+                                              ws.intersection_capacity, max_camera_tiles(cams, opt));
   if (ws.scratch_bytes < static_cast<int64_t>(layout.total)) {
     set_error("scratch holds %lld bytes; the render needs %lld", static_cast<long long>(ws.scratch_bytes),
               static_cast<long long>(layout.total));
@@ -361,7 +433,13 @@ int render_submission(const vgr_scene& sc, const vgr_cameras& cams, const vgr_ou
   if (opt.statistics != nullptr && (status = launch_fill_i64(opt.statistics, 4, 0, stream)) != 0) return status;
   if ((status = launch_camera_table(sc, cams, resolved, table, stream)) != 0) return status;
   int sorted_in_out = 1;
-  if (!opt.reuse_projection) {
+  if (!opt.reuse_projection && layout.front) {
+    if ((status = launch_front_projection(sc, cams, resolved, ws, layout, scratch_base, tiles, stream)) != 0)
+      return status;
+    if ((status = mark(1)) != 0) return status;
+    if ((status = launch_front_sort(sc, ws, layout, scratch_base, tiles, stream)) != 0) return status;
+    if ((status = mark(2)) != 0 || (status = mark(3)) != 0) return status;
+  } else if (!opt.reuse_projection) {
     if ((status = launch_projection(sc, cams, resolved, ws, layout, scratch_base, stream)) != 0) return status;
     if ((status = mark(1)) != 0) return status;
     if ((status = launch_sort(ws, layout, scratch_base, static_cast<uint32_t>(opt.lookback_spin_limit), stream,
@@ -486,7 +564,6 @@ extern "C" int32_t vgr_render_static_scratch(const vgr_scene* scene, const vgr_c
                            static_cast<hipStream_t>(stream_handle), selector_out);
 }
 
-
 extern "C" int32_t vgr_render_backward(const vgr_scene* scene, const vgr_cameras* cameras,
                                        const vgr_outputs* outputs, const vgr_workspace* workspace,
                                        const vgr_options* options, const vgr_gradients* gradients,
@@ -513,6 +590,10 @@ extern "C" int32_t vgr_render_backward(const vgr_scene* scene, const vgr_cameras
   }
   if (grads.grad_rgb == nullptr || grads.record_grads == nullptr) {
     set_error("backward needs grad_rgb and the record_grads workspace");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (ws.slot_records == nullptr) {
+    set_error("backward needs the forward's workspace.slot_records (slot-sorted training forward)");
     return VGR_E_INVALID_ARGUMENT;
   }
   const bool scene_grads = grads.grad_means != nullptr || grads.grad_covariances != nullptr ||

@@ -12,7 +12,10 @@
 // The per-pixel arithmetic is the reference's, term by term, with vgr_expf in
 // place of __expf.  The semantic winner is tracked as a Gaussian index and the
 // int64 semantic label is gathered once per pixel at the end.
+#include <type_traits>
+
 #include "vgr_composite_device.h"
+#include "vgr_project_device.h"
 
 namespace vgr {
 namespace {
@@ -114,13 +117,93 @@ struct CompositeArgs {
   float semantic_min_alpha;
   float* final_transmittance;  // optional per-pixel checkpoint (training forward)
   int32_t* last_blended;       // optional: tile-list position one past the last blended splat
+  const int32_t* slot_records;  // training forward: the sorted values are slots, mapped to records here
 };
+
+// Tile-front ordering: the lists hold packed Gaussian indices (the records are
+// recomputed from the scene), a list may be a truncated front of the tile's
+// bucket (front_lengths = full bucket sizes), and a tile with a pixel still
+// open at the end of a truncated list is appended to open_tiles.
+struct FrontCompositeArgs : CompositeArgs {
+  ProjectionArgs projection;
+  const int32_t* front_lengths;  // null: no open-tile detection (the recomposite pass)
+  int32_t* open_tiles;
+  uint32_t* open_count;
+  const uint32_t* item_count;  // persistent launches: device item count (overrides total_items)
+};
+
+template <int kDegree>
+using CompositeArgsFor = typename std::conditional<(kDegree < 0), CompositeArgs, FrontCompositeArgs>::type;
+
+template <typename Args>
+__device__ __forceinline__ int64_t item_total(const Args& args) {
+  if constexpr (std::is_same<Args, FrontCompositeArgs>::value) {
+    if (args.item_count != nullptr) return static_cast<int64_t>(*args.item_count);
+  }
+  return args.total_items;
+}
+
+// The 48-byte record of (camera, packed Gaussian g), recomputed with the
+// projection write kernel's arithmetic (same functions, same operation order),
+// so it is bit-identical to the record that kernel would store.
+template <int kDegree>
+__device__ __forceinline__ void record_from_gaussian(const ProjectionArgs& pa, const CameraParams& cam, int32_t g,
+                                                     f32x4& a, f32x4& b, f32x4& c) {
+  bool loaded;
+  float m0, m1, m2, opacity, cov[6];
+  load_gaussian(pa, g, loaded, m0, m1, m2, opacity, cov);
+  Projected p;
+  if (!loaded || !project_pair(cam, m0, m1, m2, opacity, cov, pa.near_plane, pa.far_plane, pa.support_sigma,
+                               pa.covariance_epsilon, pa.antialiased != 0, p)) {
+    a = f32x4{0.0f, 0.0f, 0.0f, 0.0f};  // unreachable for listed pairs: contributes nothing
+    b = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    c = f32x4{0.0f, 0.0f, __int_as_float(-1), 0.0f};
+    return;
+  }
+  constexpr int kChannels = 3 * (kDegree + 1) * (kDegree + 1);
+  float f[kChannels];
+  load_features<kDegree>(pa.features, g, pa.feature_width, f);
+  float rgb[3];
+  sh_colour<kDegree>(f, m0 - cam.origin_x, m1 - cam.origin_y, m2 - cam.origin_z, rgb);
+  const int32_t id = pa.gaussian_ids != nullptr ? pa.gaussian_ids[g] : g;
+  a = f32x4{p.mean_x, p.mean_y, p.conic_0, p.conic_1};
+  b = f32x4{p.conic_2, p.opacity, p.depth, rgb[0]};
+  c = f32x4{rgb[1], rgb[2], __int_as_float(id), __int_as_float(cam.camera)};
+}
+
+// Gathers one batch: stored records (kDegree < 0) or records recomputed from
+// the listed Gaussians (kDegree = SH degree, tile-front lists).
+template <int kThreads, int kBatch, int kPerThread, int kDegree, bool kSlots = false>
+__device__ __forceinline__ void gather_any(const CompositeArgsFor<kDegree>& args, const CameraParams& cam,
+                                           int64_t batch_start,
+                                           int64_t range_end, int lane, f32x4 (&a)[kPerThread],
+                                           f32x4 (&b)[kPerThread], f32x4 (&c)[kPerThread]) {
+  if constexpr (kDegree < 0) {
+    gather_batch<kThreads, kBatch, kPerThread, kSlots>(args.sorted_values, args.records, args.visible_capacity,
+                                                       batch_start, range_end, lane, a, b, c, args.slot_records);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+      const int64_t index = batch_start + k * kThreads + lane;
+      if (k * kThreads + lane < kBatch && index < range_end) {
+        const int32_t g = args.sorted_values[index];
+        if (g >= 0 && g < args.projection.gaussian_count) {
+          record_from_gaussian<kDegree>(args.projection, cam, g, a[k], b[k], c[k]);
+        } else {
+          a[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+          b[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+          c[k] = f32x4{0.0f, 0.0f, __int_as_float(-1), 0.0f};
+        }
+      }
+    }
+  }
+}
 
 // Renders one (active camera, tile) work item with the whole workgroup.  With
 // kCheckpoint the final transmittance and the list position one past the last
 // blended splat are stored per pixel for the backward pass.
-template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint>
-__device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_t block) {
+template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint, int kDegree = -1>
+__device__ __forceinline__ void composite_tile(const CompositeArgsFor<kDegree>& args, int64_t block) {
   using Lists = WaveLists<kTileSize, kBatch>;
   constexpr int kThreads = Lists::kThreads;
   constexpr int kPerThread = Lists::kPerThread;
@@ -167,15 +250,15 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
   const int64_t range_end = args.tile_ends[global_tile];
   if (range_start >= 0 && range_end > range_start) {
     f32x4 next_a[kPerThread] = {}, next_b[kPerThread] = {}, next_c[kPerThread] = {};
-    gather_batch<kThreads, kBatch, kPerThread>(args.sorted_values, args.records, args.visible_capacity, range_start,
-                                               range_end, lane, next_a, next_b, next_c);
+    gather_any<kThreads, kBatch, kPerThread, kDegree, kCheckpoint>(args, cam, range_start, range_end, lane, next_a,
+                                                                   next_b, next_c);
     for (int64_t batch_start = range_start; batch_start < range_end; batch_start += kBatch) {
       const int count = static_cast<int>(range_end - batch_start < kBatch ? range_end - batch_start : kBatch);
       __syncthreads();  // previous batch fully consumed
       build_wave_lists<kTileSize, kBatch>(s, next_a, next_b, next_c, count, lane, rect_x0, rect_x1, rect_y_base);
       if (batch_start + kBatch < range_end) {
-        gather_batch<kThreads, kBatch, kPerThread>(args.sorted_values, args.records, args.visible_capacity,
-                                                   batch_start + kBatch, range_end, lane, next_a, next_b, next_c);
+        gather_any<kThreads, kBatch, kPerThread, kDegree, kCheckpoint>(args, cam, batch_start + kBatch, range_end,
+                                                                       lane, next_a, next_b, next_c);
       }
       if (!done) {
         const uint32_t list_count = s.count(wave);
@@ -246,6 +329,17 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
       if (!__syncthreads_or(done ? 0 : 1)) break;
     }
   }
+  if constexpr (kDegree >= 0) {
+    if (args.front_lengths != nullptr) {
+      // Truncated list with a pixel still open: the rest of the bucket matters;
+      // the tile is sorted in full and composited again.
+      const int64_t full = args.front_lengths[global_tile];
+      const bool truncated = (range_start >= 0 && range_end > range_start ? range_end - range_start : 0) < full;
+      if (truncated && __syncthreads_or(done ? 0 : 1) && threadIdx.x == 0) {
+        args.open_tiles[atomicAdd(args.open_count, 1u)] = static_cast<int32_t>(global_tile);
+      }
+    }
+  }
 
   if (kCount) {
     int64_t total = evaluations;
@@ -282,46 +376,84 @@ __device__ __forceinline__ void composite_tile(const CompositeArgs& args, int64_
 // means at most 64 VGPRs.  The blend loop is latency-bound, so waves in flight
 // pay more than registers.  The evaluation-counting and persistent variants
 // need a few more registers and take seven and six (no spills).
-template <int kTileSize, int kBatch, bool kCount, bool kPersistent = false>
+template <int kTileSize, int kBatch, bool kCount, bool kPersistent = false, int kDegree = -1>
 constexpr int kCompositeWavesPerSimd =
-    kTileSize != 16 ? 1 : (kBatch >= 512 ? 4 : (kPersistent ? 6 : (kCount ? 7 : 8)));  // 512: LDS-limited
+    kTileSize != 16 ? 1
+                    : (kDegree >= 2 ? 4  // recomputed records with SH2/3 coefficients in registers
+                                    : (kDegree >= 0 ? 6
+                                                    : (kBatch >= 512 ? 4 : (kPersistent ? 6 : (kCount ? 7 : 8)))));
 
-template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint>
-__global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTileSize, kBatch, kCount>)) void composite_kernel(
-    CompositeArgs args) {
+template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint, int kDegree = -1>
+__global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTileSize, kBatch, kCount, false, kDegree>)) void
+composite_kernel(CompositeArgsFor<kDegree> args) {
   const int64_t block = args.tile_order != nullptr ? args.tile_order[blockIdx.x] : blockIdx.x;
-  composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint>(args, block);
+  composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint, kDegree>(args, block);
 }
 
 // Persistent form: a fixed grid whose workgroups pull (camera, tile) items from
 // a device counter until the queue is empty; every workgroup exits once the
 // counter passes the item count.
-template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint>
-__global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTileSize, kBatch, kCount, true>)) void
-composite_persistent_kernel(CompositeArgs args) {
+template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint, int kDegree = -1>
+__global__ __launch_bounds__(kTileSize* kTileSize, (kCompositeWavesPerSimd<kTileSize, kBatch, kCount, true, kDegree>)) void
+composite_persistent_kernel(CompositeArgsFor<kDegree> args) {
   __shared__ int64_t s_item;
+  const int64_t total = item_total(args);
   for (;;) {
     if (threadIdx.x == 0) s_item = static_cast<int64_t>(atomicAdd(args.work_counter, 1u));
     __syncthreads();
     const int64_t item = s_item;
     __syncthreads();
-    if (item >= args.total_items) break;
-    composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint>(args,
-                                                                  args.tile_order != nullptr ? args.tile_order[item]
-                                                                                             : item);
+    if (item >= total) break;
+    composite_tile<kTileSize, kBatch, kFull, kCount, kCheckpoint, kDegree>(
+        args, args.tile_order != nullptr ? args.tile_order[item] : item);
   }
 }
 
-template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint = false>
-void launch_variant(const CompositeArgs& args, int64_t blocks, int persistent, hipStream_t stream) {
+template <int kTileSize, int kBatch, bool kFull, bool kCount, bool kCheckpoint = false, int kDegree = -1>
+void launch_variant(const CompositeArgsFor<kDegree>& args, int64_t blocks, int persistent, hipStream_t stream) {
   const dim3 block(kTileSize * kTileSize);
   if (persistent > 0) {
     const int64_t grid = blocks < persistent ? blocks : persistent;
-    hipLaunchKernelGGL((composite_persistent_kernel<kTileSize, kBatch, kFull, kCount, kCheckpoint>),
+    hipLaunchKernelGGL((composite_persistent_kernel<kTileSize, kBatch, kFull, kCount, kCheckpoint, kDegree>),
                        dim3(static_cast<unsigned>(grid)), block, 0, stream, args);
   } else {
-    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, kFull, kCount, kCheckpoint>), dim3(static_cast<unsigned>(blocks)),
-                       block, 0, stream, args);
+    hipLaunchKernelGGL((composite_kernel<kTileSize, kBatch, kFull, kCount, kCheckpoint, kDegree>),
+                       dim3(static_cast<unsigned>(blocks)), block, 0, stream, args);
+  }
+}
+
+// Tile-front lists (16x16 tiles, 256-entry batches): records recomputed per SH degree.
+template <int kDegree>
+void launch_projected_degree(bool full, const FrontCompositeArgs& args, int64_t blocks, int persistent,
+                             hipStream_t stream) {
+  if (args.statistics != nullptr) {
+    if (full) {
+      launch_variant<16, 256, true, true, false, kDegree>(args, blocks, persistent, stream);
+    } else {
+      launch_variant<16, 256, false, true, false, kDegree>(args, blocks, persistent, stream);
+    }
+  } else if (full) {
+    launch_variant<16, 256, true, false, false, kDegree>(args, blocks, persistent, stream);
+  } else {
+    launch_variant<16, 256, false, false, false, kDegree>(args, blocks, persistent, stream);
+  }
+}
+
+void launch_projected(int degree, bool full, const FrontCompositeArgs& args, int64_t blocks, int persistent,
+                      hipStream_t stream) {
+  switch (degree) {
+    case 1:
+      launch_projected_degree<1>(full, args, blocks, persistent, stream);
+      break;
+    case 2:
+      launch_projected_degree<2>(full, args, blocks, persistent, stream);
+      break;
+    case 3:
+      launch_projected_degree<3>(full, args, blocks, persistent, stream);
+      break;
+    default:
+      launch_projected_degree<0>(full, args, blocks, persistent, stream);
+      break;
   }
 }
 
@@ -399,6 +531,16 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const v
   args.semantic_min_alpha = options.semantic_min_alpha;
   args.final_transmittance = outputs.final_transmittance;
   args.last_blended = outputs.last_blended;
+  args.slot_records = ws.slot_records;
+  if (args.final_transmittance != nullptr && args.slot_records == nullptr) {
+    set_error("training forwards (transmittance checkpoints) need workspace.slot_records");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  const bool front = options.ordering == VGR_ORDERING_TILE_FRONT;
+  if (front && (args.final_transmittance != nullptr || options.tile_size != 16)) {
+    set_error("tile-front ordering composites 16x16 tiles without training checkpoints");
+    return VGR_E_INVALID_ARGUMENT;
+  }
   if (args.final_transmittance != nullptr &&
       (args.last_blended == nullptr || options.full_sensor_output == 0 || options.output_srgb != 0)) {
     set_error("transmittance checkpoints need last_blended, full sensor output and linear RGB");
@@ -434,7 +576,34 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const v
   }
   const int persistent = options.persistent_composite;
   if (persistent > 0) launch_fill_u32(args.work_counter, 1, 0u, stream);
-  if (options.tile_size == 8) {
+  if (front) {
+    FrontCompositeArgs front_args;
+    static_cast<CompositeArgs&>(front_args) = args;
+    make_projection_args(scene, cameras, options, layout, scratch, front_args.projection);
+    uint32_t* open_state = reinterpret_cast<uint32_t*>(const_cast<char*>(scratch) + layout.front_open_state);
+    int32_t* open_tiles = reinterpret_cast<int32_t*>(const_cast<char*>(scratch) + layout.front_open_tiles);
+    front_args.front_lengths = reinterpret_cast<const int32_t*>(scratch + layout.front_lengths);
+    front_args.open_tiles = open_tiles;
+    front_args.open_count = open_state + 0;
+    front_args.item_count = nullptr;
+    launch_fill_u32(open_state, 4, 0u, stream);
+    launch_projected(scene.sh_degree, full, front_args, blocks, persistent, stream);
+    // Tiles left open at the end of a truncated list: full sort, then composite
+    // them again from their complete lists (fixed grids that exit at once when
+    // no tile is open).
+    constexpr int kFallbackWorkgroups = 256;
+    const int status = launch_front_fallback_sort(scene, ws, layout, const_cast<char*>(scratch), kFallbackWorkgroups,
+                                                  stream);
+    if (status != 0) return status;
+    FrontCompositeArgs again = front_args;
+    again.open_tiles = nullptr;
+    again.front_lengths = nullptr;
+    again.tile_order = open_tiles;
+    again.item_count = open_state + 0;
+    again.work_counter = open_state + 2;
+    launch_projected(scene.sh_degree, full, again, blocks, kFallbackWorkgroups, stream);
+    if (options.statistics != nullptr) launch_copy_u32_to_i64(open_state + 0, options.statistics + 3, stream);
+  } else if (options.tile_size == 8) {
     launch_batch<8>(options.raster_batch, full, args, blocks, persistent, stream);
   } else {
     launch_batch<16>(options.raster_batch, full, args, blocks, persistent, stream);

@@ -29,18 +29,22 @@ __device__ __forceinline__ int find_active_camera(const CameraParams* table, int
   return lo;
 }
 
-// Loads the records of one batch of the sorted list into registers.
-template <int kThreads, int kBatch, int kPerThread>
+// Loads the records of one batch of the sorted list into registers.  With
+// kSlots the sorted values are intersection slots and slot_records maps each
+// slot to its record (training forwards, whose backward accumulates per slot).
+template <int kThreads, int kBatch, int kPerThread, bool kSlots = false>
 __device__ __forceinline__ void gather_batch(const int32_t* __restrict__ sorted_values,
                                              const vgr_record* __restrict__ records, int64_t visible_capacity,
                                              int64_t batch_start, int64_t range_end, int lane,
                                              f32x4 (&a)[kPerThread], f32x4 (&b)[kPerThread],
-                                             f32x4 (&c)[kPerThread]) {
+                                             f32x4 (&c)[kPerThread],
+                                             const int32_t* __restrict__ slot_records = nullptr) {
 #pragma unroll
   for (int k = 0; k < kPerThread; ++k) {
     const int64_t index = batch_start + k * kThreads + lane;
     if (k * kThreads + lane < kBatch && index < range_end) {
-      const int32_t record = sorted_values[index];
+      int32_t record = sorted_values[index];
+      if constexpr (kSlots) record = record >= 0 ? slot_records[record] : -1;
       if (record >= 0 && record < visible_capacity) {
         const f32x4* src = reinterpret_cast<const f32x4*>(records + record);
         a[k] = src[0];

@@ -57,8 +57,20 @@ struct ScratchLayout {
   size_t item_totals;      // int64[2]: kept item count
   size_t work_counter;     // uint32[4]: persistent compositor queue
   size_t tile_order;       // int32[global_tiles]: backward tile launch order (longest lists first)
+  // Tile-front ordering only (zero-sized otherwise).
+  size_t front_counts;       // int64[2 * global_tiles]: (entries, 0) per tile
+  size_t front_bases;        // int64[2 * (global_tiles + scan partials)]: exclusive bases per tile
+  size_t front_totals;       // int64[2]
+  size_t front_cursors;      // uint64[global_tiles]: next free slot of each bucket
+  size_t front_lengths;      // int32[global_tiles]: bucket sizes (clamped to the capacity)
+  size_t front_block_masks;  // uint64[projection_blocks]: group cameras seen by each workgroup
+  size_t front_open_state;   // uint32[4]: open-tile count, fallback-sort and recomposite queues
+  size_t front_open_tiles;   // int32[global_tiles]: tiles to sort in full and composite again
   size_t total;
   bool culling;
+  bool front;
+  int64_t front_group_tiles;   // LDS histogram size of the count kernel
+  int64_t front_camera_tiles;  // LDS tables of the write kernel
   int64_t projection_blocks;
   int64_t projection_chunks;
   int32_t projection_groups;
@@ -105,7 +117,20 @@ int launch_covariance_backward(const float* scales, const float* rotations, cons
 // descending tile-list length, so the longest tiles are launched first.
 int launch_tile_order(const CameraParams* table, const int32_t* tile_starts, const int32_t* tile_ends,
                       int active_count, int uniform_tiles, int64_t blocks, int32_t* order, hipStream_t stream);
+// Tile-front ordering (vgr_front.hip): bucketed emission, front sort, fallback full sort.
+int launch_front_projection(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_options& options,
+                            const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, int64_t global_tiles,
+                            hipStream_t stream);
+int launch_front_sort(const vgr_scene& scene, const vgr_workspace& ws, const ScratchLayout& layout, char* scratch,
+                      int64_t global_tiles, hipStream_t stream);
+int launch_front_fallback_sort(const vgr_scene& scene, const vgr_workspace& ws, const ScratchLayout& layout,
+                               char* scratch, int workgroups, hipStream_t stream);
+// Device-wide exclusive scan of n int64 pairs (vgr_project.hip); partials hold
+// 2 * ceil(n / 2048) entries.  Totals go to `totals` and/or the counter block.
+void scan_pairs(const int64_t* counts, int64_t n, int64_t* bases, int64_t* partials, int64_t visible_capacity,
+                int64_t intersection_capacity, int64_t* counters, int64_t* totals, hipStream_t stream);
 int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream);
+int launch_copy_u32_to_i64(const uint32_t* source, int64_t* target, hipStream_t stream);
 int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream);
 int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream);
 int launch_copy_i64(const int64_t* source, int64_t* target, hipStream_t stream);

@@ -203,7 +203,8 @@ template <int kDegree>
 __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
     ProjectionArgs args, const int64_t* __restrict__ block_bases, const uint64_t* __restrict__ wave_masks,
     vgr_record* __restrict__ records, int32_t* __restrict__ radii, uint64_t* __restrict__ keys,
-    int32_t* __restrict__ values, int64_t visible_capacity, int64_t intersection_capacity) {
+    int32_t* __restrict__ values, int32_t* __restrict__ slot_records, int64_t visible_capacity,
+    int64_t intersection_capacity) {
   int64_t block;
   uint64_t mask;
   if (!resolve_block(args, block, mask)) return;
@@ -295,11 +296,13 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
         if (value >= 0) {
           const uint32_t tile = cam.tile_base + static_cast<uint32_t>((min_y + dy) * cam.tiles_x + min_x + dx);
           keys[slot] = (static_cast<uint64_t>(tile) << 32) | depth_bits;
-          values[slot] = value;
+          // Training forwards sort the slot itself (slot_records maps it back).
+          values[slot] = slot_records != nullptr ? static_cast<int32_t>(slot) : value;
         } else {
           keys[slot] = kSentinelKey;
           values[slot] = -1;
         }
+        if (slot_records != nullptr) slot_records[slot] = value;
       }
     }
     visible_base += total_visible;
@@ -685,6 +688,31 @@ void make_projection_args(const vgr_scene& scene, const vgr_cameras& cameras, co
   }
 }
 
+int launch_cull(const ProjectionArgs& args, const vgr_scene& scene, const ScratchLayout& layout, char* scratch,
+                int64_t* statistics, hipStream_t stream) {
+  const int64_t blocks = layout.projection_blocks;
+  if (blocks <= 0) return 0;
+  int64_t* cull_flags = reinterpret_cast<int64_t*>(scratch + layout.cull_counts);
+  int64_t* cull_positions = reinterpret_cast<int64_t*>(scratch + layout.cull_bases);
+  uint64_t* cull_masks = reinterpret_cast<uint64_t*>(scratch + layout.cull_masks);
+  int32_t* items = reinterpret_cast<int32_t*>(scratch + layout.item_blocks);
+  uint64_t* item_masks = reinterpret_cast<uint64_t*>(scratch + layout.item_masks);
+  int64_t* item_totals = reinterpret_cast<int64_t*>(scratch + layout.item_totals);
+  const unsigned candidate_grid = static_cast<unsigned>((blocks + kBlockThreads - 1) / kBlockThreads);
+  hipLaunchKernelGGL(cull_chunks_kernel, dim3(candidate_grid), dim3(kBlockThreads), 0, stream, args,
+                     scene.chunk_bounds, blocks, cull_flags, cull_masks);
+  scan_pairs(cull_flags, blocks, cull_positions, cull_positions + 2 * blocks, 0, 0, nullptr, item_totals, stream);
+  hipLaunchKernelGGL(compact_items_kernel, dim3(candidate_grid), dim3(kBlockThreads), 0, stream, cull_flags,
+                     cull_positions, cull_masks, blocks, items, item_masks);
+  if (statistics != nullptr) launch_copy_i64(item_totals, statistics + 2, stream);
+  const hipError_t status = hipGetLastError();
+  if (status != hipSuccess) {
+    set_error("culling launch failed: %s", hipGetErrorString(status));
+    return VGR_E_LAUNCH;
+  }
+  return 0;
+}
+
 int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_options& options,
                       const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, hipStream_t stream) {
   ProjectionArgs args;
@@ -703,22 +731,11 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
   int64_t* partials = block_bases + 2 * waves;
 
   if (culled) {
-    int64_t* cull_flags = reinterpret_cast<int64_t*>(scratch + layout.cull_counts);
-    int64_t* cull_positions = reinterpret_cast<int64_t*>(scratch + layout.cull_bases);
-    uint64_t* cull_masks = reinterpret_cast<uint64_t*>(scratch + layout.cull_masks);
-    int32_t* items = reinterpret_cast<int32_t*>(scratch + layout.item_blocks);
-    uint64_t* item_masks = reinterpret_cast<uint64_t*>(scratch + layout.item_masks);
-    int64_t* item_totals = reinterpret_cast<int64_t*>(scratch + layout.item_totals);
-    const unsigned candidate_grid = static_cast<unsigned>((blocks + kBlockThreads - 1) / kBlockThreads);
-    hipLaunchKernelGGL(cull_chunks_kernel, dim3(candidate_grid), dim3(kBlockThreads), 0, stream, args,
-                       scene.chunk_bounds, blocks, cull_flags, cull_masks);
-    scan_pairs(cull_flags, blocks, cull_positions, cull_positions + 2 * blocks, 0, 0, nullptr, item_totals, stream);
-    hipLaunchKernelGGL(compact_items_kernel, dim3(candidate_grid), dim3(kBlockThreads), 0, stream, cull_flags,
-                       cull_positions, cull_masks, blocks, items, item_masks);
-    args.item_blocks = items;
-    args.item_masks = item_masks;
-    args.item_count = item_totals;
-    if (options.statistics != nullptr) launch_copy_i64(item_totals, options.statistics + 2, stream);
+    const int status = launch_cull(args, scene, layout, scratch, options.statistics, stream);
+    if (status != 0) return status;
+    args.item_blocks = reinterpret_cast<const int32_t*>(scratch + layout.item_blocks);
+    args.item_masks = reinterpret_cast<const uint64_t*>(scratch + layout.item_masks);
+    args.item_count = reinterpret_cast<const int64_t*>(scratch + layout.item_totals);
   }
 
   if (blocks > 0) {
@@ -733,22 +750,26 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
       case 1:
         hipLaunchKernelGGL(projection_write_kernel<1>, grid, threads, 0, stream, args, block_bases, wave_masks,
                            ws.records,
-                           ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
+                           ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
+                           ws.intersection_capacity);
         break;
       case 2:
         hipLaunchKernelGGL(projection_write_kernel<2>, grid, threads, 0, stream, args, block_bases, wave_masks,
                            ws.records,
-                           ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
+                           ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
+                           ws.intersection_capacity);
         break;
       case 3:
         hipLaunchKernelGGL(projection_write_kernel<3>, grid, threads, 0, stream, args, block_bases, wave_masks,
                            ws.records,
-                           ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
+                           ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
+                           ws.intersection_capacity);
         break;
       default:
         hipLaunchKernelGGL(projection_write_kernel<0>, grid, threads, 0, stream, args, block_bases, wave_masks,
                            ws.records,
-                           ws.radii, ws.keys_in, ws.values_in, ws.visible_capacity, ws.intersection_capacity);
+                           ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
+                           ws.intersection_capacity);
         break;
     }
   }

@@ -306,6 +306,12 @@ __device__ __forceinline__ void wave_inclusive_scan2(int& a, int& b) {
 void make_projection_args(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_options& options,
                           const ScratchLayout& layout, const char* scratch, dev::ProjectionArgs& args);
 
+// Chunk culling of the (camera group x chunk) candidates: leaves the kept
+// blocks, their camera masks and their count in scratch (the projection args
+// given must have no item list yet).
+int launch_cull(const dev::ProjectionArgs& args, const vgr_scene& scene, const ScratchLayout& layout, char* scratch,
+                int64_t* statistics, hipStream_t stream);
+
 }  // namespace vgr
 
 #endif  // VGR_PROJECT_DEVICE_H

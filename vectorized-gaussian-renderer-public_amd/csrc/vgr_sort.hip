@@ -483,6 +483,10 @@ __global__ void copy_i64_kernel(const int64_t* __restrict__ source, int64_t* __r
   if (threadIdx.x == 0 && blockIdx.x == 0) *target = *source;
 }
 
+__global__ void copy_u32_to_i64_kernel(const uint32_t* __restrict__ source, int64_t* __restrict__ target) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *target = static_cast<int64_t>(*source);
+}
+
 unsigned fill_grid(int64_t n) {
   int64_t blocks = (n + kBlockThreads - 1) / kBlockThreads;
   if (blocks > 4096) blocks = 4096;
@@ -520,6 +524,11 @@ int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t strea
 
 int launch_copy_i64(const int64_t* source, int64_t* target, hipStream_t stream) {
   hipLaunchKernelGGL(copy_i64_kernel, dim3(1), dim3(64), 0, stream, source, target);
+  return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
+}
+
+int launch_copy_u32_to_i64(const uint32_t* source, int64_t* target, hipStream_t stream) {
+  hipLaunchKernelGGL(copy_u32_to_i64_kernel, dim3(1), dim3(64), 0, stream, source, target);
   return hipGetLastError() == hipSuccess ? 0 : VGR_E_LAUNCH;
 }
 

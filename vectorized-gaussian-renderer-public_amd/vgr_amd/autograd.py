@@ -71,6 +71,7 @@ class _Workspace:
     tile_ends: torch.Tensor
     counters: torch.Tensor
     scratch: torch.Tensor
+    slot_records: torch.Tensor  # intersection slot -> visible record (the sort carries slots)
 
     def as_list(self) -> list[torch.Tensor]:
         return [self.records, self.keys_in, self.keys_out, self.values_in, self.values_out, self.tile_starts,
@@ -159,7 +160,8 @@ class DifferentiableRenderer:
             tile_starts=torch.empty(tiles, dtype=torch.int32, device=device),
             tile_ends=torch.empty(tiles, dtype=torch.int32, device=device),
             counters=torch.zeros(COUNTER_SLOTS, dtype=torch.int64, device=device),
-            scratch=torch.empty(scratch_bytes, dtype=torch.uint8, device=device))
+            scratch=torch.empty(scratch_bytes, dtype=torch.uint8, device=device),
+            slot_records=torch.empty(intersections, dtype=torch.int32, device=device))
 
     # -- forward ---------------------------------------------------------------------------
     def _forward(self, means, scales, quats, opacities, features, viewmats, intrinsics, env_xforms, semantic_ids,
@@ -200,7 +202,7 @@ class DifferentiableRenderer:
                 reuse_projection=False, output_srgb=False, deterministic=True, full_sensor_output=True,
                 fixed_capacity_sort=False, sh_degree=sh_degree, sort_digit_bits=self.sort_digit_bits,
                 raster_batch=self.raster_batch, cameras_per_block=cameras_per_block,
-                checkpoints=(transmittance, last_blended))
+                checkpoints=(transmittance, last_blended), slot_records=workspace.slot_records)
             call = self.native.last_call
             if self.capacity_check == "deferred" and self._calibrated:
                 pinned = torch.empty(COUNTER_SLOTS, dtype=torch.int64, pin_memory=True)
@@ -359,7 +361,9 @@ class _RasterizeFunction(torch.autograd.Function):
         means, covariances, opacities, features = (state["scene"][0], state["covariances"], state["scene"][2],
                                                    state["scene"][3])
         workspace: _Workspace = state["workspace"]
-        record_grads = torch.empty((workspace.records.shape[0], RECORD_GRAD_FLOATS), dtype=torch.float32,
+        # One row of gradient terms per intersection slot (every slot of every tile list
+        # is written by the compositing backward, so no zero fill is needed).
+        record_grads = torch.empty((workspace.keys_in.numel(), RECORD_GRAD_FLOATS), dtype=torch.float32,
                                    device=device)
         scene_grads = None
         if ctx.scene_grads:

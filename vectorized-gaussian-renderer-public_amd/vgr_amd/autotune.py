@@ -1,9 +1,10 @@
 """Launch-parameter autotuner for the batched camera path.
 
-Picks the tile size, the Gaussians staged per LDS batch of the compositor
-(``raster_batch``), the radix digit width of the sort (``sort_digit_bits``)
-and the cameras sharing one projection workgroup (``cameras_per_block``) by
-measurement on the device, and caches the winner per (device, workload
+Picks the intersection ordering (global radix sort or tile-front), the tile
+size, the Gaussians staged per LDS batch of the compositor (``raster_batch``),
+the radix digit width of the sort (``sort_digit_bits``) and the cameras
+sharing one projection workgroup (``cameras_per_block``) by measurement on the
+device, and caches the winner per (device, workload
 signature) in a JSON file so later processes start tuned.
 
 Search is coordinate descent over the knobs in a fixed order (each knob's
@@ -28,6 +29,7 @@ from typing import Callable, Mapping, Sequence
 import torch
 
 DEFAULT_SPACE: dict[str, tuple[int, ...]] = {
+    "front_ordering": (1, 0),  # 1: tile-front ordering where the plan allows it, 0: global radix sort
     "tile_size": (16, 8),
     "raster_batch": (256, 128, 512, 64),
     "sort_digit_bits": (0, 8, 7, 6),  # 0: the library default (8, or 9 where that saves a pass)

@@ -31,7 +31,10 @@ from typing import Any
 import torch
 
 from .hip_backend import COUNTER_NAMES, RendererCapacityError
-from .native import COUNTER_SLOTS, RECORD_FLOATS, SORT_FAILURE_SLOT, infer_sh_degree, load_native_module
+from .native import (COUNTER_SLOTS, FRONT_MAX_CAMERA_TILES, MAX_INTERSECTIONS, ORDERING_GLOBAL_RADIX,
+                     ORDERING_TILE_FRONT, RECORD_FLOATS, SORT_FAILURE_SLOT, infer_sh_degree, load_native_module)
+
+ORDERINGS = {"global": ORDERING_GLOBAL_RADIX, "front": ORDERING_TILE_FRONT}
 
 
 @dataclass(frozen=True)
@@ -87,7 +90,9 @@ class BatchedRenderer:
         cameras_per_block: int = 0,
         capacity_headroom: float = 1.15,
         max_visible_per_submission: int = 1 << 28,
-        max_intersections_per_submission: int = (1 << 30) - 1,
+        max_intersections_per_submission: int = (1 << 31) - 1,
+        ordering: str = "auto",
+        front_open_limit: float = 0.05,
         native_module: Any | None = None,
     ) -> None:
         self.device = torch.device(device)
@@ -111,7 +116,14 @@ class BatchedRenderer:
                            raster_batch=raster_batch, cameras_per_block=cameras_per_block)
         self.capacity_headroom = capacity_headroom
         self.max_visible = int(max_visible_per_submission)
-        self.max_intersections = min(int(max_intersections_per_submission), (1 << 30) - 1)
+        self.max_intersections_limit = int(max_intersections_per_submission)
+        if ordering not in ("global", "front", "auto"):
+            raise ValueError("ordering must be 'global', 'front' or 'auto'.")
+        self.ordering_request = ordering
+        self.front_open_limit = float(front_open_limit)
+        self.ordering = "global"  # resolved by calibrate()
+        self.plain_per_submission = 0
+        self._scratch_floor = 0
         self.chunk_count = 1
         self._scenes: dict[int, dict[str, torch.Tensor]] = {}
         self._packed: dict[str, torch.Tensor] | None = None
@@ -119,10 +131,22 @@ class BatchedRenderer:
         self.graph: torch.cuda.CUDAGraph | None = None
         self.workspace: dict[str, torch.Tensor] = {}
 
+    @property
+    def max_intersections(self) -> int:
+        """Intersection capacity limit of one submission with the current ordering."""
+        return min(self.max_intersections_limit, MAX_INTERSECTIONS[ORDERINGS[self.ordering]])
+
+    def front_eligible(self) -> bool:
+        """Whether the tile-front ordering can render this plan (16x16 tiles, small cameras)."""
+        return self.options["tile_size"] == 16 and bool(self.cameras) and self.max_camera_tiles <= FRONT_MAX_CAMERA_TILES
+
     def configure(self, *, tile_size: int | None = None, raster_batch: int | None = None,
                   sort_digit_bits: int | None = None, cameras_per_block: int | None = None,
-                  sort_tile_items: int | None = None) -> None:
-        """Change launch parameters; takes effect at the next plan()."""
+                  sort_tile_items: int | None = None, front_ordering: int | None = None) -> None:
+        """Change launch parameters; takes effect at the next plan().  ``front_ordering`` 1 asks for
+        the tile-front ordering where the plan allows it, 0 for the global radix sort."""
+        if front_ordering is not None:
+            self.ordering_request = "front" if int(front_ordering) else "global"
         if tile_size is not None:
             if tile_size not in (8, 16):
                 raise ValueError("tile_size must be 8 or 16.")
@@ -209,6 +233,9 @@ class BatchedRenderer:
             self.pixel_offsets_host.append(self.pixel_offsets_host[-1] + count)
         self.total_pixels = sum(pixels)
         self.total_tiles = sum(math.ceil(c.width / ts) * math.ceil(c.height / ts) for c in cameras)
+        self.max_camera_tiles = max(math.ceil(c.width / ts) * math.ceil(c.height / ts) for c in cameras)
+        self.ordering = "global"
+        self.plain_per_submission = 0
         dev = self.device
         self.widths = torch.tensor([c.width for c in cameras], dtype=torch.int32, device=dev)
         self.heights = torch.tensor([c.height for c in cameras], dtype=torch.int32, device=dev)
@@ -268,26 +295,33 @@ class BatchedRenderer:
         packed = self._pack()
         g = packed["means"].shape[0]
         batched = max(self.native.scratch_bytes(
-            max_views=stop - start, height=1, width=1, gaussians=g, intersection_capacity=intersection_capacity,
+            max_views=stop - start, height=1, width=1, gaussians=g,
+            intersection_capacity=min(intersection_capacity, self.max_intersections),
             tile_size=self.options["tile_size"], near_plane=self.options["near_plane"],
             far_plane=self.options["far_plane"], sort_digit_bits=self.tuning["sort_digit_bits"],
             cameras_per_block=self.tuning["cameras_per_block"], sort_tile_items=self.tuning["sort_tile_items"],
-            chunk_culling=self.chunk_culling, resolutions=[(c.width, c.height) for c in self.cameras[start:stop]])
+            chunk_culling=self.chunk_culling, resolutions=[(c.width, c.height) for c in self.cameras[start:stop]],
+            ordering=ORDERINGS[self.ordering])
             for start, stop, _ in self.chunks)
+        plain_views = self.plain_per_submission or max(stop - start for start, stop, _ in self.chunks)
         plain = self.native.scratch_bytes(
-            max_views=max(stop - start for start, stop, _ in self.chunks), height=max(c.height for c in self.cameras),
+            max_views=plain_views, height=max(c.height for c in self.cameras),
             width=max(c.width for c in self.cameras), gaussians=g, intersection_capacity=intersection_capacity,
             tile_size=self.options["tile_size"], near_plane=self.options["near_plane"],
             far_plane=self.options["far_plane"], sort_digit_bits=self.tuning["sort_digit_bits"],
             cameras_per_block=self.tuning["cameras_per_block"], sort_tile_items=self.tuning["sort_tile_items"])
-        return max(batched, plain)
+        # Buffers are shared by both orderings and the plain path: never shrink below
+        # what an earlier calibration needed.
+        self._scratch_floor = max(self._scratch_floor, batched, plain)
+        return self._scratch_floor
 
     def _allocate(self, visible_capacity: int, intersection_capacity: int) -> None:
         dev = self.device
         self.workspace = {}
         torch.cuda.empty_cache()
         ts = self.options["tile_size"]
-        plain_tiles = max(stop - start for start, stop, _ in self.chunks) * \
+        plain_views = self.plain_per_submission or max(stop - start for start, stop, _ in self.chunks)
+        plain_tiles = plain_views * \
             math.ceil(max(c.width for c in self.cameras) / ts) * math.ceil(max(c.height for c in self.cameras) / ts)
         tiles = max(max(tiles for _, _, tiles in self.chunks), plain_tiles)
         self.workspace = {
@@ -306,10 +340,13 @@ class BatchedRenderer:
         self.graph = None
 
     # ---------------------------------------------------------------------- submit
-    def _workspace_list(self) -> list[torch.Tensor]:
+    def _workspace_list(self, ordering: str | None = None) -> list[torch.Tensor]:
+        """Workspace tensors for one submission; the intersection buffers are viewed at the
+        ordering's capacity limit (the buffers are shared and may be larger)."""
         ws = self.workspace
-        return [ws["visible_records"], ws["keys_in"], ws["keys_out"], ws["values_in"], ws["values_out"],
-                ws["tile_starts"], ws["tile_ends"], ws["counters"], ws["scratch"]]
+        limit = MAX_INTERSECTIONS[ORDERINGS[ordering or self.ordering]]
+        return [ws["visible_records"], ws["keys_in"][:limit], ws["keys_out"][:limit], ws["values_in"][:limit],
+                ws["values_out"][:limit], ws["tile_starts"], ws["tile_ends"], ws["counters"], ws["scratch"]]
 
     def _submit(self, *, stage_events=None, statistics=None) -> None:
         packed = self._pack()
@@ -339,7 +376,8 @@ class BatchedRenderer:
                 sh_degree=self.sh_degree, persistent_composite=self.persistent_composite,
                 gaussian_ids=packed["gaussian_ids"] if self.spatial_order else None,
                 chunk_bounds=packed["chunk_bounds"] if self.chunk_culling else None,
-                resolutions=resolutions, **self.tuning, **extra)
+                resolutions=resolutions, ordering=ORDERINGS[self.ordering], max_camera_tiles=self.max_camera_tiles,
+                **self.tuning, **extra)
             self.chunk_counters[index].copy_(self.workspace["counters"])
             if chunk_statistics is not None:
                 statistics.add_(chunk_statistics)
@@ -388,20 +426,52 @@ class BatchedRenderer:
 
     def calibrate(self, viewmats: torch.Tensor, intrinsics: torch.Tensor, env_xforms: torch.Tensor | None = None,
                   max_retries: int = 4) -> dict[str, int]:
-        """Eager renders that size the capacities to the demand (x headroom)."""
+        """Eager renders that size the capacities to the demand (x headroom) and resolve the ordering.
+
+        The global radix ordering is calibrated first: its submission size is also the plain
+        path's.  With ordering 'front' (or 'auto' on an eligible plan) the tile-front ordering is
+        then calibrated on the same buffers; 'auto' keeps it unless more than ``front_open_limit``
+        of the non-empty tiles had a pixel open at the end of their sorted front (those tiles are
+        sorted in full a second time, which costs more than it saves)."""
         self.set_cameras(viewmats, intrinsics, env_xforms)
+        self.ordering = "global"
+        counters = self._calibrate_capacities(max_retries)
+        self.plain_per_submission = max(stop - start for start, stop, _ in self.chunks)
+        if self.ordering_request == "global" or not self.front_eligible():
+            return counters
+        global_chunks = self.chunk_count
+        self.ordering = "front"
+        self._layout_chunks(1)
+        # One submission now spans more tiles (and needs other scratch): size the buffers for it.
+        self._allocate(self.visible_capacity, self.intersection_capacity)
+        front_counters = self._calibrate_capacities(max_retries)
+        if self.ordering_request == "auto":
+            statistics = torch.zeros(4, dtype=torch.int64, device=self.device)
+            self._submit(statistics=statistics)
+            opened = int(statistics[3])
+            if opened > self.front_open_limit * max(1, front_counters.get("active_tiles", 0)):
+                self.ordering = "global"
+                self._layout_chunks(global_chunks)
+                self._allocate(self.visible_capacity, min(self.intersection_capacity, self.max_intersections))
+                return self._calibrate_capacities(max_retries)
+        return front_counters
+
+    def _calibrate_capacities(self, max_retries: int) -> dict[str, int]:
         counters: dict[str, int] = {}
         for _ in range(max_retries + 1):
             self._submit()
             counters = self.read_counters()
-            over_limit = (counters["max_visible_gaussians"] > self.max_visible
+            # The tile-front ordering stores no visible records (compositing recomputes them).
+            records = self.ordering == "global"
+            over_limit = ((records and counters["max_visible_gaussians"] > self.max_visible)
                           or counters["max_tile_intersections"] > self.max_intersections)
             overflow = over_limit or counters["visible_overflow"] or counters["intersection_overflow"]
             if not overflow and not counters["sort_lookback_failures"]:
                 return counters
             if not overflow:
                 continue  # a sort look-back wait gave up: render again with the same buffers
-            visible = math.ceil(counters["max_visible_gaussians"] * self.capacity_headroom)
+            visible = (math.ceil(counters["max_visible_gaussians"] * self.capacity_headroom) if records
+                       else self.visible_capacity)
             intersections = math.ceil(counters["max_tile_intersections"] * self.capacity_headroom)
             if over_limit or visible > self.max_visible or intersections > self.max_intersections:
                 # One submission cannot hold a chunk: split the cameras further (the
@@ -458,7 +528,7 @@ class BatchedRenderer:
         plain = packed["plain"]
         scene = [plain["means"], plain["covariances"], plain["opacities"], plain["features"],
                  plain["semantic_ids"], packed["registered_scene_ids"], packed["scene_offsets"]]
-        per_submission = max(stop - start for start, stop, _ in self.chunks)
+        per_submission = self.plain_per_submission or max(stop - start for start, stop, _ in self.chunks)
         for group in self.plain_groups:
             idx = group["indices"]
             o = group["outputs"]
@@ -471,7 +541,7 @@ class BatchedRenderer:
                 outputs = [o["rgb"][start:stop], o["depth"][start:stop], o["alpha"][start:stop],
                            o["semantic_id"][start:stop]]
                 self.native.render(
-                    scene, cameras, outputs, self._workspace_list(),
+                    scene, cameras, outputs, self._workspace_list("global"),
                     group["height"], group["width"], 0, self.options["near_plane"], self.options["far_plane"],
                     self.options["gaussian_support_sigma"], self.options["covariance_epsilon"],
                     self.options["antialiased"], self.options["semantic_min_alpha"], False, self.options["tile_size"],

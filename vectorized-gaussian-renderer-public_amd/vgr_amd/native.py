@@ -42,13 +42,21 @@ WORKSPACE_TENSORS = (
     "scratch",
 )
 RECORD_FLOATS = 12  # 48-byte visible record, see vgr_record in include/vgr.h
-RECORD_GRAD_FLOATS = 10  # dL/d (mean_x, mean_y, conic_0..2, opacity, depth, r, g, b) per visible record
+RECORD_GRAD_FLOATS = 10  # dL/d (mean_x, mean_y, conic_0..2, opacity, depth, r, g, b) per intersection slot
 CAMERA_GRAD_FLOATS = 24  # per-camera pose partials of the backward pass
-ABI_VERSION = 3
+ABI_VERSION = 4
 # Device counter block: the reference's five counters (cuda_backend.py:22-28) plus
 # one slot counting radix-sort look-back waits that gave up (the order is then wrong).
 COUNTER_SLOTS = 6
 SORT_FAILURE_SLOT = 5
+# Intersection orderings (vgr_ordering in include/vgr.h).
+ORDERING_GLOBAL_RADIX = 0
+ORDERING_TILE_FRONT = 1
+# Largest intersection capacity of one submission per ordering: the radix sort's
+# look-back flags carry 30-bit prefixes; tile-front buckets use int32 offsets.
+MAX_INTERSECTIONS = {ORDERING_GLOBAL_RADIX: (1 << 30) - 1, ORDERING_TILE_FRONT: (1 << 31) - 1}
+# Tile-front ordering limits: tiles per camera (16x16 tiles).
+FRONT_MAX_CAMERA_TILES = 2048
 
 
 class _Scene(ctypes.Structure):
@@ -84,6 +92,7 @@ class _Cameras(ctypes.Structure):
         ("heights", ctypes.c_void_p),
         ("pixel_offsets", ctypes.c_void_p),
         ("max_global_tiles", ctypes.c_int64),
+        ("max_camera_tiles", ctypes.c_int64),
     ]
 
 
@@ -130,6 +139,7 @@ class _Workspace(ctypes.Structure):
         ("intersection_capacity", ctypes.c_int64),
         ("tile_capacity", ctypes.c_int64),
         ("scratch_bytes", ctypes.c_int64),
+        ("slot_records", ctypes.c_void_p),
     ]
 
 
@@ -159,6 +169,7 @@ class _Options(ctypes.Structure):
         ("persistent_composite", ctypes.c_int32),
         ("chunk_culling", ctypes.c_int32),
         ("lookback_spin_limit", ctypes.c_int32),
+        ("ordering", ctypes.c_int32),
     ]
 
 
@@ -293,7 +304,8 @@ class NativeRenderer:
     def scratch_bytes(self, *, max_views: int, height: int, width: int, gaussians: int, intersection_capacity: int,
                       tile_size: int, near_plane: float, far_plane: float, sort_digit_bits: int = 0,
                       cameras_per_block: int = 0, sort_tile_items: int = 0, chunk_culling: bool = False,
-                      resolutions: Sequence[tuple[int, int]] | None = None) -> int:
+                      resolutions: Sequence[tuple[int, int]] | None = None,
+                      ordering: int = ORDERING_GLOBAL_RADIX) -> int:
         """Scratch bytes one render of up to ``max_views`` cameras needs."""
         cameras = _Cameras()
         cameras.batch = max_views
@@ -301,12 +313,15 @@ class NativeRenderer:
         if resolutions is not None:
             cameras.widths = 1  # marks per-camera resolutions; only the tile total is read
             cameras.heights = 1
-            cameras.max_global_tiles = sum(((w + tile_size - 1) // tile_size) * ((h + tile_size - 1) // tile_size)
-                                           for w, h in resolutions)
+            per_camera = [((w + tile_size - 1) // tile_size) * ((h + tile_size - 1) // tile_size)
+                          for w, h in resolutions]
+            cameras.max_global_tiles = sum(per_camera)
+            cameras.max_camera_tiles = max(per_camera)
         options = self._options(height=height, width=width, near_plane=near_plane, far_plane=far_plane,
                                 gaussian_support_sigma=3.0, covariance_epsilon=0.0, tile_size=tile_size,
                                 sort_digit_bits=sort_digit_bits, cameras_per_block=cameras_per_block,
-                                sort_tile_items=sort_tile_items, chunk_culling=int(bool(chunk_culling)))
+                                sort_tile_items=sort_tile_items, chunk_culling=int(bool(chunk_culling)),
+                                ordering=int(ordering))
         sizes = _Sizes()
         self._check(self.lib.vgr_workspace_sizes_query(ctypes.byref(cameras), ctypes.byref(options), int(gaussians),
                                                         int(intersection_capacity), ctypes.byref(sizes)),
@@ -393,6 +408,9 @@ class NativeRenderer:
         checkpoints: tuple[torch.Tensor, torch.Tensor] | None = None,
         lookback_spin_limit: int = 0,
         static_scratch: bool = False,
+        ordering: int = ORDERING_GLOBAL_RADIX,
+        max_camera_tiles: int = 0,
+        slot_records: torch.Tensor | None = None,
     ) -> int:
         """One render submission; returns the sorted-buffer selector (1: *_out holds the result).
 
@@ -405,6 +423,11 @@ class NativeRenderer:
         gives up is counted in ``counters[SORT_FAILURE_SLOT]``.  ``static_scratch`` calls
         ``vgr_render_static_scratch``: the library's own per-device scratch array replaces the
         ``scratch`` workspace tensor (which may then be empty), and no device memory is allocated.
+        ``ordering`` selects the intersection ordering (``ORDERING_TILE_FRONT``: per-tile buckets with
+        an LDS sort of each tile's nearest entries, see include/vgr.h); ``max_camera_tiles`` is the
+        largest tile count of one camera when ``resolutions`` are given (0: unknown).  Training
+        forwards (``checkpoints``) also take ``slot_records`` (int32, one per intersection slot):
+        the sort then carries slots, which :meth:`render_backward` needs.
         """
         if ray_gaussian_evaluation:
             raise ValueError("ray_gaussian_evaluation is not available on the gfx950 tile path.")
@@ -468,6 +491,7 @@ class NativeRenderer:
             c_cameras.heights = _ptr(heights)
             c_cameras.pixel_offsets = _ptr(pixel_offsets)
             c_cameras.max_global_tiles = int(total_tiles)
+            c_cameras.max_camera_tiles = int(max_camera_tiles)
         c_outputs = _Outputs(rgb=_ptr(rgb), depth=_ptr(depth) if full_sensor_output else None,
                              alpha=_ptr(alpha) if full_sensor_output else None,
                              semantic=_ptr(semantic) if full_sensor_output else None)
@@ -487,6 +511,12 @@ class NativeRenderer:
             tile_ends=_ptr(tile_ends), counters=_ptr(counters), scratch=_ptr(scratch),
             visible_capacity=int(records.shape[0]), intersection_capacity=int(keys_in.numel()),
             tile_capacity=int(tile_starts.numel()), scratch_bytes=int(scratch.numel()))
+        if slot_records is not None:
+            if (slot_records.dtype != torch.int32 or slot_records.device != device
+                    or slot_records.numel() < keys_in.numel() or not slot_records.is_contiguous()):
+                raise ValueError("slot_records must be a contiguous int32 tensor with one entry per intersection "
+                                 "slot on the render device")
+            c_workspace.slot_records = slot_records.data_ptr()
         c_options = self._options(
             height=int(height), width=int(width), near_plane=float(near_plane), far_plane=float(far_plane),
             gaussian_support_sigma=float(gaussian_support_sigma), covariance_epsilon=float(covariance_epsilon),
@@ -496,7 +526,8 @@ class NativeRenderer:
             sort_digit_bits=int(sort_digit_bits), raster_batch=int(raster_batch),
             cameras_per_block=int(cameras_per_block), reuse_projection=int(bool(reuse_projection)),
             sort_tile_items=int(sort_tile_items), persistent_composite=int(persistent_composite),
-            chunk_culling=int(chunk_bounds is not None), lookback_spin_limit=int(lookback_spin_limit))
+            chunk_culling=int(chunk_bounds is not None), lookback_spin_limit=int(lookback_spin_limit),
+            ordering=int(ordering))
         event_array = None
         if stage_events is not None:
             if len(stage_events) != 5:
@@ -541,8 +572,10 @@ class NativeRenderer:
                 continue
             if tensor.dtype != torch.float32 or tensor.device != device or not tensor.is_contiguous():
                 raise ValueError(f"{name} must be a contiguous float32 tensor on {device}")
-        if record_grads.numel() < call.workspace.visible_capacity * RECORD_GRAD_FLOATS:
-            raise ValueError("record_grads must hold 10 floats per visible record")
+        if call.workspace.slot_records is None:
+            raise ValueError("render_backward needs a forward rendered with slot_records")
+        if record_grads.numel() < call.workspace.intersection_capacity * RECORD_GRAD_FLOATS:
+            raise ValueError("record_grads must hold 10 floats per intersection slot")
         gradients = _Gradients(grad_rgb=_ptr(grad_rgb), grad_depth=_ptr(grad_depth), grad_alpha=_ptr(grad_alpha),
                                record_grads=_ptr(record_grads), sorted_selector=call.selector)
         if scene_grads is not None:
