@@ -163,6 +163,10 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   offset = align_up(offset + sizeof(int32_t) * front_tiles);
   layout.front_block_masks = offset;
   offset = align_up(offset + sizeof(uint64_t) * (layout.front ? blocks + 1 : 0u));
+  layout.front_wave_masks = offset;
+  offset = align_up(offset + sizeof(uint64_t) * (layout.front ? (blocks + 1) * (kBlockThreads / kWave) : 0u));
+  layout.front_block_visible = offset;
+  offset = align_up(offset + sizeof(int64_t) * (layout.front ? blocks + 1 : 0u));
   layout.front_open_state = offset;
   offset = align_up(offset + sizeof(uint32_t) * 4);
   layout.front_open_tiles = offset;

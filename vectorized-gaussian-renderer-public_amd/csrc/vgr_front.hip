@@ -36,11 +36,12 @@ namespace {
 
 using namespace dev;
 
-constexpr int kFrontThreads = 256;
+constexpr int kFrontThreads = 512;
 constexpr int kFrontWaves = kFrontThreads / kWave;
-constexpr int kFrontItems = 16;
+constexpr int kFrontItems = 8;
 constexpr int kFrontCapacity = kFrontThreads * kFrontItems;  // entries sorted in LDS per tile (4096)
 constexpr int kFrontWaveRun = kFrontCapacity / kFrontWaves;  // entries ranked by one wave per pass
+constexpr int kFrontLoads = 4;  // global keys in flight per thread in the bucket scans
 constexpr int kFrontDigitBits = 9;
 constexpr int kFrontRadix = 1 << kFrontDigitBits;
 constexpr uint32_t kFrontInvalidKey = 0xffffffffu;
@@ -67,9 +68,12 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, int bits, uint64
 __global__ __launch_bounds__(kBlockThreads) void front_count_kernel(ProjectionArgs args, int group_tile_limit,
                                                                     unsigned long long* __restrict__ tile_counts,
                                                                     uint64_t* __restrict__ block_masks,
+                                                                    uint64_t* __restrict__ wave_masks,
+                                                                    int64_t* __restrict__ block_visible,
                                                                     int64_t* __restrict__ counters) {
   extern __shared__ uint32_t s_hist[];  // [group_tile_limit]
   __shared__ unsigned long long s_mask;
+  __shared__ int64_t s_visible[kBlockThreads / kWave];
   int64_t block;
   uint64_t mask;
   if (!resolve_block(args, block, mask)) return;
@@ -83,10 +87,13 @@ __global__ __launch_bounds__(kBlockThreads) void front_count_kernel(ProjectionAr
   const int64_t group_base = first_camera.tile_base;
   const int64_t group_tiles = static_cast<int64_t>(last_camera.tile_base) +
                               static_cast<int64_t>(last_camera.tiles_x) * last_camera.tiles_y - group_base;
+  const int64_t wave_slot = static_cast<int64_t>(blockIdx.x) * (kBlockThreads / kWave) + threadIdx.x / kWave;
   if (group_tiles <= 0 || group_tiles > group_tile_limit) {
     // Layout outside what the host sized the histogram for: fail closed.
+    if ((threadIdx.x & (kWave - 1)) == 0) wave_masks[wave_slot] = 0;
     if (threadIdx.x == 0) {
       block_masks[blockIdx.x] = 0;
+      block_visible[blockIdx.x] = 0;
       atomicAdd(reinterpret_cast<unsigned long long*>(counters + VGR_COUNTER_SORT_FAILURES), 1ull);
     }
     return;
@@ -115,20 +122,54 @@ __global__ __launch_bounds__(kBlockThreads) void front_count_kernel(ProjectionAr
     }
     if (__ballot(seen) != 0ull) wave_mask |= 1ull << (a - first);
   }
+  // Per-workgroup partial counts (summed by front_totals_kernel): atomics of every
+  // wave on one counter word would serialize behind each other.
   visible = wave_sum_i64(visible);
   if ((threadIdx.x & (kWave - 1)) == 0) {
     if (wave_mask != 0ull) atomicOr(&s_mask, static_cast<unsigned long long>(wave_mask));
-    if (visible != 0) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(counters + VGR_COUNTER_VISIBLE),
-                static_cast<unsigned long long>(visible));
-    }
+    s_visible[threadIdx.x / kWave] = visible;
+    wave_masks[wave_slot] = wave_mask;
   }
   __syncthreads();
   for (int64_t i = threadIdx.x; i < group_tiles; i += kBlockThreads) {
     const uint32_t c = s_hist[i];
     if (c != 0u) atomicAdd(&tile_counts[2 * (group_base + i)], static_cast<unsigned long long>(c));
   }
-  if (threadIdx.x == 0) block_masks[blockIdx.x] = s_mask;
+  if (threadIdx.x == 0) {
+    block_masks[blockIdx.x] = s_mask;
+    int64_t total = 0;
+    for (int w = 0; w < kBlockThreads / kWave; ++w) total += s_visible[w];
+    block_visible[blockIdx.x] = total;
+  }
+}
+
+// One workgroup: the visible-pair counter (sum of the per-workgroup counts of the
+// kept workgroups) and the active-tile counter (non-empty buckets).
+__global__ __launch_bounds__(1024) void front_totals_kernel(const int64_t* __restrict__ block_visible,
+                                                            int64_t blocks, const int64_t* __restrict__ kept,
+                                                            const int32_t* __restrict__ lengths, int64_t tiles,
+                                                            int64_t* __restrict__ counters) {
+  __shared__ int64_t s_sum[2][1024 / kWave];
+  const int64_t n = kept != nullptr ? (*kept < blocks ? *kept : blocks) : blocks;
+  int64_t visible = 0, active = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) visible += block_visible[i];
+  for (int64_t t = threadIdx.x; t < tiles; t += 1024) active += lengths[t] > 0 ? 1 : 0;
+  visible = wave_sum_i64(visible);
+  active = wave_sum_i64(active);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_sum[0][threadIdx.x / kWave] = visible;
+    s_sum[1][threadIdx.x / kWave] = active;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t total_visible = 0, total_active = 0;
+    for (int w = 0; w < 1024 / kWave; ++w) {
+      total_visible += s_sum[0][w];
+      total_active += s_sum[1][w];
+    }
+    counters[VGR_COUNTER_VISIBLE] = total_visible;
+    counters[VGR_COUNTER_ACTIVE_TILES] = total_active;
+  }
 }
 
 // ---- tile starts ---------------------------------------------------------------
@@ -166,6 +207,7 @@ __global__ __launch_bounds__(kBlockThreads) void front_starts_kernel(const int64
 
 __global__ __launch_bounds__(kBlockThreads) void front_write_kernel(ProjectionArgs args, int camera_tile_limit,
                                                                     const uint64_t* __restrict__ block_masks,
+                                                                    const uint64_t* __restrict__ wave_masks,
                                                                     unsigned long long* __restrict__ cursors,
                                                                     uint32_t* __restrict__ keys,
                                                                     int32_t* __restrict__ values, int64_t capacity,
@@ -183,6 +225,10 @@ __global__ __launch_bounds__(kBlockThreads) void front_write_kernel(ProjectionAr
   const int64_t g = chunk * kBlockThreads + threadIdx.x;
   const int first = group * args.cameras_per_block;
   uint64_t cameras = block_masks[blockIdx.x] & mask;
+  // Cameras that see none of this wave's Gaussians (per the count pass) are not
+  // evaluated by the wave; it still joins the workgroup's barriers.
+  const uint64_t wave_cameras = wave_masks[static_cast<int64_t>(blockIdx.x) * (kBlockThreads / kWave) +
+                                           threadIdx.x / kWave];
   for (int i = threadIdx.x; i < camera_tile_limit; i += kBlockThreads) s_count[i] = 0u;
   __syncthreads();
   bool loaded;
@@ -201,8 +247,9 @@ __global__ __launch_bounds__(kBlockThreads) void front_write_kernel(ProjectionAr
     }
     Projected p;
     TileRect rect;
-    int tiles;
-    const bool seen = evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    int tiles = 0;
+    const bool seen = ((wave_cameras >> (a - first)) & 1ull) != 0ull &&
+                      evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
     const bool ranked = seen && tiles <= kRanks;
     uint32_t rank[kRanks] = {0u, 0u, 0u, 0u};
     if (ranked) {
@@ -275,14 +322,23 @@ __device__ __forceinline__ int32_t front_gaussian_id(const int32_t* gaussian_ids
   return gaussian_ids != nullptr ? gaussian_ids[value] : value;
 }
 
-// Exclusive scan of the 512 histogram bins (two per thread); returns nothing,
-// leaves the starts in s.digit_start.  Ends with a barrier.
+// Exclusive scan of the kFrontRadix histogram bins (kFrontBinsPerThread
+// consecutive bins per thread) into s.digit_start.  Ends with a barrier.
+constexpr int kFrontBinsPerThread = kFrontRadix / kFrontThreads;
+static_assert(kFrontBinsPerThread >= 1 && kFrontBinsPerThread * kFrontThreads == kFrontRadix,
+              "the bin scan gives every thread the same number of bins");
+
 __device__ __forceinline__ void front_scan_bins(FrontShared& s) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  const uint32_t a = s.histogram[2 * threadIdx.x];
-  const uint32_t b = s.histogram[2 * threadIdx.x + 1];
-  uint32_t inclusive = a + b;
+  uint32_t bins[kFrontBinsPerThread];
+  uint32_t own = 0u;
+#pragma unroll
+  for (int b = 0; b < kFrontBinsPerThread; ++b) {
+    bins[b] = s.histogram[kFrontBinsPerThread * threadIdx.x + b];
+    own += bins[b];
+  }
+  uint32_t inclusive = own;
 #pragma unroll
   for (int offset = 1; offset < kWave; offset <<= 1) {
     const uint32_t t = __shfl_up(inclusive, offset, kWave);
@@ -290,11 +346,13 @@ __device__ __forceinline__ void front_scan_bins(FrontShared& s) {
   }
   if (lane == kWave - 1) s.wave_totals[wave] = inclusive;
   __syncthreads();
-  uint32_t prefix = 0;
-  for (int w = 0; w < wave; ++w) prefix += s.wave_totals[w];
-  const uint32_t exclusive = prefix + inclusive - (a + b);
-  s.digit_start[2 * threadIdx.x] = exclusive;
-  s.digit_start[2 * threadIdx.x + 1] = exclusive + a;
+  uint32_t running = inclusive - own;
+  for (int w = 0; w < wave; ++w) running += s.wave_totals[w];
+#pragma unroll
+  for (int b = 0; b < kFrontBinsPerThread; ++b) {
+    s.digit_start[kFrontBinsPerThread * threadIdx.x + b] = running;
+    running += bins[b];
+  }
   __syncthreads();
 }
 
@@ -369,15 +427,25 @@ __device__ __forceinline__ void front_histogram(FrontShared& s, const uint32_t* 
   __syncthreads();
   const uint32_t digit_mask = (1u << bits) - 1u;
   const uint64_t lt = front_lanemask_lt();
-  for (int64_t base = 0; base < count; base += kFrontThreads) {
-    const int64_t i = base + threadIdx.x;
-    const uint32_t key = i < count ? keys[start + i] : kFrontInvalidKey;
-    const bool in_bucket = i < count && (prefix_shift >= 32 || (key >> prefix_shift) == prefix);
-    const uint32_t digit = in_bucket ? (key >> shift) & digit_mask : 0u;
-    const uint64_t live = __ballot(in_bucket);
-    if (live == 0ull) continue;
-    const uint64_t peers = match_digit(digit, bits, live);
-    if (in_bucket && (peers & lt) == 0ull) atomicAdd(&s.histogram[digit], static_cast<uint32_t>(__popcll(peers)));
+  // kFrontLoads keys per thread per step, all loads issued before any is used.
+  for (int64_t base = 0; base < count; base += kFrontThreads * kFrontLoads) {
+    uint32_t loaded[kFrontLoads];
+#pragma unroll
+    for (int r = 0; r < kFrontLoads; ++r) {
+      const int64_t i = base + r * kFrontThreads + threadIdx.x;
+      loaded[r] = i < count ? keys[start + i] : kFrontInvalidKey;
+    }
+#pragma unroll
+    for (int r = 0; r < kFrontLoads; ++r) {
+      const int64_t i = base + r * kFrontThreads + threadIdx.x;
+      const uint32_t key = loaded[r];
+      const bool in_bucket = i < count && (prefix_shift >= 32 || (key >> prefix_shift) == prefix);
+      const uint32_t digit = in_bucket ? (key >> shift) & digit_mask : 0u;
+      const uint64_t live = __ballot(in_bucket);
+      if (live == 0ull) continue;
+      const uint64_t peers = match_digit(digit, bits, live);
+      if (in_bucket && (peers & lt) == 0ull) atomicAdd(&s.histogram[digit], static_cast<uint32_t>(__popcll(peers)));
+    }
   }
   __syncthreads();
 }
@@ -392,10 +460,7 @@ __global__ __launch_bounds__(kFrontThreads) void front_sort_kernel(
   const int64_t start = tile_starts[t];
   const int64_t count = lengths[t];
   if (count <= 0) return;
-  if (threadIdx.x == 0) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(counters + VGR_COUNTER_ACTIVE_TILES), 1ull);
-    s.fill = 0u;
-  }
+  if (threadIdx.x == 0) s.fill = 0u;
   __syncthreads();
   // Threshold: every entry with key < threshold is taken (at most kFrontCapacity of them).
   uint32_t threshold = kFrontInvalidKey;
@@ -439,20 +504,29 @@ __global__ __launch_bounds__(kFrontThreads) void front_sort_kernel(
   }
   // Gather the selected entries into LDS (order irrelevant: sorted next).
   const uint64_t lt = front_lanemask_lt();
-  for (int64_t base = 0; base < count; base += kFrontThreads) {
-    const int64_t i = base + threadIdx.x;
-    const uint32_t key = i < count ? keys[start + i] : kFrontInvalidKey;
-    const bool take = i < count && (all || key < threshold);
-    const uint64_t vote = __ballot(take);
-    if (vote == 0ull) continue;
-    uint32_t wave_base = 0u;
-    if ((threadIdx.x & (kWave - 1)) == 0) wave_base = atomicAdd(&s.fill, static_cast<uint32_t>(__popcll(vote)));
-    wave_base = __shfl(wave_base, 0, kWave);
-    if (take) {
-      const uint32_t position = wave_base + static_cast<uint32_t>(__popcll(vote & lt));
-      if (position < static_cast<uint32_t>(kFrontCapacity)) {
-        s.keys[position] = key;
-        s.values[position] = values[start + i];
+  for (int64_t base = 0; base < count; base += kFrontThreads * kFrontLoads) {
+    uint32_t loaded[kFrontLoads];
+#pragma unroll
+    for (int r = 0; r < kFrontLoads; ++r) {
+      const int64_t i = base + r * kFrontThreads + threadIdx.x;
+      loaded[r] = i < count ? keys[start + i] : kFrontInvalidKey;
+    }
+#pragma unroll
+    for (int r = 0; r < kFrontLoads; ++r) {
+      const int64_t i = base + r * kFrontThreads + threadIdx.x;
+      const uint32_t key = loaded[r];
+      const bool take = i < count && (all || key < threshold);
+      const uint64_t vote = __ballot(take);
+      if (vote == 0ull) continue;
+      uint32_t wave_base = 0u;
+      if ((threadIdx.x & (kWave - 1)) == 0) wave_base = atomicAdd(&s.fill, static_cast<uint32_t>(__popcll(vote)));
+      wave_base = __shfl(wave_base, 0, kWave);
+      if (take) {
+        const uint32_t position = wave_base + static_cast<uint32_t>(__popcll(vote & lt));
+        if (position < static_cast<uint32_t>(kFrontCapacity)) {
+          s.keys[position] = key;
+          s.values[position] = values[start + i];
+        }
       }
     }
   }
@@ -619,6 +693,8 @@ int launch_front_projection(const vgr_scene& scene, const vgr_cameras& cameras, 
   unsigned long long* cursors = reinterpret_cast<unsigned long long*>(scratch + layout.front_cursors);
   int32_t* lengths = reinterpret_cast<int32_t*>(scratch + layout.front_lengths);
   uint64_t* block_masks = reinterpret_cast<uint64_t*>(scratch + layout.front_block_masks);
+  uint64_t* wave_masks = reinterpret_cast<uint64_t*>(scratch + layout.front_wave_masks);
+  int64_t* block_visible = reinterpret_cast<int64_t*>(scratch + layout.front_block_visible);
 
   launch_fill_i64(ws.counters, VGR_COUNTER_COUNT, 0, stream);
   launch_fill_i64(reinterpret_cast<int64_t*>(tile_counts), 2 * global_tiles, 0, stream);
@@ -636,7 +712,7 @@ int launch_front_projection(const vgr_scene& scene, const vgr_cameras& cameras, 
     const size_t count_lds = sizeof(uint32_t) * static_cast<size_t>(layout.front_group_tiles);
     hipLaunchKernelGGL(front_count_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), count_lds,
                        stream, args, static_cast<int>(layout.front_group_tiles), tile_counts, block_masks,
-                       ws.counters);
+                       wave_masks, block_visible, ws.counters);
   }
   scan_pairs(reinterpret_cast<const int64_t*>(tile_counts), global_tiles, tile_bases, tile_bases + 2 * global_tiles,
              0, 0, nullptr, totals, stream);
@@ -646,10 +722,12 @@ int launch_front_projection(const vgr_scene& scene, const vgr_cameras& cameras, 
                        global_tiles, totals, ws.intersection_capacity, ws.tile_starts, ws.tile_ends, lengths, cursors,
                        ws.counters);
   }
+  hipLaunchKernelGGL(front_totals_kernel, dim3(1), dim3(1024), 0, stream, block_visible, blocks,
+                     culled ? args.item_count : nullptr, lengths, global_tiles, ws.counters);
   if (blocks > 0) {
     const size_t write_lds = 2 * sizeof(uint32_t) * static_cast<size_t>(layout.front_camera_tiles);
     hipLaunchKernelGGL(front_write_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), write_lds,
-                       stream, args, static_cast<int>(layout.front_camera_tiles), block_masks, cursors,
+                       stream, args, static_cast<int>(layout.front_camera_tiles), block_masks, wave_masks, cursors,
                        reinterpret_cast<uint32_t*>(ws.keys_in), ws.values_in, ws.intersection_capacity,
                        layout.depth_base, ws.counters);
   }

@@ -64,6 +64,8 @@ struct ScratchLayout {
   size_t front_cursors;      // uint64[global_tiles]: next free slot of each bucket
   size_t front_lengths;      // int32[global_tiles]: bucket sizes (clamped to the capacity)
   size_t front_block_masks;  // uint64[projection_blocks]: group cameras seen by each workgroup
+  size_t front_wave_masks;   // uint64[projection_blocks * waves]: group cameras seen by each wave
+  size_t front_block_visible;  // int64[projection_blocks]: visible pairs of each workgroup
   size_t front_open_state;   // uint32[4]: open-tile count, fallback-sort and recomposite queues
   size_t front_open_tiles;   // int32[global_tiles]: tiles to sort in full and composite again
   size_t total;
