@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define VGR_ABI_VERSION 4
+#define VGR_ABI_VERSION 5
 
 enum vgr_status {
   VGR_OK = 0,
@@ -288,7 +288,11 @@ int32_t vgr_render_static_scratch(const vgr_scene* scene, const vgr_cameras* cam
  * gradients are indexed like the packed scene arrays and are ADDED to (the
  * caller zero-fills them); grad_viewmats [B,4,4] may be NULL (no pose
  * gradient).  record_grads is workspace of 10 floats per intersection slot
- * (dL/d mean_x, mean_y, conic_0..2, opacity, depth, r, g, b). */
+ * (dL/d mean_x, mean_y, conic_0..2, opacity, depth, r, g, b).  camera_grads
+ * holds VGR_POSE_REPLICAS copies of 24 pose partials per active camera: the
+ * projection workgroups add into copy (workgroup % VGR_POSE_REPLICAS), so
+ * that the adds of one camera spread over that many addresses. */
+#define VGR_POSE_REPLICAS 32
 typedef struct vgr_gradients {
   const float* grad_rgb;            /* [.,3] dL/d rgb */
   const float* grad_depth;          /* [.]   dL/d depth (ignored where depth is +inf) */
@@ -300,7 +304,7 @@ typedef struct vgr_gradients {
   float* grad_opacities;            /* [G] */
   float* grad_features;             /* [G,feature_width]; the first 3*(d+1)^2 columns receive gradients */
   float* grad_viewmats;             /* [B,16] or NULL */
-  float* camera_grads;              /* [active_count, 24] workspace when grad_viewmats != NULL */
+  float* camera_grads;              /* [VGR_POSE_REPLICAS * active_count, 24] workspace when grad_viewmats != NULL */
   int64_t sorted_selector;          /* the forward's *selector_out */
 } vgr_gradients;
 
@@ -325,6 +329,17 @@ int32_t vgr_render_backward(const vgr_scene* scene, const vgr_cameras* cameras,
  * normalization is differentiated).  Outputs are overwritten. */
 int32_t vgr_covariance_backward(const float* scales, const float* rotations, const float* grad_covariances,
                                 int64_t count, float* grad_scales, float* grad_rotations, void* stream);
+
+/* Training-time camera poses: views[b] = exp(xi_b^) @ base_views[b] for se(3)
+ * twists xi = (rho, phi) [B,6] and row-major 4x4 base views [B,16] (Rodrigues
+ * rotation, left-Jacobian translation), and its backward: dL/d twists [B,6]
+ * and, when grad_base_views is not NULL, dL/d base views [B,16] from
+ * dL/d views.  Computed per camera in double.  No reference counterpart (the
+ * reference is forward-only); it is the pose parameterisation of the
+ * differentiable path. */
+int32_t vgr_pose_compose(const float* twists, const float* base_views, int64_t count, float* views, void* stream);
+int32_t vgr_pose_compose_backward(const float* twists, const float* base_views, const float* grad_views,
+                                  int64_t count, float* grad_twists, float* grad_base_views, void* stream);
 
 /* Logical-batch chunking helpers (renderer_cuda.cu:2307-2360). */
 int32_t vgr_prepare_chunked_active_ids(const int64_t* active_ids, int64_t active_count,

@@ -17,7 +17,7 @@ import torch
 
 from oracle import dense_reference as dense
 from vgr_amd import synthetic
-from vgr_amd.autograd import DifferentiableRenderer, se3_exp
+from vgr_amd.autograd import DifferentiableRenderer, compose_pose, se3_exp
 
 pytestmark = pytest.mark.gpu
 
@@ -65,7 +65,7 @@ def hip_gradients(params32, base_views, intrinsics, height, width, sh_degree, we
     renderer = DifferentiableRenderer(height=height, width=width, device=device(), **options)
     leaves = [p.to(device()).clone().requires_grad_(True) for p in params32]
     means, log_scales, quats, logits, features, xi = leaves
-    views = (se3_exp(xi) @ base_views.to(device())).contiguous()
+    views = compose_pose(xi, base_views.to(device()).float().contiguous())
     out = renderer.render(means, log_scales, quats, logits, features, views, intrinsics.to(device()),
                           sh_degree=sh_degree)
     w = [t.to(device()).float() for t in weights]
@@ -214,3 +214,31 @@ def test_deferred_capacity_check_fails_loud_and_recovers():
     recovered = step()  # buffers were grown: synchronous again, then correct
     for a, b in zip(recovered, reference):
         torch.testing.assert_close(a, b, rtol=1e-5, atol=1e-6)
+
+
+def test_compose_pose_matches_fp64_se3_exp_autograd():
+    """vgr_pose_compose(+_backward) against torch autograd of se3_exp in float64:
+    zero, tiny, series-boundary, moderate and large rotation angles."""
+    g = torch.Generator().manual_seed(SEED + 7)
+    angles = torch.tensor([0.0, 1e-6, 1e-3, 0.049, 0.051, 0.3, 1.0, 2.5, 3.1], dtype=torch.float64)
+    count = angles.numel()
+    axes = torch.nn.functional.normalize(torch.randn((count, 3), generator=g, dtype=torch.float64), dim=-1)
+    xi = torch.cat((torch.randn((count, 3), generator=g, dtype=torch.float64), axes * angles[:, None]), dim=-1)
+    rig = synthetic.orbit_rig(count, 64, 64, 55.0, seed=SEED, radius=(9.0, 11.0))
+    base = rig.viewmats.double()
+    upstream = torch.randn((count, 4, 4), generator=g, dtype=torch.float64)
+    xi64 = xi.clone().requires_grad_(True)
+    base64 = base.clone().requires_grad_(True)
+    expected = se3_exp(xi64) @ base64
+    (expected * upstream).sum().backward()
+    xi32 = xi.float().to(device()).requires_grad_(True)
+    base32 = base.float().to(device()).requires_grad_(True)
+    views = compose_pose(xi32, base32)
+    (views * upstream.float().to(device())).sum().backward()
+    torch.cuda.synchronize()
+    # float32 inputs and outputs around double arithmetic: errors of a few float32 ulps.
+    assert torch.allclose(views.double().cpu(), expected.detach(), rtol=0, atol=1e-5 * float(base.abs().max()))
+    scale_xi = float(xi64.grad.abs().max())
+    assert float((xi32.grad.double().cpu() - xi64.grad).abs().max()) <= 2e-5 * scale_xi
+    scale_base = float(base64.grad.abs().max())
+    assert float((base32.grad.double().cpu() - base64.grad).abs().max()) <= 2e-5 * scale_base

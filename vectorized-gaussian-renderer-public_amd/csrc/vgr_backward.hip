@@ -1070,7 +1070,12 @@ __global__ __launch_bounds__(kBlockThreads) void projection_backward_kernel(Proj
         float sum = 0.0f;
 #pragma unroll
         for (int w = 0; w < kBlockThreads / kWave; ++w) sum += s_pose[w][threadIdx.x];
-        if (sum != 0.0f) atomic_add_f32(args.camera_grads + static_cast<int64_t>(a) * kCameraGrads + threadIdx.x, sum);
+        // One of VGR_POSE_REPLICAS copies of the camera's row, picked by workgroup:
+        // the workgroups of a camera spread their adds over that many addresses.
+        const int64_t replica = blockIdx.x % VGR_POSE_REPLICAS;
+        if (sum != 0.0f) {
+          atomic_add_f32(args.camera_grads + (replica * fw.active_count + a) * kCameraGrads + threadIdx.x, sum);
+        }
       }
     }
     visible_base += total_visible;
@@ -1101,7 +1106,13 @@ __global__ void camera_grads_finalize_kernel(const CameraParams* __restrict__ ta
   if (a >= active_count) return;
   const CameraParams& cam = table[a];
   if (cam.camera < 0) return;
-  const float* acc = camera_grads + static_cast<int64_t>(a) * kCameraGrads;
+  // Sum of the replicas of the camera's row (fixed order).
+  double acc[kCameraGrads];
+  for (int k = 0; k < kCameraGrads; ++k) acc[k] = 0.0;
+  for (int r = 0; r < VGR_POSE_REPLICAS; ++r) {
+    const float* row = camera_grads + (static_cast<int64_t>(r) * active_count + a) * kCameraGrads;
+    for (int k = 0; k < kCameraGrads; ++k) acc[k] += row[k];
+  }
   const float* e = cam.env;
   const float* v = cam.view;
   double m[9];
@@ -1253,8 +1264,10 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
   if (cameras.active_count <= 0) return 0;
   const CameraParams* table = reinterpret_cast<const CameraParams*>(scratch + layout.camera_table);
   const bool pose = grads.grad_viewmats != nullptr;
-  if (pose) launch_fill_f32(grads.camera_grads, static_cast<int64_t>(cameras.active_count) * kCameraGrads, 0.0f,
-                            stream);
+  if (pose) {
+    launch_fill_f32(grads.camera_grads,
+                    static_cast<int64_t>(VGR_POSE_REPLICAS) * cameras.active_count * kCameraGrads, 0.0f, stream);
+  }
 
   CompositeBackwardArgs cargs;
   cargs.table = table;

@@ -631,6 +631,27 @@ extern "C" int32_t vgr_covariance_backward(const float* scales, const float* rot
                                     static_cast<hipStream_t>(stream_handle));
 }
 
+extern "C" int32_t vgr_pose_compose(const float* twists, const float* base_views, int64_t count, float* views,
+                                    void* stream_handle) {
+  if (count < 0 || (count > 0 && (twists == nullptr || base_views == nullptr || views == nullptr))) {
+    set_error("vgr_pose_compose: null tensor or negative count");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  return launch_pose_compose(twists, base_views, count, views, static_cast<hipStream_t>(stream_handle));
+}
+
+extern "C" int32_t vgr_pose_compose_backward(const float* twists, const float* base_views, const float* grad_views,
+                                             int64_t count, float* grad_twists, float* grad_base_views,
+                                             void* stream_handle) {
+  if (count < 0 || (count > 0 && (twists == nullptr || base_views == nullptr || grad_views == nullptr ||
+                                  grad_twists == nullptr))) {
+    set_error("vgr_pose_compose_backward: null tensor or negative count");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  return launch_pose_compose_backward(twists, base_views, grad_views, count, grad_twists, grad_base_views,
+                                      static_cast<hipStream_t>(stream_handle));
+}
+
 namespace vgr {
 namespace {
 

@@ -132,6 +132,9 @@ int launch_front_fallback_sort(const vgr_scene& scene, const vgr_workspace& ws, 
 void scan_pairs(const int64_t* counts, int64_t n, int64_t* bases, int64_t* partials, int64_t visible_capacity,
                 int64_t intersection_capacity, int64_t* counters, int64_t* totals, hipStream_t stream);
 int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream);
+int launch_pose_compose(const float* twists, const float* base, int64_t count, float* views, hipStream_t stream);
+int launch_pose_compose_backward(const float* twists, const float* base, const float* grad_views, int64_t count,
+                                 float* grad_twists, float* grad_base, hipStream_t stream);
 int launch_copy_u32_to_i64(const uint32_t* source, int64_t* target, hipStream_t stream);
 int launch_fill_u32(uint32_t* data, int64_t n, uint32_t value, hipStream_t stream);
 int launch_fill_f32(float* data, int64_t n, float value, hipStream_t stream);

@@ -12,8 +12,10 @@ this module adds the training path on top of the same forward:
 Parameters follow the 3DGS convention: means, log-scales, unnormalized WXYZ
 quaternions, opacity logits and SH features; ``exp`` and ``sigmoid`` are
 applied here in torch so autograd chains them.  Camera poses are the
-world->camera ``viewmats`` [B,4,4]; :func:`se3_exp` maps a 6-DoF twist to a
-pose for pose optimization (``viewmats = se3_exp(xi) @ base``).
+world->camera ``viewmats`` [B,4,4]; :func:`compose_pose` maps 6-DoF twists
+onto base poses for pose optimization (``viewmats = exp(xi^) @ base`` in one
+kernel each way, ``vgr_pose_compose``); :func:`se3_exp` is the same map in
+torch operations.
 
 Every forward reads the device counters once to verify that the visible and
 intersection buffers held the whole batch (and re-renders with grown
@@ -28,8 +30,8 @@ from dataclasses import dataclass
 import torch
 
 from . import native as native_module
-from .native import (CAMERA_GRAD_FLOATS, COUNTER_SLOTS, RECORD_FLOATS, RECORD_GRAD_FLOATS, SORT_FAILURE_SLOT,
-                     infer_sh_degree)
+from .native import (CAMERA_GRAD_FLOATS, COUNTER_SLOTS, POSE_REPLICAS, RECORD_FLOATS, RECORD_GRAD_FLOATS,
+                     SORT_FAILURE_SLOT, infer_sh_degree)
 
 _GROWTH = 1.25
 
@@ -58,6 +60,43 @@ def se3_exp(xi: torch.Tensor) -> torch.Tensor:
     bottom = torch.zeros((xi.shape[0], 1, 4), dtype=xi.dtype, device=xi.device)
     bottom[:, 0, 3] = 1.0
     return torch.cat((top, bottom), dim=1)
+
+
+class _PoseCompose(torch.autograd.Function):
+    """views = exp(xi^) @ base on the device (forward and analytic backward per camera)."""
+
+    @staticmethod
+    def forward(ctx, twists: torch.Tensor, base_views: torch.Tensor, native) -> torch.Tensor:
+        twists_c = twists.detach().float().contiguous()
+        base_c = base_views.detach().float().contiguous()
+        views = torch.empty_like(base_c)
+        native.pose_compose(twists_c, base_c, views)
+        ctx.save_for_backward(twists_c, base_c)
+        ctx.native = native
+        return views
+
+    @staticmethod
+    def backward(ctx, grad_views: torch.Tensor):
+        twists, base = ctx.saved_tensors
+        grad_twists = torch.empty_like(twists)
+        grad_base = torch.empty_like(base) if ctx.needs_input_grad[1] else None
+        ctx.native.pose_compose_backward(twists, base, grad_views.float().contiguous(), grad_twists, grad_base)
+        return grad_twists, grad_base, None
+
+
+def compose_pose(xi: torch.Tensor, base_views: torch.Tensor,
+                 native: native_module.NativeRenderer | None = None) -> torch.Tensor:
+    """[B,6] twists (rho, phi) and [B,4,4] base poses -> exp(xi^) @ base [B,4,4], differentiable
+    in both; the device form of ``se3_exp(xi) @ base_views``."""
+    if xi.ndim != 2 or xi.shape[1] != 6:
+        raise ValueError("compose_pose expects [B,6] twists")
+    if base_views.shape != (xi.shape[0], 4, 4):
+        raise ValueError("compose_pose expects [B,4,4] base views matching the twists")
+    if xi.dtype != torch.float32 or base_views.dtype != torch.float32:
+        raise ValueError("compose_pose expects float32 twists and base views")
+    if not xi.is_cuda or base_views.device != xi.device:
+        raise ValueError("compose_pose runs on the GPU: twists and base views on one device")
+    return _PoseCompose.apply(xi, base_views, native if native is not None else native_module.load_native_module())
 
 
 @dataclass
@@ -373,7 +412,8 @@ class _RasterizeFunction(torch.autograd.Function):
         batch = int(state["cameras"][0].shape[0])
         if ctx.pose:
             grad_viewmats = torch.zeros((batch, 4, 4), dtype=torch.float32, device=device)
-            camera_grads = torch.empty((batch, CAMERA_GRAD_FLOATS), dtype=torch.float32, device=device)
+            camera_grads = torch.empty((POSE_REPLICAS * batch, CAMERA_GRAD_FLOATS), dtype=torch.float32,
+                                       device=device)
         renderer.native.render_backward(state["call"], grad_rgb, grad_depth, grad_alpha, record_grads,
                                         scene_grads=scene_grads, grad_viewmats=grad_viewmats,
                                         camera_grads=camera_grads)

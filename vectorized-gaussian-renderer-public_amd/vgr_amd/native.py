@@ -44,7 +44,8 @@ WORKSPACE_TENSORS = (
 RECORD_FLOATS = 12  # 48-byte visible record, see vgr_record in include/vgr.h
 RECORD_GRAD_FLOATS = 10  # dL/d (mean_x, mean_y, conic_0..2, opacity, depth, r, g, b) per intersection slot
 CAMERA_GRAD_FLOATS = 24  # per-camera pose partials of the backward pass
-ABI_VERSION = 4
+POSE_REPLICAS = 32  # copies of the pose partials (VGR_POSE_REPLICAS)
+ABI_VERSION = 5
 # Device counter block: the reference's five counters (cuda_backend.py:22-28) plus
 # one slot counting radix-sort look-back waits that gave up (the order is then wrong).
 COUNTER_SLOTS = 6
@@ -216,6 +217,8 @@ EXPORTED_SYMBOLS: dict[str, tuple[type, tuple]] = {
     "vgr_aggregate_chunk_counters": (_I32, (_P, _I64, _I64, _P, _P, _P)),
     "vgr_fill_i64": (_I32, (_P, _I64, _I64, _P)),
     "vgr_fill_f32": (_I32, (_P, _I64, ctypes.c_float, _P)),
+    "vgr_pose_compose": (_I32, (_P, _P, _I64, _P, _P)),
+    "vgr_pose_compose_backward": (_I32, (_P, _P, _P, _I64, _P, _P, _P)),
 }
 
 
@@ -592,8 +595,10 @@ class NativeRenderer:
             gradients.grad_opacities = _ptr(opacities)
             gradients.grad_features = _ptr(features)
         if grad_viewmats is not None:
-            if camera_grads is None or camera_grads.numel() < call.cameras.active_count * CAMERA_GRAD_FLOATS:
-                raise ValueError("pose gradients need camera_grads workspace of 24 floats per active camera")
+            if camera_grads is None or (camera_grads.numel() <
+                                        POSE_REPLICAS * call.cameras.active_count * CAMERA_GRAD_FLOATS):
+                raise ValueError("pose gradients need camera_grads workspace of POSE_REPLICAS x 24 floats per "
+                                 "active camera")
             if grad_viewmats.numel() < 16 * call.cameras.batch:
                 raise ValueError("grad_viewmats must be [B,4,4]")
             gradients.grad_viewmats = _ptr(grad_viewmats)
@@ -613,6 +618,25 @@ class NativeRenderer:
             self._check(self.lib.vgr_covariance_backward(_ptr(scales), _ptr(rotations), _ptr(grad_covariances), count,
                                                          _ptr(grad_scales), _ptr(grad_rotations),
                                                          _stream(scales.device)), "vgr_covariance_backward")
+
+    def pose_compose(self, twists: torch.Tensor, base_views: torch.Tensor, views: torch.Tensor) -> None:
+        count = int(twists.shape[0])
+        if count == 0:
+            return
+        with torch.cuda.device(twists.device):
+            self._check(self.lib.vgr_pose_compose(_ptr(twists), _ptr(base_views), count, _ptr(views),
+                                                  _stream(twists.device)), "vgr_pose_compose")
+
+    def pose_compose_backward(self, twists: torch.Tensor, base_views: torch.Tensor, grad_views: torch.Tensor,
+                              grad_twists: torch.Tensor, grad_base_views: torch.Tensor | None) -> None:
+        count = int(twists.shape[0])
+        if count == 0:
+            return
+        with torch.cuda.device(twists.device):
+            self._check(self.lib.vgr_pose_compose_backward(
+                _ptr(twists), _ptr(base_views), _ptr(grad_views), count, _ptr(grad_twists),
+                _ptr(grad_base_views) if grad_base_views is not None else None, _stream(twists.device)),
+                "vgr_pose_compose_backward")
 
 
 class RenderCall:
