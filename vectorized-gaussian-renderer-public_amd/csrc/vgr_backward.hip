@@ -1309,10 +1309,14 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
       // 64 keeps the prefetch registers and LDS small enough for five waves per SIMD.
       // One wave walks a whole tile list, so a long tile that starts late runs
       // alone at the end of the grid: tiles are launched longest first.
+      // The checkpointed forward ordered grids of up to kOrderedCompositeBlocks tiles
+      // into this scratch from the same lists; its order is reused.
       int32_t* order = reinterpret_cast<int32_t*>(const_cast<char*>(scratch) + layout.tile_order);
-      const int status = launch_tile_order(table, ws.tile_starts, ws.tile_ends, cameras.active_count,
-                                           cargs.uniform_tiles, tile_blocks, order, stream);
-      if (status != 0) return status;
+      if (tile_blocks > kOrderedCompositeBlocks) {
+        const int status = launch_tile_order(table, ws.tile_starts, ws.tile_ends, cameras.active_count,
+                                             cargs.uniform_tiles, tile_blocks, order, stream);
+        if (status != 0) return status;
+      }
       cargs.tile_order = order;
       launch_composite_backward_wave<64>(cargs, tile_blocks, stream);
     }

@@ -544,6 +544,13 @@ extern "C" int32_t vgr_render_static_scratch(const vgr_scene* scene, const vgr_c
     set_error("vgr_render_static_scratch: null argument struct");
     return VGR_E_INVALID_ARGUMENT;
   }
+  if (outputs->final_transmittance != nullptr || outputs->last_blended != nullptr ||
+      workspace->slot_records != nullptr) {
+    // vgr_render_backward re-reads the forward's scratch from workspace->scratch.
+    set_error("vgr_render_static_scratch: checkpointed (training) forwards go through vgr_render, whose scratch "
+              "the backward pass re-reads");
+    return VGR_E_INVALID_ARGUMENT;
+  }
   void* address = nullptr;
   int status = static_scratch_address(&address);
   if (status != 0) return status;

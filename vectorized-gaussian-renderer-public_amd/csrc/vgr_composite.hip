@@ -32,7 +32,6 @@ using namespace dev;
 // forward+backward 9.6 -> 8.0 ms.
 constexpr int kOrderBuckets = 1024;
 constexpr int kOrderThreads = 1024;
-constexpr int64_t kOrderedCompositeBlocks = 32768;  // forward grids up to this many tiles are ordered
 
 __device__ __forceinline__ int order_bucket(const CameraParams* __restrict__ table,
                                             const int32_t* __restrict__ tile_starts,

@@ -88,6 +88,9 @@ struct ScratchLayout {
 };
 
 constexpr int kProjectionWaves = kBlockThreads / kWave;  // waves of a projection workgroup
+// Compositing grids up to this many tiles run in a longest-list-first order
+// (ScratchLayout::tile_order), which the backward of a checkpointed forward reuses.
+constexpr int64_t kOrderedCompositeBlocks = 32768;
 constexpr int kSortThreads = 512;      // threads per onesweep workgroup (8 waves)
 constexpr int kDefaultSortItems = 16;  // keys per thread per sort tile (tile = kSortThreads * items keys)
 

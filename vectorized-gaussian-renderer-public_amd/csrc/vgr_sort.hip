@@ -196,11 +196,16 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
   __shared__ int32_t s_global_start[kRadix];  // capacities are below 2^30
   __shared__ uint32_t s_ticket;
 
+  // The grid covers the capacity; the device-side key count needs its first
+  // ceil(n / tile) workgroups.  Those take the tickets (tiles in the order they
+  // ask, so every predecessor of a ticket holder is running or done), and every
+  // workgroup past them leaves at once, before any atomic or LDS work.
+  const int64_t n = element_count(counters, capacity);
+  if (static_cast<int64_t>(blockIdx.x) * kSortTile >= n) return;
   if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1u);
   for (int i = threadIdx.x; i < kWaves * kRadix; i += kSortThreads) (&s_count[0][0])[i] = 0;
   __syncthreads();
   const int64_t tile = s_ticket;
-  const int64_t n = element_count(counters, capacity);
   const int64_t tile_start = tile * kSortTile;
   if (tile_start >= n) return;
   const int valid_count = static_cast<int>(n - tile_start < kSortTile ? n - tile_start : kSortTile);

@@ -392,11 +392,10 @@ class _RasterizeFunction(torch.autograd.Function):
         if grad_rgb is None:
             grad_rgb = torch.zeros_like(outputs["rgb"])
         grad_rgb = grad_rgb.contiguous().float()
+        # depth is +inf where nothing was blended; the compositing backward reads dL/d depth
+        # only where the accumulated alpha is positive, so whatever it holds there is ignored.
         grad_depth = grad_depth.contiguous().float() if grad_depth is not None else None
         grad_alpha = grad_alpha.contiguous().float() if grad_alpha is not None else None
-        if grad_depth is not None:
-            # depth is +inf where nothing was blended; its gradient there carries no information.
-            grad_depth = torch.where(torch.isfinite(outputs["depth"]), grad_depth, torch.zeros_like(grad_depth))
         means, covariances, opacities, features = (state["scene"][0], state["covariances"], state["scene"][2],
                                                    state["scene"][3])
         workspace: _Workspace = state["workspace"]

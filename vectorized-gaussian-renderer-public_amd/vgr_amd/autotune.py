@@ -9,7 +9,9 @@ signature) in a JSON file so later processes start tuned.
 
 Search is coordinate descent over the knobs in a fixed order (each knob's
 best value is kept before the next knob is swept), which needs about a dozen
-timed configurations instead of the full product.  Every configuration is
+timed configurations instead of the full product.  It starts from the library
+defaults (the first value of every knob) and moves only for a gain above
+SWITCH_MARGIN.  Every configuration is
 measured as the median of a few timed renders after one untimed render that
 also sizes its capacities.
 
@@ -31,10 +33,13 @@ import torch
 DEFAULT_SPACE: dict[str, tuple[int, ...]] = {
     "front_ordering": (1, 0),  # 1: tile-front ordering where the plan allows it, 0: global radix sort
     "tile_size": (16, 8),
-    "raster_batch": (256, 128, 512, 64),
+    "raster_batch": (0, 256, 128, 512, 64),  # 0: the library default
     "sort_digit_bits": (0, 8, 7, 6),  # 0: the library default (8, or 9 where that saves a pass)
-    "cameras_per_block": (32, 16, 8, 64),
+    "cameras_per_block": (0, 32, 16, 8, 64),  # 0: the library default
 }
+# A candidate replaces the best configuration only when it is faster by more than
+# this fraction, so timing noise does not move the search off the defaults.
+SWITCH_MARGIN = 0.01
 
 DEFAULT_CACHE = Path(os.environ.get("VGR_AUTOTUNE_CACHE", Path.home() / ".cache" / "vgr_amd" / "autotune.json"))
 
@@ -128,7 +133,7 @@ class Autotuner:
                 candidate = dict(best)
                 candidate[name] = value
                 elapsed = self._time(measure, candidate)
-                if elapsed < best_time:
+                if elapsed < best_time * (1.0 - SWITCH_MARGIN):
                     best, best_time = candidate, elapsed
         if best_time == float("inf"):
             raise RuntimeError("no candidate configuration could be measured")

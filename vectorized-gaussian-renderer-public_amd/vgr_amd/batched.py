@@ -114,6 +114,9 @@ class BatchedRenderer:
         self.persistent_composite = persistent_composite
         self.tuning = dict(sort_digit_bits=sort_digit_bits, sort_tile_items=sort_tile_items,
                            raster_batch=raster_batch, cameras_per_block=cameras_per_block)
+        # The plain per-group path keeps the constructed launch parameters: configure()
+        # and the autotuner change the batched path only.
+        self.plain_tuning = dict(self.tuning)
         self.capacity_headroom = capacity_headroom
         self.max_visible = int(max_visible_per_submission)
         self.max_intersections_limit = int(max_intersections_per_submission)
@@ -143,8 +146,10 @@ class BatchedRenderer:
     def configure(self, *, tile_size: int | None = None, raster_batch: int | None = None,
                   sort_digit_bits: int | None = None, cameras_per_block: int | None = None,
                   sort_tile_items: int | None = None, front_ordering: int | None = None) -> None:
-        """Change launch parameters; takes effect at the next plan().  ``front_ordering`` 1 asks for
-        the tile-front ordering where the plan allows it, 0 for the global radix sort."""
+        """Change the batched path's launch parameters; takes effect at the next plan().
+        ``front_ordering`` 1 asks for the tile-front ordering where the plan allows it, 0 for the
+        global radix sort.  The tile size applies to both paths; the plain per-group path keeps
+        the other constructed parameters."""
         if front_ordering is not None:
             self.ordering_request = "front" if int(front_ordering) else "global"
         if tile_size is not None:
@@ -308,8 +313,9 @@ class BatchedRenderer:
             max_views=plain_views, height=max(c.height for c in self.cameras),
             width=max(c.width for c in self.cameras), gaussians=g, intersection_capacity=intersection_capacity,
             tile_size=self.options["tile_size"], near_plane=self.options["near_plane"],
-            far_plane=self.options["far_plane"], sort_digit_bits=self.tuning["sort_digit_bits"],
-            cameras_per_block=self.tuning["cameras_per_block"], sort_tile_items=self.tuning["sort_tile_items"])
+            far_plane=self.options["far_plane"], sort_digit_bits=self.plain_tuning["sort_digit_bits"],
+            cameras_per_block=self.plain_tuning["cameras_per_block"],
+            sort_tile_items=self.plain_tuning["sort_tile_items"])
         # Buffers are shared by both orderings and the plain path: never shrink below
         # what an earlier calibration needed.
         self._scratch_floor = max(self._scratch_floor, batched, plain)
@@ -546,5 +552,5 @@ class BatchedRenderer:
                     self.options["gaussian_support_sigma"], self.options["covariance_epsilon"],
                     self.options["antialiased"], self.options["semantic_min_alpha"], False, self.options["tile_size"],
                     4096, 64, False, False, False, self.options["output_srgb"], True, self.full_sensor_output, False,
-                    sh_degree=self.sh_degree, **self.tuning)
+                    sh_degree=self.sh_degree, **self.plain_tuning)
         return self.plain_groups
