@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define VGR_ABI_VERSION 5
+#define VGR_ABI_VERSION 6
 
 enum vgr_status {
   VGR_OK = 0,
@@ -166,7 +166,7 @@ typedef struct vgr_options {
   float covariance_epsilon;
   float semantic_min_alpha;
   int32_t antialiased;
-  int32_t tile_size;                /* 8 or 16 */
+  int32_t tile_size;                /* 1, 2, 4, 8 or 16 (power of two, renderer_cuda.cu tile_size) */
   int32_t output_srgb;
   int32_t deterministic;            /* accepted; ordering is always deterministic */
   int32_t full_sensor_output;       /* 1: rgb+depth+alpha+semantic, 0: rgb only */
@@ -195,6 +195,20 @@ typedef struct vgr_options {
                                        0 = default (2^22); small values make give-ups likely and -1 makes
                                        every wait give up at once (both for tests of the fail-closed path) */
   int32_t ordering;                 /* VGR_ORDERING_GLOBAL_RADIX (default) or VGR_ORDERING_TILE_FRONT */
+  /* Dense per-pixel path (tile_size 1, renderer_cuda.cu:1093-1690, 1764-1805).
+   * Intersections are (pixel, record) pairs whose alpha at the pixel centre
+   * reaches 1/255; records (and workspace.radii) are written first, then the
+   * pixel intersections are emitted per record.  depth_bucket_count > 0 adds
+   * the optical-thickness cutoff: records are bucketed by depth (buckets
+   * uniform over [near, far]), each pixel sums a conservative optical
+   * thickness (8-term -log(1 - alpha) series x 0.999, in 1/65536 fixed point
+   * so the sums do not depend on the order of the adds) bucket by bucket,
+   * depth_bucket_group_size buckets per pass, and its intersections behind the
+   * first bucket where the sum reaches 9.2103 (T <= 1e-4) are not emitted.  The
+   * compositor stops before such entries in any case, so images are identical
+   * with and without the cutoff; only the intersection count shrinks. */
+  int32_t depth_bucket_count;       /* 0 = no cutoff; else the bucket count (reference default 4096) */
+  int32_t depth_bucket_group_size;  /* buckets per cutoff pass: 0 = default (64) */
 } vgr_options;
 
 /* Intersection ordering of a render.

@@ -38,7 +38,8 @@ class _Options(ctypes.Structure):
                 ("far_plane", ctypes.c_float), ("gaussian_support_sigma", ctypes.c_float),
                 ("covariance_epsilon", ctypes.c_float), ("semantic_min_alpha", ctypes.c_float),
                 ("antialiased", ctypes.c_int32), ("tile_size", ctypes.c_int32), ("output_srgb", ctypes.c_int32),
-                ("full_sensor_output", ctypes.c_int32), ("threads", ctypes.c_int32)]
+                ("full_sensor_output", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("depth_bucket_count", ctypes.c_int32), ("depth_bucket_group_size", ctypes.c_int32)]
 
 
 class _Outputs(ctypes.Structure):
@@ -145,6 +146,8 @@ def render(
     threads: int = 0,
     want_sorted: bool = False,
     background: bool = True,
+    depth_bucket_count: int = 0,
+    depth_bucket_group_size: int = 0,
 ) -> dict[str, np.ndarray]:
     """Render a batch on the CPU.
 
@@ -192,7 +195,7 @@ def render(
                       0 if active is None else active.shape[0], _p(widths), _p(heights), _p(pixel_offsets))
     c_opt = _Options(height, width, near_plane, far_plane, gaussian_support_sigma, covariance_epsilon,
                      semantic_min_alpha, int(antialiased), tile_size, int(output_srgb), int(full_sensor_output),
-                     threads)
+                     threads, int(depth_bucket_count), int(depth_bucket_group_size))
     rgb = np.zeros(shape + (3,), dtype=np.float32)
     depth = np.full(shape + (1,), np.inf if background else 0.0, dtype=np.float32)
     alpha = np.zeros(shape + (1,), dtype=np.float32)

@@ -429,6 +429,118 @@ static int tile_rectangle(const projected* p, int tile_size, int tiles_x, int ti
   return (min_x < max_x && min_y < max_y) ? 1 : 0;
 }
 
+/* ---- tile size 1: per-pixel emission and the optical-thickness cutoff ---------- */
+
+/* depth_bucket_index, renderer_cuda.cu:168-177. */
+static int depth_bucket_index(float depth, float near_plane, float far_plane, int bucket_count) {
+  const float normalized = (depth - near_plane) / (far_plane - near_plane);
+  int index = (int)(normalized * (float)bucket_count);
+  if (index < 0) index = 0;
+  return index < bucket_count - 1 ? index : bucket_count - 1;
+}
+
+/* splat_alpha_at_pixel, renderer_cuda.cu:179-200 (vgr_expf for __expf). */
+static float splat_alpha_at_pixel(const projected* p, int x, int y) {
+  const float dx = ((float)x + 0.5f) - p->mean_x;
+  const float dy = ((float)y + 0.5f) - p->mean_y;
+  const float power = -0.5f * (p->conic_0 * dx * dx + 2.0f * p->conic_1 * dx * dy + p->conic_2 * dy * dy);
+  if (power > 0.0f || power < -20.0f) return 0.0f;
+  const float alpha = fminf(VGR_MAX_ALPHA, p->opacity * vgr_expf(power));
+  return alpha >= VGR_ALPHA_THRESHOLD ? alpha : 0.0f;
+}
+
+/* conservative_optical_thickness, renderer_cuda.cu:247-257, in 1/65536 units
+ * rounded down (this build sums the thickness in integers). */
+static uint32_t optical_thickness_fixed(float alpha) {
+  float term = alpha;
+  float result = term;
+  for (int order = 2; order <= 8; ++order) {
+    term *= alpha;
+    result += term / (float)order;
+  }
+  return (uint32_t)((result * 0.999f) * 65536.0f);
+}
+
+/* ceil(9.210340371976184 * 65536): transmittance 1e-4, renderer_cuda.cu:1672. */
+#define ORACLE_TAU_THRESHOLD_FIXED 603610ull
+
+typedef struct tau_entry {
+  int64_t pixel;
+  int32_t bucket;
+  uint32_t tau;
+} tau_entry;
+
+static int compare_tau_entries(const void* a, const void* b) {
+  const tau_entry* x = (const tau_entry*)a;
+  const tau_entry* y = (const tau_entry*)b;
+  if (x->pixel != y->pixel) return x->pixel < y->pixel ? -1 : 1;
+  if (x->bucket != y->bucket) return x->bucket < y->bucket ? -1 : 1;
+  return 0;
+}
+
+/* Per-pixel cutoff bucket of one camera (advance_depth_cutoff_kernel,
+ * renderer_cuda.cu:1658-1690, over all buckets at once): the first bucket where
+ * the running thickness reaches the threshold, else the last bucket.  Per
+ * (pixel, bucket) sums wrap in 32 bits, as the device table's do. */
+static int32_t* depth_cutoffs(const camera_job* job, int buckets) {
+  const camera_setup* c = job->cam;
+  const oracle_options* opt = job->opt;
+  const int64_t pixels = (int64_t)c->width * c->height;
+  int32_t* cutoff = (int32_t*)malloc(sizeof(int32_t) * (size_t)(pixels > 0 ? pixels : 1));
+  for (int64_t i = 0; i < pixels; ++i) cutoff[i] = buckets - 1;
+  int64_t entries = 0, capacity = 1024;
+  tau_entry* list = (tau_entry*)malloc(sizeof(tau_entry) * (size_t)capacity);
+  for (int64_t v = 0; v < job->visible; ++v) {
+    const projected* p = &job->proj[job->records[v]];
+    int rect[4];
+    if (!tile_rectangle(p, 1, c->tiles_x, c->tiles_y, rect)) continue;
+    const int bucket = depth_bucket_index(p->depth, opt->near_plane, opt->far_plane, buckets);
+    for (int y = rect[1]; y < rect[3]; ++y) {
+      for (int x = rect[0]; x < rect[2]; ++x) {
+        const float alpha = splat_alpha_at_pixel(p, x, y);
+        if (alpha == 0.0f) continue;
+        if (entries == capacity) {
+          capacity *= 2;
+          list = (tau_entry*)realloc(list, sizeof(tau_entry) * (size_t)capacity);
+        }
+        list[entries].pixel = (int64_t)y * c->width + x;
+        list[entries].bucket = bucket;
+        list[entries].tau = optical_thickness_fixed(alpha);
+        ++entries;
+      }
+    }
+  }
+  qsort(list, (size_t)entries, sizeof(tau_entry), compare_tau_entries);
+  for (int64_t i = 0; i < entries;) {
+    const int64_t pixel = list[i].pixel;
+    uint64_t running = 0;
+    int found = -1;
+    while (i < entries && list[i].pixel == pixel) {
+      const int32_t bucket = list[i].bucket;
+      uint32_t bucket_sum = 0;
+      while (i < entries && list[i].pixel == pixel && list[i].bucket == bucket) bucket_sum += list[i++].tau;
+      running += bucket_sum;
+      if (found < 0 && running >= ORACLE_TAU_THRESHOLD_FIXED) found = bucket;
+    }
+    if (found >= 0) cutoff[pixel] = found;
+  }
+  free(list);
+  return cutoff;
+}
+
+/* Whether a visible splat emits an intersection at tile (tx, ty): every tile of
+ * its rectangle at tile sizes above 1; at tile size 1 only pixels where its alpha
+ * reaches 1/255 and that lie in front of the pixel's cutoff (bin_tiles_kernel,
+ * renderer_cuda.cu:1764-1805). */
+static int emits_at(const camera_job* job, const projected* p, int tx, int ty, const int32_t* cutoff) {
+  if (job->opt->tile_size != 1) return 1;
+  if (splat_alpha_at_pixel(p, tx, ty) == 0.0f) return 0;
+  if (cutoff == NULL) return 1;
+  const int bucket = depth_bucket_index(p->depth, job->opt->near_plane, job->opt->far_plane,
+                                        job->opt->depth_bucket_count);
+  return bucket <= cutoff[(int64_t)ty * job->cam->width + tx];
+}
+
 static int compare_intersections(const void* a, const void* b) {
   const intersection* x = (const intersection*)a;
   const intersection* y = (const intersection*)b;
@@ -520,6 +632,10 @@ int32_t oracle_render(const oracle_scene* scene, const oracle_cameras* cams, con
     snprintf(g_error, sizeof(g_error), "tile_size must be a power of two in [1, 16]");
     return 1;
   }
+  if (opt->depth_bucket_count < 0 || (opt->depth_bucket_count > 0 && opt->tile_size != 1)) {
+    snprintf(g_error, sizeof(g_error), "depth_bucket_count applies to tile_size 1 and must be non-negative");
+    return 1;
+  }
   int threads = opt->threads;
   if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
   if (threads <= 0) threads = 1;
@@ -545,12 +661,18 @@ int32_t oracle_render(const oracle_scene* scene, const oracle_cameras* cams, con
     const int64_t tiles = (int64_t)cam.tiles_x * cam.tiles_y;
     job.tile_offsets = (int64_t*)calloc((size_t)tiles + 1, sizeof(int64_t));
     for (int64_t local = 0; local < count; ++local) {
-      if (!job.proj[local].visible) continue;
-      job.records[job.visible++] = (int32_t)local;
+      if (job.proj[local].visible) job.records[job.visible++] = (int32_t)local;
+    }
+    int32_t* cutoff = (opt->tile_size == 1 && opt->depth_bucket_count > 0)
+                          ? depth_cutoffs(&job, opt->depth_bucket_count)
+                          : NULL;
+    for (int64_t v = 0; v < job.visible; ++v) {
+      const projected* p = &job.proj[job.records[v]];
       int rect[4];
-      if (!tile_rectangle(&job.proj[local], opt->tile_size, cam.tiles_x, cam.tiles_y, rect)) continue;
+      if (!tile_rectangle(p, opt->tile_size, cam.tiles_x, cam.tiles_y, rect)) continue;
       for (int ty = rect[1]; ty < rect[3]; ++ty)
-        for (int tx = rect[0]; tx < rect[2]; ++tx) job.tile_offsets[(int64_t)ty * cam.tiles_x + tx + 1] += 1;
+        for (int tx = rect[0]; tx < rect[2]; ++tx)
+          if (emits_at(&job, p, tx, ty, cutoff)) job.tile_offsets[(int64_t)ty * cam.tiles_x + tx + 1] += 1;
     }
     for (int64_t t = 0; t < tiles; ++t) job.tile_offsets[t + 1] += job.tile_offsets[t];
     const int64_t total = job.tile_offsets[tiles];
@@ -563,6 +685,7 @@ int32_t oracle_render(const oracle_scene* scene, const oracle_cameras* cams, con
       if (!tile_rectangle(p, opt->tile_size, cam.tiles_x, cam.tiles_y, rect)) continue;
       for (int ty = rect[1]; ty < rect[3]; ++ty) {
         for (int tx = rect[0]; tx < rect[2]; ++tx) {
+          if (!emits_at(&job, p, tx, ty, cutoff)) continue;
           intersection* item = &job.items[cursor[(int64_t)ty * cam.tiles_x + tx]++];
           memcpy(&item->depth_bits, &p->depth, sizeof(uint32_t));
           item->gaussian = p->gaussian;
@@ -571,6 +694,7 @@ int32_t oracle_render(const oracle_scene* scene, const oracle_cameras* cams, con
       }
     }
     free(cursor);
+    free(cutoff);
     parallel_for(threads, tiles, 16, sort_tiles, &job);
     parallel_for(threads, tiles, 4, composite_tiles, &job);
 
@@ -608,12 +732,21 @@ int32_t oracle_render(const oracle_scene* scene, const oracle_cameras* cams, con
       parallel_for(threads, count, 4096, project_range, &job);
       const int64_t tiles = (int64_t)cam.tiles_x * cam.tiles_y;
       job.tile_offsets = (int64_t*)calloc((size_t)tiles + 1, sizeof(int64_t));
+      job.records = (int32_t*)malloc(sizeof(int32_t) * (size_t)(count > 0 ? count : 1));
+      for (int64_t local = 0; local < count; ++local) {
+        if (job.proj[local].visible) job.records[job.visible++] = (int32_t)local;
+      }
+      int32_t* cutoff = (opt->tile_size == 1 && opt->depth_bucket_count > 0)
+                            ? depth_cutoffs(&job, opt->depth_bucket_count)
+                            : NULL;
       for (int64_t local = 0; local < count; ++local) {
         int rect[4];
         if (!job.proj[local].visible) continue;
         if (!tile_rectangle(&job.proj[local], opt->tile_size, cam.tiles_x, cam.tiles_y, rect)) continue;
         for (int ty = rect[1]; ty < rect[3]; ++ty)
-          for (int tx = rect[0]; tx < rect[2]; ++tx) job.tile_offsets[(int64_t)ty * cam.tiles_x + tx + 1] += 1;
+          for (int tx = rect[0]; tx < rect[2]; ++tx)
+            if (emits_at(&job, &job.proj[local], tx, ty, cutoff))
+              job.tile_offsets[(int64_t)ty * cam.tiles_x + tx + 1] += 1;
       }
       for (int64_t t = 0; t < tiles; ++t) job.tile_offsets[t + 1] += job.tile_offsets[t];
       const int64_t total = job.tile_offsets[tiles];
@@ -626,6 +759,7 @@ int32_t oracle_render(const oracle_scene* scene, const oracle_cameras* cams, con
         if (!p->visible || !tile_rectangle(p, opt->tile_size, cam.tiles_x, cam.tiles_y, rect)) continue;
         for (int ty = rect[1]; ty < rect[3]; ++ty) {
           for (int tx = rect[0]; tx < rect[2]; ++tx) {
+            if (!emits_at(&job, p, tx, ty, cutoff)) continue;
             intersection* item = &job.items[cursor[(int64_t)ty * cam.tiles_x + tx]++];
             memcpy(&item->depth_bits, &p->depth, sizeof(uint32_t));
             item->gaussian = p->gaussian;
@@ -634,6 +768,8 @@ int32_t oracle_render(const oracle_scene* scene, const oracle_cameras* cams, con
         }
       }
       free(cursor);
+      free(cutoff);
+      free(job.records);
       parallel_for(threads, tiles, 16, sort_tiles, &job);
       for (int64_t t = 0; t < tiles && written < out->sorted_capacity; ++t) {
         for (int64_t i = job.tile_offsets[t]; i < job.tile_offsets[t + 1] && written < out->sorted_capacity; ++i) {

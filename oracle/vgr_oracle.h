@@ -68,6 +68,10 @@ typedef struct oracle_options {
   int32_t output_srgb;
   int32_t full_sensor_output;
   int32_t threads;           /* 0 = all online CPUs */
+  /* tile_size 1 only: depth buckets of the optical-thickness cutoff (0 = none)
+   * and buckets per pass (accepted; the cutoff does not depend on it). */
+  int32_t depth_bucket_count;
+  int32_t depth_bucket_group_size;
 } oracle_options;
 
 typedef struct oracle_outputs {

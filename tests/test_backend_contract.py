@@ -258,14 +258,41 @@ def test_environment_transforms_are_forwarded_and_must_align():
 
 
 def test_unsupported_reference_modes_are_rejected():
-    with pytest.raises(ValueError, match="dense 1x1"):
-        CustomHipBackend(compact_projection_cache=True, tile_size=1)
     with pytest.raises(ValueError, match="ray_gaussian_evaluation"):
         CustomHipBackend(ray_gaussian_evaluation=True)
-    with pytest.raises(ValueError, match="8 or 16"):
-        CustomHipBackend(tile_size=4)
     with pytest.raises(ValueError, match="fixed_capacity_sort"):
         CustomHipBackend(fixed_capacity_sort=True, deterministic=True)
+
+
+def test_dense_path_knobs_follow_the_reference_rules():
+    """cuda_backend.py:156-191: tile sizes are powers of two in [1, 16]; the compact
+    projection cache needs tile size 1, the screen-space model and nondeterministic
+    mode; materialized records need the compact cache."""
+    for tile_size in (1, 2, 4, 8, 16):
+        CustomHipBackend(tile_size=tile_size)
+    CustomHipBackend(tile_size=1, compact_projection_cache=True, materialize_projected_records=True)
+    for kwargs, message in (({"tile_size": 3}, "power of two"), ({"tile_size": 32}, "power of two"),
+                            ({"compact_projection_cache": True, "tile_size": 4}, "requires tile_size=1"),
+                            ({"compact_projection_cache": True, "tile_size": 1, "deterministic": True},
+                             "requires deterministic=False"),
+                            ({"materialize_projected_records": True, "tile_size": 1},
+                             "requires compact_projection_cache=True"),
+                            ({"depth_bucket_count": 0}, "depth_bucket_count"),
+                            ({"depth_bucket_group_size": 0}, "depth_bucket_group_size")):
+        with pytest.raises(ValueError, match=message):
+            CustomHipBackend(**kwargs)
+
+
+def test_dense_cutoff_follows_the_reference_modes():
+    """The cutoff runs at tile size 1 in nondeterministic mode (grouped) or with the
+    compact cache (one pass over every bucket), renderer_cuda.cu:2888-3144."""
+    from vgr_amd.native import dense_cutoff
+    assert dense_cutoff(1, 4096, 64, False, False) == (4096, 64)
+    assert dense_cutoff(1, 4096, 64, True, False) == (4096, 4096)
+    assert dense_cutoff(1, 4096, 64, False, True) == (0, 0)
+    assert dense_cutoff(1, 32, 64, False, False) == (32, 32)
+    assert dense_cutoff(2, 4096, 64, False, False) == (0, 0)
+    assert dense_cutoff(16, 4096, 64, True, False) == (0, 0)
 
 
 @pytest.mark.parametrize("kwargs", [

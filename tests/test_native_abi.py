@@ -64,5 +64,5 @@ def test_scratch_query_is_monotonic_in_tile_count_with_default_digits():
 def test_invalid_options_are_reported_with_a_message():
     module = native.NativeRenderer()
     with pytest.raises(RuntimeError, match="tile_size"):
-        module.scratch_bytes(max_views=1, height=8, width=8, gaussians=1, intersection_capacity=1, tile_size=4,
+        module.scratch_bytes(max_views=1, height=8, width=8, gaussians=1, intersection_capacity=1, tile_size=3,
                              near_plane=0.01, far_plane=100.0)

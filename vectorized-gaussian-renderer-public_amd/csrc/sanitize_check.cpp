@@ -82,6 +82,25 @@ void planning() {
   options.ordering = VGR_ORDERING_TILE_FRONT;
   expect(vgr_workspace_sizes_query(&cameras, &options, 5000000, 1000, &sizes) == VGR_E_INVALID_ARGUMENT,
          "front rejects large cameras");
+  // Dense path: per-pixel tiles with and without the depth cutoff; the cutoff only at tile size 1.
+  vgr_cameras dense = uniform_cameras(3);
+  vgr_options dense_options = default_options();
+  dense_options.tile_size = 1;
+  vgr_workspace_sizes plain_dense, cut_dense;
+  expect(vgr_workspace_sizes_query(&dense, &dense_options, 100000, 4000000, &plain_dense) == 0, "dense sizes");
+  expect(plain_dense.tile_capacity == 3LL * 640 * 480, "dense tiles are pixels");
+  dense_options.depth_bucket_count = 4096;
+  dense_options.depth_bucket_group_size = 64;
+  expect(vgr_workspace_sizes_query(&dense, &dense_options, 100000, 4000000, &cut_dense) == 0, "cutoff sizes");
+  expect(cut_dense.scratch_bytes >= plain_dense.scratch_bytes + 3LL * 640 * 480 * (64 * 4 + 8 + 4),
+         "cutoff tables in scratch");
+  dense_options.tile_size = 16;
+  expect(vgr_workspace_sizes_query(&dense, &dense_options, 100000, 4000000, &sizes) == VGR_E_INVALID_ARGUMENT,
+         "cutoff needs tile size 1");
+  dense_options.tile_size = 3;
+  dense_options.depth_bucket_count = 0;
+  expect(vgr_workspace_sizes_query(&dense, &dense_options, 100000, 4000000, &sizes) == VGR_E_INVALID_ARGUMENT,
+         "tile size must be a power of two");
 }
 
 void validation() {
@@ -154,6 +173,11 @@ void validation() {
   expect(vgr_render(&scene, &cameras, &outputs, &workspace, &options, nullptr, &selector) ==
              VGR_E_INVALID_ARGUMENT, "front needs all cameras");
   cameras.active_ids = nullptr;
+  options = default_options();
+  options.tile_size = 1;
+  options.depth_bucket_count = 64;
+  expect(vgr_render(&scene, &cameras, &outputs, &workspace, &options, nullptr, &selector) ==
+             VGR_E_INVALID_ARGUMENT, "cutoff needs radii");
   options = default_options();
   vgr_gradients gradients;
   std::memset(&gradients, 0, sizeof(gradients));

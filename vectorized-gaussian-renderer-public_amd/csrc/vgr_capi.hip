@@ -52,6 +52,13 @@ int resolve_cameras_per_block(const vgr_options& options) {
   return options.cameras_per_block == 0 ? 32 : options.cameras_per_block;
 }
 
+// Depth-cutoff buckets per pass of the dense path: 0 = 64 (the reference default),
+// never more than the bucket count.
+int32_t resolve_depth_group(const vgr_options& options) {
+  const int32_t group = options.depth_bucket_group_size == 0 ? 64 : options.depth_bucket_group_size;
+  return group < options.depth_bucket_count ? group : options.depth_bucket_count;
+}
+
 // Tile-front ordering: LDS budgets of the bucketed emission (count kernel
 // histogram over a camera group's tiles; write kernel tables over one camera's
 // tiles).
@@ -67,7 +74,8 @@ int64_t max_camera_tiles(const vgr_cameras& cameras, const vgr_options& options)
 }
 
 ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const vgr_options& options,
-                             int64_t global_tiles, int64_t intersection_capacity, int64_t camera_tiles) {
+                             int64_t global_tiles, int64_t intersection_capacity, int64_t camera_tiles,
+                             int32_t batch) {
   ScratchLayout layout{};
   layout.cameras_per_block = resolve_cameras_per_block(options);
   layout.front = options.ordering == VGR_ORDERING_TILE_FRONT;
@@ -171,13 +179,25 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   offset = align_up(offset + sizeof(uint32_t) * 4);
   layout.front_open_tiles = offset;
   offset = align_up(offset + sizeof(int32_t) * front_tiles);
+  const bool cutoff = options.tile_size == 1 && options.depth_bucket_count > 0;
+  layout.dense_pixels = cutoff ? global_tiles : 0;
+  layout.dense_group_size = cutoff ? resolve_depth_group(options) : 0;
+  const size_t dense_pixels = static_cast<size_t>(layout.dense_pixels);
+  layout.dense_camera_map = offset;
+  offset = align_up(offset + sizeof(int32_t) * (cutoff ? static_cast<size_t>(batch > 0 ? batch : 1) : 0u));
+  layout.dense_cutoff = offset;
+  offset = align_up(offset + sizeof(int32_t) * dense_pixels);
+  layout.dense_accumulated = offset;
+  offset = align_up(offset + sizeof(uint64_t) * dense_pixels);
+  layout.dense_tau = offset;
+  offset = align_up(offset + sizeof(uint32_t) * dense_pixels * static_cast<size_t>(layout.dense_group_size));
   layout.total = offset;
   return layout;
 }
 
 int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
-  if (options.tile_size != 8 && options.tile_size != 16) {
-    set_error("tile_size must be 8 or 16 on this path; got %d", options.tile_size);
+  if (options.tile_size <= 0 || options.tile_size > 16 || (options.tile_size & (options.tile_size - 1)) != 0) {
+    set_error("tile_size must be a power of two in [1, 16]; got %d", options.tile_size);
     return VGR_E_INVALID_ARGUMENT;
   }
   const bool uniform = cameras.widths == nullptr && cameras.heights == nullptr;
@@ -223,6 +243,17 @@ int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
   const int group = resolve_cameras_per_block(options);
   if (group < 1 || group > 64) {
     set_error("cameras_per_block must be in [1, 64]; got %d", options.cameras_per_block);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (options.depth_bucket_count < 0 || options.depth_bucket_group_size < 0 ||
+      (options.depth_bucket_count > 0 && options.tile_size != 1)) {
+    set_error("depth_bucket_count (>= 0) and depth_bucket_group_size (>= 0) apply to tile_size 1; got %d, %d at "
+              "tile_size %d",
+              options.depth_bucket_count, options.depth_bucket_group_size, options.tile_size);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (options.depth_bucket_count > (1 << 24)) {
+    set_error("depth_bucket_count must be at most 2^24; got %d", options.depth_bucket_count);
     return VGR_E_INVALID_ARGUMENT;
   }
   if (options.ordering != VGR_ORDERING_GLOBAL_RADIX && options.ordering != VGR_ORDERING_TILE_FRONT) {
@@ -283,7 +314,8 @@ extern "C" int32_t vgr_workspace_sizes_query(const vgr_cameras* cameras, const v
   if (status != 0) return status;
   const int64_t tiles = global_tile_count(*cameras, *options);
   const ScratchLayout layout = compute_layout(cameras->active_count, scene_gaussians, *options, tiles,
-                                              intersection_capacity, max_camera_tiles(*cameras, *options));
+                                              intersection_capacity, max_camera_tiles(*cameras, *options),
+                                              cameras->batch);
   out->scratch_bytes = static_cast<int64_t>(layout.total);
   out->tile_capacity = tiles;
   out->sort_key_bits = layout.sort_key_bits;
@@ -297,7 +329,7 @@ extern "C" int64_t vgr_sort_temp_bytes(int64_t n_items, int64_t n_segments) {
   options.tile_size = 16;
   options.near_plane = 0.01f;
   options.far_plane = 100.0f;
-  const ScratchLayout layout = compute_layout(0, 0, options, n_segments, n_items, 0);
+  const ScratchLayout layout = compute_layout(0, 0, options, n_segments, n_items, 0, 0);
   return static_cast<int64_t>(layout.total - layout.sort_histogram);
 }
 
@@ -374,6 +406,14 @@ int prepare_render(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outpu
     set_error("global tile count %lld exceeds the int32/uint32 key range", static_cast<long long>(tiles));
     return VGR_E_INVALID_ARGUMENT;
   }
+  if (opt.tile_size < 8 && (out.final_transmittance != nullptr || ws.slot_records != nullptr)) {
+    set_error("training checkpoints need 8x8 or 16x16 tiles; got tile_size %d", opt.tile_size);
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (opt.tile_size == 1 && opt.depth_bucket_count > 0 && ws.radii == nullptr) {
+    set_error("the depth cutoff (tile_size 1, depth_bucket_count > 0) needs workspace.radii");
+    return VGR_E_INVALID_ARGUMENT;
+  }
   if (ws.slot_records != nullptr && sc.gaussian_ids != nullptr) {
     set_error("slot-sorted (training) forwards take the scene in registration order (gaussian_ids must be NULL)");
     return VGR_E_INVALID_ARGUMENT;
@@ -384,7 +424,7 @@ int prepare_render(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outpu
     return VGR_E_INVALID_ARGUMENT;
   }
   const ScratchLayout layout = compute_layout(cams.active_count, sc.gaussian_count, opt, tiles,
-                                              ws.intersection_capacity, max_camera_tiles(cams, opt));
+                                              ws.intersection_capacity, max_camera_tiles(cams, opt), cams.batch);
   if (ws.scratch_bytes < static_cast<int64_t>(layout.total)) {
     set_error("scratch holds %lld bytes; the render needs %lld", static_cast<long long>(ws.scratch_bytes),
               static_cast<long long>(layout.total));

@@ -371,6 +371,105 @@ __device__ __forceinline__ void composite_tile(const CompositeArgsFor<kDegree>& 
   }
 }
 
+// ---- small tiles (1x1, 2x2, 4x4): one thread per pixel ---------------------------
+//
+// The tile lists are short and private to a few pixels, so there is nothing to
+// stage in LDS: each thread walks its tile's list with direct record loads (the
+// threads of one tile read the same records, which the caches share).  The
+// per-pixel equation is the tiled compositor's: the same alpha test, blend,
+// stop rule, depth normalisation and semantic pick (renderer_cuda.cu:2076-2304).
+template <bool kFull, bool kCount>
+__global__ __launch_bounds__(256) void composite_pixels_kernel(CompositeArgs args, int tile_size,
+                                                               int64_t thread_count) {
+  const int64_t index = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (index >= thread_count) return;
+  const int tile_pixels = tile_size * tile_size;
+  const int64_t block = index / tile_pixels;
+  const int local = static_cast<int>(index - block * tile_pixels);
+  int active;
+  int64_t image_tile;
+  if (args.uniform_tiles > 0) {
+    active = static_cast<int>(block / args.uniform_tiles);
+    if (active >= args.active_count) return;
+    image_tile = block - static_cast<int64_t>(active) * args.uniform_tiles;
+  } else {
+    active = find_active_camera(args.table, args.active_count, block);
+    image_tile = block - args.table[active].active_tile_begin;
+    if (image_tile >= static_cast<int64_t>(args.table[active].tiles_x) * args.table[active].tiles_y) return;
+  }
+  const CameraParams& cam = args.table[active];
+  if (cam.camera < 0) return;
+  const int tile_x = static_cast<int>(image_tile % cam.tiles_x);
+  const int tile_y = static_cast<int>(image_tile / cam.tiles_x);
+  const int pixel_x = tile_x * tile_size + local % tile_size;
+  const int pixel_y = tile_y * tile_size + local / tile_size;
+  if (pixel_x >= cam.width || pixel_y >= cam.height) return;
+  const uint32_t global_tile = cam.tile_base + static_cast<uint32_t>(image_tile);
+  const float px = static_cast<float>(pixel_x) + 0.5f;
+  const float py = static_cast<float>(pixel_y) + 0.5f;
+
+  float transmittance = 1.0f;
+  float red = 0.0f, green = 0.0f, blue = 0.0f, depth_sum = 0.0f;
+  float best_weight = -1.0f;
+  int best_gaussian = -1;
+  int64_t evaluations = 0;
+  const int64_t range_start = args.tile_starts[global_tile];
+  const int64_t range_end = args.tile_ends[global_tile];
+  for (int64_t i = range_start < 0 ? range_end : range_start; i < range_end; ++i) {
+    const int32_t record = args.sorted_values[i];
+    if (record < 0 || record >= args.visible_capacity) continue;  // only with a corrupted list
+    const f32x4* src = reinterpret_cast<const f32x4*>(args.records + record);
+    const f32x4 a = src[0];  // mean_x, mean_y, conic_0, conic_1
+    const f32x4 b = src[1];  // conic_2, opacity, depth, red
+    if (kCount) ++evaluations;
+    const float dx = px - a.x;
+    const float dy = py - a.y;
+    const float power = -0.5f * (a.z * dx * dx + 2.0f * a.w * dx * dy + b.x * dy * dy);
+    if (power > 0.0f || power < -20.0f) continue;
+    const float alpha = fminf(VGR_MAX_ALPHA, b.y * vgr_expf_nonpositive(power));
+    if (alpha < VGR_ALPHA_THRESHOLD) continue;
+    const float next_transmittance = transmittance * (1.0f - alpha);
+    if (next_transmittance <= VGR_TRANSMITTANCE_THRESHOLD) break;
+    const f32x4 c = src[2];  // green, blue, gaussian id bits, camera bits
+    const float weight = transmittance * alpha;
+    red = fmaf(weight, b.w, red);
+    green = fmaf(weight, c.x, green);
+    blue = fmaf(weight, c.y, blue);
+    if (kFull) {
+      depth_sum = fmaf(weight, b.z, depth_sum);
+      if (weight > best_weight) {
+        best_weight = weight;
+        best_gaussian = __float_as_int(c.z);
+      }
+    }
+    transmittance = next_transmittance;
+  }
+  if (kCount) {
+    int64_t total = evaluations;
+    for (int offset = 32; offset > 0; offset >>= 1) total += __shfl_xor(total, offset, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && total != 0) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(args.statistics), static_cast<unsigned long long>(total));
+    }
+  }
+  const int64_t pixel = cam.pixel_offset + static_cast<int64_t>(pixel_y) * cam.width + pixel_x;
+  const float accumulated_alpha = 1.0f - transmittance;
+  if (args.output_srgb) {
+    red = vgr_linear_to_srgb(red);
+    green = vgr_linear_to_srgb(green);
+    blue = vgr_linear_to_srgb(blue);
+  }
+  args.rgb[pixel * 3 + 0] = red;
+  args.rgb[pixel * 3 + 1] = green;
+  args.rgb[pixel * 3 + 2] = blue;
+  if (kFull) {
+    args.alpha[pixel] = accumulated_alpha;
+    args.depth[pixel] = accumulated_alpha > 1.0e-8f ? depth_sum / accumulated_alpha : __int_as_float(0x7f800000);
+    const bool labelled = best_weight >= 0.0f && accumulated_alpha >= args.semantic_min_alpha &&
+                          best_gaussian >= 0 && best_gaussian < args.gaussian_count;
+    args.semantic[pixel] = labelled ? args.semantic_ids[best_gaussian] : -1;
+  }
+}
+
 // Minimum waves per SIMD for 16x16 tiles (launch-bounds occupancy target): eight
 // means at most 64 VGPRs.  The blend loop is latency-bound, so waves in flight
 // pay more than registers.  The evaluation-counting and persistent variants
@@ -561,6 +660,29 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const v
     return VGR_E_INVALID_ARGUMENT;
   }
   const bool full = options.full_sensor_output != 0;
+  if (options.tile_size < 8) {
+    const int64_t threads = blocks * options.tile_size * options.tile_size;
+    const dim3 grid(static_cast<unsigned>((threads + 255) / 256)), block(256);
+    const bool count = options.statistics != nullptr;
+    if (full && count) {
+      hipLaunchKernelGGL((composite_pixels_kernel<true, true>), grid, block, 0, stream, args, options.tile_size, threads);
+    } else if (full) {
+      hipLaunchKernelGGL((composite_pixels_kernel<true, false>), grid, block, 0, stream, args, options.tile_size,
+                         threads);
+    } else if (count) {
+      hipLaunchKernelGGL((composite_pixels_kernel<false, true>), grid, block, 0, stream, args, options.tile_size,
+                         threads);
+    } else {
+      hipLaunchKernelGGL((composite_pixels_kernel<false, false>), grid, block, 0, stream, args, options.tile_size,
+                         threads);
+    }
+    const hipError_t error = hipGetLastError();
+    if (error != hipSuccess) {
+      set_error("composite launch failed: %s", hipGetErrorString(error));
+      return VGR_E_LAUNCH;
+    }
+    return 0;
+  }
   args.total_items = blocks;
   args.work_counter = reinterpret_cast<uint32_t*>(const_cast<char*>(scratch) + layout.work_counter);
   // Longest tiles first pays when the grid is a few waves deep per SIMD; on deep

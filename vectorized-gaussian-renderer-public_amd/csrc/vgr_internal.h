@@ -68,6 +68,13 @@ struct ScratchLayout {
   size_t front_block_visible;  // int64[projection_blocks]: visible pairs of each workgroup
   size_t front_open_state;   // uint32[4]: open-tile count, fallback-sort and recomposite queues
   size_t front_open_tiles;   // int32[global_tiles]: tiles to sort in full and composite again
+  // Dense (tile size 1) depth cutoff only (zero-sized otherwise).
+  size_t dense_camera_map;   // int32[batch]: camera -> active index
+  size_t dense_cutoff;       // int32[pixels]: last depth bucket emitted per pixel (-1 while open)
+  size_t dense_accumulated;  // uint64[pixels]: running fixed-point optical thickness
+  size_t dense_tau;          // uint32[pixels * group]: per-bucket thickness of the current group
+  int64_t dense_pixels;
+  int32_t dense_group_size;
   size_t total;
   bool culling;
   bool front;
@@ -135,6 +142,8 @@ int launch_front_fallback_sort(const vgr_scene& scene, const vgr_workspace& ws, 
 void scan_pairs(const int64_t* counts, int64_t n, int64_t* bases, int64_t* partials, int64_t visible_capacity,
                 int64_t intersection_capacity, int64_t* counters, int64_t* totals, hipStream_t stream);
 int launch_fill_i64(int64_t* data, int64_t n, int64_t value, hipStream_t stream);
+int launch_dense_cutoff(const vgr_cameras& cameras, const vgr_options& options, const vgr_workspace& ws,
+                        const ScratchLayout& layout, char* scratch, hipStream_t stream);
 int launch_pose_compose(const float* twists, const float* base, int64_t count, float* views, hipStream_t stream);
 int launch_pose_compose_backward(const float* twists, const float* base, const float* grad_views, int64_t count,
                                  float* grad_twists, float* grad_base, hipStream_t stream);

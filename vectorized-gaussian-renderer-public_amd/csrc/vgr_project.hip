@@ -151,6 +151,7 @@ __global__ void build_camera_table_kernel(
 }
 
 
+template <bool kDense>
 __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(ProjectionArgs args,
                                                                          int64_t* __restrict__ block_counts,
                                                                          uint64_t* __restrict__ wave_masks) {
@@ -181,7 +182,8 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
     Projected p;
     TileRect rect;
     int tiles;
-    const bool seen = evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    const bool seen = kDense ? evaluate_dense_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles)
+                             : evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
     if (seen) {
       visible += 1;
       intersections += tiles;
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
   }
 }
 
-template <int kDegree>
+template <int kDegree, bool kDense>
 __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
     ProjectionArgs args, const int64_t* __restrict__ block_bases, const uint64_t* __restrict__ wave_masks,
     vgr_record* __restrict__ records, int32_t* __restrict__ radii, uint64_t* __restrict__ keys,
@@ -237,13 +239,15 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
     Projected p;
     TileRect rect;
     int tiles;
-    const bool visible = evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    const bool visible = kDense
+                             ? evaluate_dense_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles)
+                             : evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
     int inclusive_visible = visible ? 1 : 0;
     int inclusive_tiles = tiles;
     wave_inclusive_scan2(inclusive_visible, inclusive_tiles);
     const int total_visible = __shfl(inclusive_visible, kWave - 1, kWave);
     const int total_tiles = __shfl(inclusive_tiles, kWave - 1, kWave);
-    if (visible) {
+    if (visible && !(kDense && args.dense_mode == 2)) {  // the emission-only pass keeps the records
       const int64_t record = visible_base + inclusive_visible - 1;
       const bool stored = record < visible_capacity;
       if (stored) {
@@ -258,6 +262,24 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
           radii[record * 2 + 1] = p.radius_y;
         }
       }
+    }
+    if constexpr (kDense) {
+      // Pixel intersections: each lane writes its own pixels from its scanned offset.
+      if (visible && tiles > 0) {
+        const int64_t record = visible_base + inclusive_visible - 1;
+        const int32_t value = record < visible_capacity ? static_cast<int32_t>(record) : -1;
+        const uint32_t depth_bits = vgr_float_bits(p.depth);
+        const int64_t first_slot = intersection_base + inclusive_tiles - tiles;
+        dense_pixels(args, cam, p, rect, [&](int k, uint32_t pixel) {
+          const int64_t slot = first_slot + k;
+          if (slot >= intersection_capacity) return;
+          keys[slot] = value >= 0 ? ((static_cast<uint64_t>(pixel) << 32) | depth_bits) : kSentinelKey;
+          values[slot] = value;
+        });
+      }
+      visible_base += total_visible;
+      intersection_base += total_tiles;
+      continue;
     }
     // Tile intersections of the wave: the wave writes its total_tiles slots as
     // contiguous 64-slot runs (coalesced stores).  Output slot j belongs to the
@@ -672,6 +694,9 @@ void make_projection_args(const vgr_scene& scene, const vgr_cameras& cameras, co
   args.feature_width = scene.feature_width;
   args.sh_degree = scene.sh_degree;
   args.tile_size = options.tile_size;
+  args.dense_mode = 0;
+  args.depth_buckets = 0;
+  args.depth_cutoff = nullptr;
   args.antialiased = options.antialiased;
   args.near_plane = options.near_plane;
   args.far_plane = options.far_plane;
@@ -713,6 +738,47 @@ int launch_cull(const ProjectionArgs& args, const vgr_scene& scene, const Scratc
   return 0;
 }
 
+namespace {
+
+// One count -> scan -> write pass over the projection grid (records and/or intersections).
+template <bool kDense>
+void projection_pass(const ProjectionArgs& args, const vgr_workspace& ws, int64_t* block_counts, int64_t* block_bases,
+                     uint64_t* wave_masks, int64_t* partials, int64_t blocks, hipStream_t stream) {
+  const int64_t waves = blocks * kProjectionWaves;
+  if (blocks > 0) {
+    hipLaunchKernelGGL(projection_count_kernel<kDense>, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0,
+                       stream, args, block_counts, wave_masks);
+  }
+  scan_pairs(block_counts, waves, block_bases, partials, ws.visible_capacity, ws.intersection_capacity,
+             ws.counters, nullptr, stream);
+  if (blocks <= 0) return;
+  const dim3 grid(static_cast<unsigned>(blocks)), threads(kBlockThreads);
+  switch (args.sh_degree) {
+    case 1:
+      hipLaunchKernelGGL((projection_write_kernel<1, kDense>), grid, threads, 0, stream, args, block_bases, wave_masks,
+                         ws.records, ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
+                         ws.intersection_capacity);
+      break;
+    case 2:
+      hipLaunchKernelGGL((projection_write_kernel<2, kDense>), grid, threads, 0, stream, args, block_bases, wave_masks,
+                         ws.records, ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
+                         ws.intersection_capacity);
+      break;
+    case 3:
+      hipLaunchKernelGGL((projection_write_kernel<3, kDense>), grid, threads, 0, stream, args, block_bases, wave_masks,
+                         ws.records, ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
+                         ws.intersection_capacity);
+      break;
+    default:
+      hipLaunchKernelGGL((projection_write_kernel<0, kDense>), grid, threads, 0, stream, args, block_bases, wave_masks,
+                         ws.records, ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
+                         ws.intersection_capacity);
+      break;
+  }
+}
+
+}  // namespace
+
 int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const vgr_options& options,
                       const vgr_workspace& ws, const ScratchLayout& layout, char* scratch, hipStream_t stream) {
   ProjectionArgs args;
@@ -738,40 +804,23 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
     args.item_count = reinterpret_cast<const int64_t*>(scratch + layout.item_totals);
   }
 
-  if (blocks > 0) {
-    hipLaunchKernelGGL(projection_count_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0,
-                       stream, args, block_counts, wave_masks);
-  }
-  scan_pairs(block_counts, waves, block_bases, partials, ws.visible_capacity, ws.intersection_capacity,
-             ws.counters, nullptr, stream);
-  if (blocks > 0) {
-    const dim3 grid(static_cast<unsigned>(blocks)), threads(kBlockThreads);
-    switch (args.sh_degree) {
-      case 1:
-        hipLaunchKernelGGL(projection_write_kernel<1>, grid, threads, 0, stream, args, block_bases, wave_masks,
-                           ws.records,
-                           ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
-                           ws.intersection_capacity);
-        break;
-      case 2:
-        hipLaunchKernelGGL(projection_write_kernel<2>, grid, threads, 0, stream, args, block_bases, wave_masks,
-                           ws.records,
-                           ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
-                           ws.intersection_capacity);
-        break;
-      case 3:
-        hipLaunchKernelGGL(projection_write_kernel<3>, grid, threads, 0, stream, args, block_bases, wave_masks,
-                           ws.records,
-                           ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
-                           ws.intersection_capacity);
-        break;
-      default:
-        hipLaunchKernelGGL(projection_write_kernel<0>, grid, threads, 0, stream, args, block_bases, wave_masks,
-                           ws.records,
-                           ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
-                           ws.intersection_capacity);
-        break;
-    }
+  if (options.tile_size != 1) {
+    projection_pass<false>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
+  } else if (options.depth_bucket_count <= 0) {
+    args.dense_mode = 0;  // records and pixel intersections in one pass
+    projection_pass<true>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
+  } else {
+    // Records first, then the per-pixel depth cutoff from them, then the pixel
+    // intersections that survive it (the second pass re-evaluates the same pairs,
+    // so its scanned record indices are those of the first).
+    args.dense_mode = 1;
+    projection_pass<true>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
+    const int status = launch_dense_cutoff(cameras, options, ws, layout, scratch, stream);
+    if (status != 0) return status;
+    args.dense_mode = 2;
+    args.depth_buckets = options.depth_bucket_count;
+    args.depth_cutoff = reinterpret_cast<const int32_t*>(scratch + layout.dense_cutoff);
+    projection_pass<true>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
   }
   const hipError_t status = hipGetLastError();
   if (status != hipSuccess) {

@@ -237,7 +237,52 @@ struct ProjectionArgs {
   const int32_t* item_blocks;    // kept (group, chunk) blocks in order, or null (every block)
   const uint64_t* item_masks;    // cameras of the group that may see the chunk
   const int64_t* item_count;     // device count of kept blocks
+  // Dense (tile size 1) passes: 0 records and pixel intersections together, 1
+  // records only (the cutoff pass needs them first), 2 pixel intersections only.
+  int dense_mode;
+  int depth_buckets;             // depth buckets of the cutoff (depth_cutoff non-null)
+  const int32_t* depth_cutoff;   // per global pixel: last depth bucket emitted, or null (no cutoff)
 };
+
+// ---- dense (tile size 1) per-pixel decisions --------------------------------------
+
+constexpr float kDenseMinPower = -20.0f;
+// Fixed-point scale of the optical-thickness sums and the cutoff threshold
+// ceil(9.210340371976184 * 65536) (transmittance 1e-4, renderer_cuda.cu:1672).
+constexpr float kTauScale = 65536.0f;
+constexpr uint64_t kTauThresholdFixed = 603610ull;
+
+// Depth bucket of a splat: uniform buckets over [near, far] (renderer_cuda.cu:168-177).
+__device__ __forceinline__ int depth_bucket(float depth, float near_plane, float far_plane, int buckets) {
+  const float normalized = (depth - near_plane) / (far_plane - near_plane);
+  const int index = static_cast<int>(normalized * static_cast<float>(buckets));
+  return min(buckets - 1, max(0, index));
+}
+
+// Alpha of a splat at the centre of pixel (x, y), 0 below 1/255
+// (renderer_cuda.cu:179-200, with vgr_expf for the exponential).
+__device__ __forceinline__ float pixel_alpha(const Projected& p, int x, int y) {
+  const float dx = (static_cast<float>(x) + 0.5f) - p.mean_x;
+  const float dy = (static_cast<float>(y) + 0.5f) - p.mean_y;
+  const float power = -0.5f * (p.conic_0 * dx * dx + 2.0f * p.conic_1 * dx * dy + p.conic_2 * dy * dy);
+  if (power > 0.0f || power < kDenseMinPower) return 0.0f;
+  const float alpha = fminf(VGR_MAX_ALPHA, p.opacity * vgr_expf(power));
+  return alpha >= VGR_ALPHA_THRESHOLD ? alpha : 0.0f;
+}
+
+// Conservative optical thickness of one splat: 0.999 x the first eight terms of
+// -log(1 - alpha) = sum alpha^k / k (renderer_cuda.cu:247-257), in 1/65536 units
+// rounded down.
+__device__ __forceinline__ uint32_t optical_thickness_fixed(float alpha) {
+  float term = alpha;
+  float result = term;
+#pragma unroll
+  for (int order = 2; order <= 8; ++order) {
+    term *= alpha;
+    result += term / static_cast<float>(order);
+  }
+  return static_cast<uint32_t>((result * 0.999f) * kTauScale);
+}
 
 // Resolves the (group, chunk) block and camera mask processed by this workgroup.
 // Returns false for workgroups beyond the kept item count.
@@ -263,6 +308,38 @@ __device__ __forceinline__ bool evaluate_pair(const ProjectionArgs& args, const 
   }
   rect = tile_rect(p, args.tile_size, cam.tiles_x, cam.tiles_y);
   tiles = rect_tiles(rect);
+  return true;
+}
+
+// Pixel intersections of a visible pair at tile size 1: the pixels of its rectangle
+// where its alpha reaches 1/255 and (with a cutoff) whose cutoff bucket is not in
+// front of the pair's depth bucket (renderer_cuda.cu:1764-1805).  `emit(slot, pixel)`
+// receives them in row-major order.
+template <typename Emit>
+__device__ __forceinline__ int dense_pixels(const ProjectionArgs& args, const CameraParams& cam, const Projected& p,
+                                            const TileRect& rect, Emit&& emit) {
+  const int bucket =
+      args.depth_cutoff != nullptr ? depth_bucket(p.depth, args.near_plane, args.far_plane, args.depth_buckets) : 0;
+  int count = 0;
+  for (int y = rect.min_y; y < rect.max_y; ++y) {
+    for (int x = rect.min_x; x < rect.max_x; ++x) {
+      if (pixel_alpha(p, x, y) == 0.0f) continue;
+      const uint32_t pixel = cam.tile_base + static_cast<uint32_t>(y * cam.tiles_x + x);
+      if (args.depth_cutoff != nullptr && bucket > args.depth_cutoff[pixel]) continue;
+      emit(count, pixel);
+      ++count;
+    }
+  }
+  return count;
+}
+
+// evaluate_pair for the dense passes: the pair's intersections are its emitted
+// pixels (none in the records-only pass).
+__device__ __forceinline__ bool evaluate_dense_pair(const ProjectionArgs& args, const CameraParams& cam, int64_t g,
+                                                    bool loaded, float m0, float m1, float m2, float opacity,
+                                                    const float* cov, Projected& p, TileRect& rect, int& tiles) {
+  if (!evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles)) return false;
+  tiles = args.dense_mode == 1 ? 0 : dense_pixels(args, cam, p, rect, [](int, uint32_t) {});
   return true;
 }
 

@@ -49,6 +49,7 @@ from .native import (
     RECORD_FLOATS,
     SORT_FAILURE_SLOT,
     WORKSPACE_TENSORS,
+    dense_cutoff,
     float_bits,
     infer_sh_degree,
     load_native_module,
@@ -290,10 +291,13 @@ class CustomHipBackend:
         if query is None:  # a native module with only the reference's surface
             return int(self._native.sort_temp_bytes(intersections, self._global_tiles()))
         gaussians = int(self._packed.means.shape[0]) if self._packed is not None else 0
+        buckets, group = dense_cutoff(o.tile_size, o.depth_bucket_count, o.depth_bucket_group_size,
+                                      o.compact_projection_cache, o.deterministic)
         return int(query(max_views=layout.physical_views, height=layout.height, width=layout.width,
                          gaussians=gaussians, intersection_capacity=intersections, tile_size=o.tile_size,
                          near_plane=o.near_plane, far_plane=o.far_plane, sort_digit_bits=o.sort_digit_bits,
-                         cameras_per_block=o.cameras_per_block, sort_tile_items=o.sort_tile_items))
+                         cameras_per_block=o.cameras_per_block, sort_tile_items=o.sort_tile_items,
+                         depth_bucket_count=buckets, depth_bucket_group_size=group))
 
     def _check_native_limits(self, capacity: Capacity) -> None:
         if capacity.visible_records <= 0 or capacity.tile_intersections <= 0:
@@ -616,7 +620,8 @@ class CustomHipBackend:
             self._packed.native_list(), cameras, outputs, workspace, self._layout.height, self._layout.width,
             self._packed.max_scene_gaussians, o.near_plane, o.far_plane, o.gaussian_support_sigma,
             o.covariance_epsilon, o.rasterize_mode == "antialiased", o.semantic_min_alpha, False, o.tile_size,
-            o.depth_bucket_count, min(o.depth_bucket_group_size, o.depth_bucket_count), False, False, reuse,
+            o.depth_bucket_count, min(o.depth_bucket_group_size, o.depth_bucket_count), o.compact_projection_cache,
+            o.materialize_projected_records, reuse,
             o.output_srgb, o.deterministic, full, o.fixed_capacity_sort, sh_degree=self._sh_degree,
             sort_digit_bits=o.sort_digit_bits, raster_batch=o.raster_batch, cameras_per_block=o.cameras_per_block,
             sort_tile_items=o.sort_tile_items, **extra))

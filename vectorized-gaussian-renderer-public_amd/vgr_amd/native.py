@@ -45,7 +45,7 @@ RECORD_FLOATS = 12  # 48-byte visible record, see vgr_record in include/vgr.h
 RECORD_GRAD_FLOATS = 10  # dL/d (mean_x, mean_y, conic_0..2, opacity, depth, r, g, b) per intersection slot
 CAMERA_GRAD_FLOATS = 24  # per-camera pose partials of the backward pass
 POSE_REPLICAS = 32  # copies of the pose partials (VGR_POSE_REPLICAS)
-ABI_VERSION = 5
+ABI_VERSION = 6
 # Device counter block: the reference's five counters (cuda_backend.py:22-28) plus
 # one slot counting radix-sort look-back waits that gave up (the order is then wrong).
 COUNTER_SLOTS = 6
@@ -171,7 +171,23 @@ class _Options(ctypes.Structure):
         ("chunk_culling", ctypes.c_int32),
         ("lookback_spin_limit", ctypes.c_int32),
         ("ordering", ctypes.c_int32),
+        ("depth_bucket_count", ctypes.c_int32),
+        ("depth_bucket_group_size", ctypes.c_int32),
     ]
+
+
+def dense_cutoff(tile_size: int, depth_bucket_count: int, depth_bucket_group_size: int,
+                 compact_projection_cache: bool, deterministic: bool) -> tuple[int, int]:
+    """(buckets, buckets per pass) of the depth cutoff for the reference's knobs, (0, 0) for none.
+
+    The reference runs the cutoff at tile size 1 in nondeterministic mode, over
+    ``depth_bucket_group_size`` buckets per pass, or over every bucket in one pass with
+    ``compact_projection_cache`` (renderer_cuda.cu:2888-3144).  Images do not depend on it.
+    """
+    if tile_size != 1 or depth_bucket_count <= 0 or (deterministic and not compact_projection_cache):
+        return 0, 0
+    group = depth_bucket_count if compact_projection_cache else min(depth_bucket_group_size, depth_bucket_count)
+    return int(depth_bucket_count), int(group)
 
 
 class _Sizes(ctypes.Structure):
@@ -308,8 +324,10 @@ class NativeRenderer:
                       tile_size: int, near_plane: float, far_plane: float, sort_digit_bits: int = 0,
                       cameras_per_block: int = 0, sort_tile_items: int = 0, chunk_culling: bool = False,
                       resolutions: Sequence[tuple[int, int]] | None = None,
-                      ordering: int = ORDERING_GLOBAL_RADIX) -> int:
-        """Scratch bytes one render of up to ``max_views`` cameras needs."""
+                      ordering: int = ORDERING_GLOBAL_RADIX, depth_bucket_count: int = 0,
+                      depth_bucket_group_size: int = 0) -> int:
+        """Scratch bytes one render of up to ``max_views`` cameras needs (``depth_bucket_count`` /
+        ``depth_bucket_group_size``: the effective cutoff of :func:`dense_cutoff`)."""
         cameras = _Cameras()
         cameras.batch = max_views
         cameras.active_count = max_views
@@ -324,7 +342,8 @@ class NativeRenderer:
                                 gaussian_support_sigma=3.0, covariance_epsilon=0.0, tile_size=tile_size,
                                 sort_digit_bits=sort_digit_bits, cameras_per_block=cameras_per_block,
                                 sort_tile_items=sort_tile_items, chunk_culling=int(bool(chunk_culling)),
-                                ordering=int(ordering))
+                                ordering=int(ordering), depth_bucket_count=int(depth_bucket_count),
+                                depth_bucket_group_size=int(depth_bucket_group_size))
         sizes = _Sizes()
         self._check(self.lib.vgr_workspace_sizes_query(ctypes.byref(cameras), ctypes.byref(options), int(gaussians),
                                                         int(intersection_capacity), ctypes.byref(sizes)),
@@ -434,8 +453,13 @@ class NativeRenderer:
         """
         if ray_gaussian_evaluation:
             raise ValueError("ray_gaussian_evaluation is not available on the gfx950 tile path.")
-        if compact_projection_cache or materialize_projected_records:
-            raise ValueError("The dense 1x1 compact projection path is not available on the gfx950 build.")
+        if (compact_projection_cache or materialize_projected_records) and int(tile_size) != 1:
+            raise ValueError("compact_projection_cache requires tile_size=1.")
+        # Records are always materialized here (48-byte records written by the projection
+        # pass), so materialize_projected_records selects nothing further.
+        cutoff_buckets, cutoff_group = dense_cutoff(int(tile_size), int(depth_bucket_count),
+                                                    int(depth_bucket_group_size), bool(compact_projection_cache),
+                                                    bool(deterministic))
         if len(scene) != 7 or len(cameras) != 5 or len(outputs) != 4 or len(workspace) != len(WORKSPACE_TENSORS):
             raise ValueError("render expects 7 scene, 5 camera, 4 output and "
                              f"{len(WORKSPACE_TENSORS)} workspace tensors.")
@@ -514,6 +538,13 @@ class NativeRenderer:
             tile_ends=_ptr(tile_ends), counters=_ptr(counters), scratch=_ptr(scratch),
             visible_capacity=int(records.shape[0]), intersection_capacity=int(keys_in.numel()),
             tile_capacity=int(tile_starts.numel()), scratch_bytes=int(scratch.numel()))
+        if cutoff_buckets > 0:
+            # The cutoff reads the records' radii between the records pass and the
+            # emission pass; keys_out is idle until the sort, so it holds them.
+            if keys_out.numel() < int(records.shape[0]):
+                raise ValueError("the depth cutoff keeps the records' radii in keys_out: the intersection "
+                                 "capacity must be at least the visible capacity")
+            c_workspace.radii = keys_out.data_ptr()
         if slot_records is not None:
             if (slot_records.dtype != torch.int32 or slot_records.device != device
                     or slot_records.numel() < keys_in.numel() or not slot_records.is_contiguous()):
@@ -530,7 +561,7 @@ class NativeRenderer:
             cameras_per_block=int(cameras_per_block), reuse_projection=int(bool(reuse_projection)),
             sort_tile_items=int(sort_tile_items), persistent_composite=int(persistent_composite),
             chunk_culling=int(chunk_bounds is not None), lookback_spin_limit=int(lookback_spin_limit),
-            ordering=int(ordering))
+            ordering=int(ordering), depth_bucket_count=cutoff_buckets, depth_bucket_group_size=cutoff_group)
         event_array = None
         if stage_events is not None:
             if len(stage_events) != 5:

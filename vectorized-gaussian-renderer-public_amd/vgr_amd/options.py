@@ -91,13 +91,15 @@ _RULES: tuple[tuple[Callable[[BackendOptions], bool], str], ...] = (
      "semantic_min_alpha must be finite and in [0, 1]."),
     (lambda o: not (1 <= o.tile_size <= 16 and o.tile_size & (o.tile_size - 1) == 0),
      "tile_size must be a power of two in [1, 16]."),
-    (lambda o: o.tile_size not in (8, 16),
-     "tile_size must be 8 or 16 on the gfx950 tile path (the dense 1x1 path is not built)."),
     (lambda o: o.depth_bucket_count <= 0, "depth_bucket_count must be positive."),
     (lambda o: o.depth_bucket_group_size <= 0, "depth_bucket_group_size must be positive."),
-    (lambda o: o.compact_projection_cache or o.materialize_projected_records,
-     "compact_projection_cache / materialize_projected_records (the dense 1x1 path) are not available in the "
-     "gfx950 build."),
+    (lambda o: o.compact_projection_cache and o.tile_size != 1, "compact_projection_cache requires tile_size=1."),
+    (lambda o: o.compact_projection_cache and o.ray_gaussian_evaluation,
+     "compact_projection_cache currently supports only the screen-space Gaussian model."),
+    (lambda o: o.compact_projection_cache and o.deterministic,
+     "compact_projection_cache currently requires deterministic=False."),
+    (lambda o: o.materialize_projected_records and not o.compact_projection_cache,
+     "materialize_projected_records requires compact_projection_cache=True."),
     (lambda o: o.ray_gaussian_evaluation, "ray_gaussian_evaluation is not available in the gfx950 build."),
     (lambda o: o.fixed_capacity_sort and o.deterministic,
      "fixed_capacity_sort applies to the nondeterministic global-radix path only; it cannot be combined with "
