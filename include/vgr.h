@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define VGR_ABI_VERSION 6
+#define VGR_ABI_VERSION 7
 
 enum vgr_status {
   VGR_OK = 0,
@@ -154,6 +154,11 @@ typedef struct vgr_workspace {
    * (record, tile) pair's gradient terms in its own row without atomics.  No
    * reference counterpart (the reference is forward-only). */
   int32_t* slot_records;
+  /* Exact-ray renders (options.ray_evaluation) only, else NULL: [visible_capacity, 12]
+   * per visible record, the camera-space precision matrix (6 floats: 00, 01, 02,
+   * 11, 12, 22), precision x mean (3), mean^T precision mean (1) and 2 pad floats
+   * (renderer_cuda.cu:795-860, kVisibleRayPrecisions / kVisibleRayPrecisionMeans). */
+  float* ray_records;
 } vgr_workspace;
 
 /* Render options (reference scalar arguments, renderer.cpp:214-238). */
@@ -209,6 +214,17 @@ typedef struct vgr_options {
    * with and without the cutoff; only the intersection count shrinks. */
   int32_t depth_bucket_count;       /* 0 = no cutoff; else the bucket count (reference default 4096) */
   int32_t depth_bucket_group_size;  /* buckets per cutoff pass: 0 = default (64) */
+  /* Exact-ray evaluation (reference ray_gaussian_evaluation, renderer_cuda.cu:203-245,
+   * 795-948, 2180-2262): a splat's alpha at a pixel is opacity x exp(-m^2 / 2) with m
+   * the Mahalanobis distance of the camera-space Gaussian's closest point on the
+   * pixel's ray (0 beyond gaussian_support_sigma, capped at 1, no 1/255 floor), and
+   * compositing stops after the entry that takes transmittance to 1e-4.  Tiles are
+   * binned over the screen ellipse of the support quadric; with 16x16 raster tiles the
+   * bins are 32x32 (four raster tiles share one sorted list; tile counts, ranges and
+   * workspace tile capacities are in bins).  Needs 8x8 or 16x16 tiles, uniform
+   * resolutions, workspace.ray_records, no antialiasing, the global radix ordering and
+   * no training checkpoints. */
+  int32_t ray_evaluation;
 } vgr_options;
 
 /* Intersection ordering of a render.

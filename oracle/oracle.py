@@ -39,7 +39,8 @@ class _Options(ctypes.Structure):
                 ("covariance_epsilon", ctypes.c_float), ("semantic_min_alpha", ctypes.c_float),
                 ("antialiased", ctypes.c_int32), ("tile_size", ctypes.c_int32), ("output_srgb", ctypes.c_int32),
                 ("full_sensor_output", ctypes.c_int32), ("threads", ctypes.c_int32),
-                ("depth_bucket_count", ctypes.c_int32), ("depth_bucket_group_size", ctypes.c_int32)]
+                ("depth_bucket_count", ctypes.c_int32), ("depth_bucket_group_size", ctypes.c_int32),
+                ("ray_evaluation", ctypes.c_int32)]
 
 
 class _Outputs(ctypes.Structure):
@@ -148,6 +149,7 @@ def render(
     background: bool = True,
     depth_bucket_count: int = 0,
     depth_bucket_group_size: int = 0,
+    ray_evaluation: bool = False,
 ) -> dict[str, np.ndarray]:
     """Render a batch on the CPU.
 
@@ -178,9 +180,10 @@ def render(
     c_scene = _Scene(_p(means), _p(cov), _p(opacities), _p(features), _p(semantic_ids), _p(registered), _p(offsets),
                      means.shape[0], features.shape[1], len(scenes), sh_degree, 0)
     widths = heights = pixel_offsets = None
+    bin_size = 32 if ray_evaluation and tile_size == 16 else tile_size
     if resolutions is None:
         pixels = batch * height * width
-        tiles = batch * ((width + tile_size - 1) // tile_size) * ((height + tile_size - 1) // tile_size)
+        tiles = batch * ((width + bin_size - 1) // bin_size) * ((height + bin_size - 1) // bin_size)
         shape = (batch, height, width)
     else:
         widths = np.asarray([r[0] for r in resolutions], dtype=np.int32)
@@ -195,7 +198,7 @@ def render(
                       0 if active is None else active.shape[0], _p(widths), _p(heights), _p(pixel_offsets))
     c_opt = _Options(height, width, near_plane, far_plane, gaussian_support_sigma, covariance_epsilon,
                      semantic_min_alpha, int(antialiased), tile_size, int(output_srgb), int(full_sensor_output),
-                     threads, int(depth_bucket_count), int(depth_bucket_group_size))
+                     threads, int(depth_bucket_count), int(depth_bucket_group_size), int(bool(ray_evaluation)))
     rgb = np.zeros(shape + (3,), dtype=np.float32)
     depth = np.full(shape + (1,), np.inf if background else 0.0, dtype=np.float32)
     alpha = np.zeros(shape + (1,), dtype=np.float32)

@@ -243,6 +243,10 @@ typedef struct projected {
   int radius_x, radius_y;
   int32_t gaussian;
   int visible;
+  /* Exact-ray data: camera-space precision (00, 01, 02, 11, 12, 22), P mu, mu^T P mu. */
+  float precision[6];
+  float precision_mean[3];
+  float mean_precision_mean;
 } projected;
 
 static int clamp_to_int(float value) {
@@ -309,23 +313,96 @@ static int project_one(const camera_setup* c, const float* mean, float opacity, 
   const int radius_x = clamp_to_int(ceilf(extend * sqrtf(cov00)));
   const int radius_y = clamp_to_int(ceilf(extend * sqrtf(cov11)));
   if (radius_x <= 0 && radius_y <= 0) return 0;
+  float cull_mean_x = mean_x, cull_mean_y = mean_y;
+  int cull_radius_x = radius_x, cull_radius_y = radius_y;
+  if (opt->ray_evaluation) {
+    /* Exact-ray extent, :795-948: precision in double, the screen ellipse of the
+     * rays whose closest approach lies within the support sigma. */
+    const double d00 = c00, d01 = c01, d02 = c02, d11 = c11, d12 = c12, d22 = c22;
+    const double det3 = d00 * (d11 * d22 - d12 * d12) - d01 * (d01 * d22 - d02 * d12) + d02 * (d01 * d12 - d02 * d11);
+    if (!(det3 > 1.0e-48)) return 0;
+    const double inv3 = 1.0 / det3;
+    const double p00 = (d11 * d22 - d12 * d12) * inv3;
+    const double p01 = (d02 * d12 - d01 * d22) * inv3;
+    const double p02 = (d01 * d12 - d02 * d11) * inv3;
+    const double p11 = (d00 * d22 - d02 * d02) * inv3;
+    const double p12 = (d01 * d02 - d00 * d12) * inv3;
+    const double p22 = (d00 * d11 - d01 * d01) * inv3;
+    const double x = cam_x, y = cam_y, z = cam_z;
+    const double pm0 = p00 * x + p01 * y + p02 * z;
+    const double pm1 = p01 * x + p11 * y + p12 * z;
+    const double pm2 = p02 * x + p12 * y + p22 * z;
+    const double mpm = x * pm0 + y * pm1 + z * pm2;
+    p->precision[0] = (float)p00;
+    p->precision[1] = (float)p01;
+    p->precision[2] = (float)p02;
+    p->precision[3] = (float)p11;
+    p->precision[4] = (float)p12;
+    p->precision[5] = (float)p22;
+    p->precision_mean[0] = (float)pm0;
+    p->precision_mean[1] = (float)pm1;
+    p->precision_mean[2] = (float)pm2;
+    p->mean_precision_mean = (float)mpm;
+    const double support2 = (double)opt->gaussian_support_sigma * (double)opt->gaussian_support_sigma;
+    const double q = mpm - support2;
+    const double k00 = q * p00 - pm0 * pm0, k01 = q * p01 - pm0 * pm1, k02 = q * p02 - pm0 * pm2;
+    const double k11 = q * p11 - pm1 * pm1, k12 = q * p12 - pm1 * pm2, k22 = q * p22 - pm2 * pm2;
+    const double k_det = k00 * k11 - k01 * k01;
+    int finite = 0;
+    if (k00 > 0.0 && k_det > 0.0) {
+      const double inv_k = 1.0 / k_det;
+      const double centre_x = (k01 * k12 - k11 * k02) * inv_k;
+      const double centre_y = (k01 * k02 - k00 * k12) * inv_k;
+      const double scale = -(k22 + k02 * centre_x + k12 * centre_y);
+      if (scale > 0.0) {
+        const double extent_x = sqrt(scale * k11 * inv_k);
+        const double extent_y = sqrt(scale * k00 * inv_k);
+        const double mx = (double)c->fx * centre_x + (double)c->cx;
+        const double my = (double)c->fy * centre_y + (double)c->cy;
+        cull_mean_x = (float)mx;
+        cull_mean_y = (float)my;
+        const int rx = clamp_to_int(ceilf((float)(fabs((double)c->fx) * extent_x)));
+        const int ry = clamp_to_int(ceilf((float)(fabs((double)c->fy) * extent_y)));
+        cull_radius_x = rx > 1 ? rx : 1;
+        cull_radius_y = ry > 1 ? ry : 1;
+        finite = isfinite(mx) && isfinite(my) && isfinite(extent_x) && isfinite(extent_y);
+      }
+    }
+    if (!finite) {
+      cull_mean_x = 0.5f * (float)c->width;
+      cull_mean_y = 0.5f * (float)c->height;
+      cull_radius_x = cull_radius_y = c->width > c->height ? c->width : c->height;
+    }
+  }
   /* :950-956 */
-  if (mean_x + (float)radius_x <= 0.0f || mean_y + (float)radius_y <= 0.0f ||
-      mean_x - (float)radius_x >= (float)c->width || mean_y - (float)radius_y >= (float)c->height) {
+  if (cull_mean_x + (float)cull_radius_x <= 0.0f || cull_mean_y + (float)cull_radius_y <= 0.0f ||
+      cull_mean_x - (float)cull_radius_x >= (float)c->width || cull_mean_y - (float)cull_radius_y >= (float)c->height) {
     return 0;
   }
   /* :958, 1041-1051 */
   const float inverse_det = 1.0f / det;
-  p->mean_x = mean_x;
-  p->mean_y = mean_y;
+  p->mean_x = cull_mean_x;
+  p->mean_y = cull_mean_y;
   p->conic_0 = cov11 * inverse_det;
   p->conic_1 = -cov01 * inverse_det;
   p->conic_2 = cov00 * inverse_det;
   p->depth = cam_z;
   p->opacity = fminf(fmaxf(opacity, 0.0f), 1.0f);
-  p->radius_x = radius_x;
-  p->radius_y = radius_y;
+  p->radius_x = cull_radius_x;
+  p->radius_y = cull_radius_y;
   return 1;
+}
+
+/* splat_alpha_at_ray, renderer_cuda.cu:203-245 (vgr_expf for __expf). */
+static float splat_alpha_at_ray(const projected* p, float ray_x, float ray_y, float support_sigma) {
+  const float* P = p->precision;
+  const float along = P[0] * ray_x * ray_x + 2.0f * P[1] * ray_x * ray_y + 2.0f * P[2] * ray_x + P[3] * ray_y * ray_y +
+                      2.0f * P[4] * ray_y + P[5];
+  if (!(along > 0.0f)) return 0.0f;
+  const float cross = ray_x * p->precision_mean[0] + ray_y * p->precision_mean[1] + p->precision_mean[2];
+  const float distance2 = fmaxf(0.0f, p->mean_precision_mean - cross * cross / along);
+  if (distance2 > support_sigma * support_sigma) return 0.0f;
+  return fminf(1.0f, p->opacity * vgr_expf(-0.5f * distance2));
 }
 
 /* SH colour: base RGB + real SH bands 1..degree at the unit view direction
@@ -580,16 +657,26 @@ static void composite_tiles(void* context, int64_t begin, int64_t end) {
         float transmittance = 1.0f, red = 0.0f, green = 0.0f, blue = 0.0f, depth_sum = 0.0f;
         float best_weight = -1.0f;
         int32_t best = -1;
+        const float ray_x = (fx - c->cx) / c->fx;
+        const float ray_y = (fy - c->cy) / c->fy;
         for (int64_t i = first; i < last; ++i) {
           const projected* p = &job->proj[job->records[job->items[i].record]];
-          const float dx = fx - p->mean_x;
-          const float dy = fy - p->mean_y;
-          const float power = -0.5f * (p->conic_0 * dx * dx + 2.0f * p->conic_1 * dx * dy + p->conic_2 * dy * dy);
-          if (power > 0.0f || power < -20.0f) continue;
-          const float alpha = fminf(VGR_MAX_ALPHA, p->opacity * vgr_expf(power));
-          if (alpha < VGR_ALPHA_THRESHOLD) continue;
+          float alpha;
+          if (opt->ray_evaluation) {
+            alpha = splat_alpha_at_ray(p, ray_x, ray_y, opt->gaussian_support_sigma);
+            if (!(alpha > 0.0f)) continue;
+          } else {
+            const float dx = fx - p->mean_x;
+            const float dy = fy - p->mean_y;
+            const float power =
+                -0.5f * (p->conic_0 * dx * dx + 2.0f * p->conic_1 * dx * dy + p->conic_2 * dy * dy);
+            if (power > 0.0f || power < -20.0f) continue;
+            alpha = fminf(VGR_MAX_ALPHA, p->opacity * vgr_expf(power));
+            if (alpha < VGR_ALPHA_THRESHOLD) continue;
+          }
           const float next = transmittance * (1.0f - alpha);
-          if (next <= VGR_TRANSMITTANCE_THRESHOLD) break; /* break before accumulating, :2245-2252 */
+          /* Screen model: stop before accumulating (:2245-2252); exact ray: after it (:2268-2272). */
+          if (!opt->ray_evaluation && next <= VGR_TRANSMITTANCE_THRESHOLD) break;
           const float weight = transmittance * alpha;
           red = fmaf(weight, p->rgb[0], red);
           green = fmaf(weight, p->rgb[1], green);
@@ -600,6 +687,7 @@ static void composite_tiles(void* context, int64_t begin, int64_t end) {
             best = p->gaussian;
           }
           transmittance = next;
+          if (opt->ray_evaluation && next <= VGR_TRANSMITTANCE_THRESHOLD) break;
         }
         const int64_t pixel = c->pixel_offset + (int64_t)py * c->width + px;
         const float accumulated = 1.0f - transmittance;
@@ -622,13 +710,39 @@ static void composite_tiles(void* context, int64_t begin, int64_t end) {
   }
 }
 
+static int32_t render_binned(const oracle_scene* scene, const oracle_cameras* cams, const oracle_options* opt,
+                             oracle_outputs* out);
+
+/* Exact-ray renders bin 16x16 raster tiles' intersections in 32x32 tiles
+ * (cuda_backend.py:1973-1985); every pixel reads the same list either way, so the
+ * oracle composites whole bins. */
 int32_t oracle_render(const oracle_scene* scene, const oracle_cameras* cams, const oracle_options* opt,
                       oracle_outputs* out) {
+  if (opt != NULL && opt->ray_evaluation) {
+    if (opt->tile_size != 8 && opt->tile_size != 16) {
+      snprintf(g_error, sizeof(g_error), "exact-ray evaluation renders 8x8 or 16x16 tiles");
+      return 1;
+    }
+    if (opt->antialiased || opt->depth_bucket_count != 0 || cams->widths != NULL) {
+      snprintf(g_error, sizeof(g_error),
+               "exact-ray evaluation takes no antialiasing, depth cutoff or per-camera resolutions");
+      return 1;
+    }
+    oracle_options binned = *opt;
+    if (opt->tile_size == 16) binned.tile_size = 32;
+    return render_binned(scene, cams, &binned, out);
+  }
+  return render_binned(scene, cams, opt, out);
+}
+
+static int32_t render_binned(const oracle_scene* scene, const oracle_cameras* cams, const oracle_options* opt,
+                             oracle_outputs* out) {
   if (!scene || !cams || !opt || !out || !out->rgb) {
     snprintf(g_error, sizeof(g_error), "null argument");
     return 1;
   }
-  if (opt->tile_size <= 0 || (opt->tile_size & (opt->tile_size - 1)) != 0 || opt->tile_size > 16) {
+  if (opt->tile_size <= 0 || (opt->tile_size & (opt->tile_size - 1)) != 0 ||
+      opt->tile_size > (opt->ray_evaluation ? 32 : 16)) {
     snprintf(g_error, sizeof(g_error), "tile_size must be a power of two in [1, 16]");
     return 1;
   }

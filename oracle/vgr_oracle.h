@@ -72,6 +72,8 @@ typedef struct oracle_options {
    * and buckets per pass (accepted; the cutoff does not depend on it). */
   int32_t depth_bucket_count;
   int32_t depth_bucket_group_size;
+  /* Exact-ray evaluation (8x8 or 16x16 raster tiles; 16x16 bins in 32x32 tiles). */
+  int32_t ray_evaluation;
 } oracle_options;
 
 typedef struct oracle_outputs {

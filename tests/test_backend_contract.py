@@ -258,8 +258,13 @@ def test_environment_transforms_are_forwarded_and_must_align():
 
 
 def test_unsupported_reference_modes_are_rejected():
-    with pytest.raises(ValueError, match="ray_gaussian_evaluation"):
-        CustomHipBackend(ray_gaussian_evaluation=True)
+    with pytest.raises(ValueError, match="incompatible with ray_gaussian_evaluation"):
+        CustomHipBackend(ray_gaussian_evaluation=True, rasterize_mode="antialiased")
+    with pytest.raises(ValueError, match="8x8 or 16x16"):
+        CustomHipBackend(ray_gaussian_evaluation=True, tile_size=4)
+    assert CustomHipBackend(ray_gaussian_evaluation=True).bin_tile_size == 32
+    assert CustomHipBackend(ray_gaussian_evaluation=True, tile_size=8).bin_tile_size == 8
+    assert CustomHipBackend().bin_tile_size == 16
     with pytest.raises(ValueError, match="fixed_capacity_sort"):
         CustomHipBackend(fixed_capacity_sort=True, deterministic=True)
 

@@ -52,6 +52,15 @@ int resolve_cameras_per_block(const vgr_options& options) {
   return options.cameras_per_block == 0 ? 32 : options.cameras_per_block;
 }
 
+// Exact-ray renders with 16x16 raster tiles bin their intersections in 32x32
+// tiles (four raster tiles share one sorted list, cuda_backend.py:1973-1985):
+// every tile count, range and capacity of such a render is in bins.
+vgr_options binning_options(const vgr_options& options) {
+  vgr_options binned = options;
+  if (options.ray_evaluation != 0 && options.tile_size == 16) binned.tile_size = 32;
+  return binned;
+}
+
 // Depth-cutoff buckets per pass of the dense path: 0 = 64 (the reference default),
 // never more than the bucket count.
 int32_t resolve_depth_group(const vgr_options& options) {
@@ -256,6 +265,25 @@ int validate_options(const vgr_options& options, const vgr_cameras& cameras) {
     set_error("depth_bucket_count must be at most 2^24; got %d", options.depth_bucket_count);
     return VGR_E_INVALID_ARGUMENT;
   }
+  if (options.ray_evaluation != 0) {
+    if (options.tile_size != 8 && options.tile_size != 16) {
+      set_error("exact-ray evaluation renders 8x8 or 16x16 raster tiles; got tile_size %d", options.tile_size);
+      return VGR_E_INVALID_ARGUMENT;
+    }
+    if (options.antialiased != 0) {
+      set_error("antialiased rasterization is incompatible with exact-ray evaluation (the ray model has no "
+                "screen-space covariance dilation to compensate)");
+      return VGR_E_INVALID_ARGUMENT;
+    }
+    if (!uniform) {
+      set_error("exact-ray evaluation renders batches of one resolution (cameras.widths/heights must be NULL)");
+      return VGR_E_INVALID_ARGUMENT;
+    }
+    if (options.ordering != VGR_ORDERING_GLOBAL_RADIX || options.depth_bucket_count != 0) {
+      set_error("exact-ray evaluation uses the global radix ordering without a depth cutoff");
+      return VGR_E_INVALID_ARGUMENT;
+    }
+  }
   if (options.ordering != VGR_ORDERING_GLOBAL_RADIX && options.ordering != VGR_ORDERING_TILE_FRONT) {
     set_error("ordering must be VGR_ORDERING_GLOBAL_RADIX (0) or VGR_ORDERING_TILE_FRONT (1); got %d",
               options.ordering);
@@ -312,9 +340,10 @@ extern "C" int32_t vgr_workspace_sizes_query(const vgr_cameras* cameras, const v
   }
   const int status = validate_options(*options, *cameras);
   if (status != 0) return status;
-  const int64_t tiles = global_tile_count(*cameras, *options);
-  const ScratchLayout layout = compute_layout(cameras->active_count, scene_gaussians, *options, tiles,
-                                              intersection_capacity, max_camera_tiles(*cameras, *options),
+  const vgr_options binned = binning_options(*options);
+  const int64_t tiles = global_tile_count(*cameras, binned);
+  const ScratchLayout layout = compute_layout(cameras->active_count, scene_gaussians, binned, tiles,
+                                              intersection_capacity, max_camera_tiles(*cameras, binned),
                                               cameras->batch);
   out->scratch_bytes = static_cast<int64_t>(layout.total);
   out->tile_capacity = tiles;
@@ -338,9 +367,17 @@ namespace {
 
 // Validates one render's arguments and resolves its scratch layout and options.
 int prepare_render(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outputs& out, const vgr_workspace& ws,
-                   const vgr_options& opt, ScratchLayout& layout_out, int64_t& tiles_out, vgr_options& resolved_out) {
-  int status = validate_options(opt, cams);
+                   const vgr_options& raster_opt, ScratchLayout& layout_out, int64_t& tiles_out,
+                   vgr_options& resolved_out) {
+  int status = validate_options(raster_opt, cams);
   if (status != 0) return status;
+  if (raster_opt.ray_evaluation != 0 &&
+      (ws.ray_records == nullptr || out.final_transmittance != nullptr || ws.slot_records != nullptr)) {
+    set_error("exact-ray evaluation needs workspace.ray_records and takes no training checkpoints");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  // Tile geometry from here on is the binning geometry.
+  const vgr_options opt = binning_options(raster_opt);
   if (sc.gaussian_count < 0 || sc.gaussian_count > 0x7fffffffLL) {
     set_error("gaussian_count must be in [0, INT32_MAX]; got %lld", static_cast<long long>(sc.gaussian_count));
     return VGR_E_INVALID_ARGUMENT;

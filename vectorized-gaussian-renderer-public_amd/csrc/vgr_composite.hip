@@ -470,6 +470,145 @@ __global__ __launch_bounds__(256) void composite_pixels_kernel(CompositeArgs arg
   }
 }
 
+// ---- exact-ray compositing ---------------------------------------------------------
+//
+// One workgroup per kRaster x kRaster raster tile, reading the sorted list of the
+// bin that holds it (32x32 bins under 16x16 raster tiles; bins equal raster tiles
+// at 8x8).  A batch of list entries is staged in LDS (record and precision rows,
+// coalesced), then every thread evaluates its pixel's ray against the batch in
+// list order (renderer_cuda.cu:2176-2262): alpha is opacity x exp(-m^2/2) for the
+// ray's closest approach m (0 beyond the support, capped at 1), each entry with
+// alpha > 0 is blended, and the pixel stops after the entry that takes its
+// transmittance to 1e-4.
+__device__ __forceinline__ float ray_alpha(float ray_x, float ray_y, const f32x4& a, const f32x4& b,
+                                           const f32x4& c, float support_sigma) {
+  // a = (P00, P01, P02, P11), b = (P12, P22, Pm0, Pm1), c = (Pm2, mPm, opacity, depth).
+  const float along = a.x * ray_x * ray_x + 2.0f * a.y * ray_x * ray_y + 2.0f * a.z * ray_x + a.w * ray_y * ray_y +
+                      2.0f * b.x * ray_y + b.y;
+  if (!(along > 0.0f)) return 0.0f;
+  const float cross = ray_x * b.z + ray_y * b.w + c.x;
+  const float distance2 = fmaxf(0.0f, c.y - cross * cross / along);
+  if (distance2 > support_sigma * support_sigma) return 0.0f;
+  return fminf(1.0f, c.z * vgr_expf(-0.5f * distance2));
+}
+
+template <int kRaster, bool kFull>
+__global__ __launch_bounds__(kRaster* kRaster) void composite_ray_kernel(CompositeArgs args,
+                                                                         const float* __restrict__ ray_records,
+                                                                         float support_sigma, int raster_tiles_x,
+                                                                         int raster_tiles, int bin_size) {
+  constexpr int kThreads = kRaster * kRaster;
+  __shared__ f32x4 s_a[kThreads];
+  __shared__ f32x4 s_b[kThreads];
+  __shared__ f32x4 s_c[kThreads];
+  __shared__ f32x4 s_colour[kThreads];  // red, green, blue, gaussian id bits
+  const int64_t block = blockIdx.x;
+  const int active = static_cast<int>(block / raster_tiles);
+  if (active >= args.active_count) return;
+  const CameraParams& cam = args.table[active];
+  if (cam.camera < 0) return;
+  const int raster_tile = static_cast<int>(block - static_cast<int64_t>(active) * raster_tiles);
+  const int tile_x = raster_tile % raster_tiles_x;
+  const int tile_y = raster_tile / raster_tiles_x;
+  const int pixel_x = tile_x * kRaster + static_cast<int>(threadIdx.x) % kRaster;
+  const int pixel_y = tile_y * kRaster + static_cast<int>(threadIdx.x) / kRaster;
+  const bool valid_pixel = pixel_x < cam.width && pixel_y < cam.height;
+  const int bin_x = tile_x * kRaster / bin_size;
+  const int bin_y = tile_y * kRaster / bin_size;
+  const uint32_t global_bin = cam.tile_base + static_cast<uint32_t>(bin_y * cam.tiles_x + bin_x);
+  const float ray_x = ((static_cast<float>(pixel_x) + 0.5f) - cam.center_x) / cam.focal_x;
+  const float ray_y = ((static_cast<float>(pixel_y) + 0.5f) - cam.center_y) / cam.focal_y;
+
+  float transmittance = 1.0f;
+  float red = 0.0f, green = 0.0f, blue = 0.0f, depth_sum = 0.0f;
+  float best_weight = -1.0f;
+  int best_gaussian = -1;
+  bool done = !valid_pixel;
+  const int64_t range_start = args.tile_starts[global_bin];
+  const int64_t range_end = args.tile_ends[global_bin];
+  if (range_start >= 0 && range_end > range_start) {
+    for (int64_t batch_start = range_start; batch_start < range_end; batch_start += kThreads) {
+      const int count = static_cast<int>(range_end - batch_start < kThreads ? range_end - batch_start : kThreads);
+      __syncthreads();  // previous batch fully consumed
+      if (static_cast<int>(threadIdx.x) < count) {
+        const int32_t record = args.sorted_values[batch_start + threadIdx.x];
+        if (record >= 0 && record < args.visible_capacity) {
+          const f32x4* rec = reinterpret_cast<const f32x4*>(args.records + record);
+          const f32x4* ray = reinterpret_cast<const f32x4*>(ray_records + static_cast<int64_t>(record) * 12);
+          const f32x4 r1 = rec[1], r2 = rec[2], q2 = ray[2];
+          s_a[threadIdx.x] = ray[0];
+          s_b[threadIdx.x] = ray[1];
+          s_c[threadIdx.x] = f32x4{q2.x, q2.y, r1.y, r1.z};
+          s_colour[threadIdx.x] = f32x4{r1.w, r2.x, r2.y, r2.z};
+        } else {
+          // Only reachable with a corrupted list: an entry that never blends.
+          s_a[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+          s_b[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+          s_c[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+          s_colour[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, __int_as_float(-1)};
+        }
+      }
+      __syncthreads();
+      for (int item = 0; !done && item < count; ++item) {
+        const f32x4 c = s_c[item];
+        const float alpha = ray_alpha(ray_x, ray_y, s_a[item], s_b[item], c, support_sigma);
+        if (!(alpha > 0.0f)) continue;
+        const float next_transmittance = transmittance * (1.0f - alpha);
+        const f32x4 col = s_colour[item];
+        const float weight = transmittance * alpha;
+        red = fmaf(weight, col.x, red);
+        green = fmaf(weight, col.y, green);
+        blue = fmaf(weight, col.z, blue);
+        if (kFull) {
+          depth_sum = fmaf(weight, c.w, depth_sum);
+          if (weight > best_weight) {
+            best_weight = weight;
+            best_gaussian = __float_as_int(col.w);
+          }
+        }
+        transmittance = next_transmittance;
+        if (next_transmittance <= VGR_TRANSMITTANCE_THRESHOLD) done = true;
+      }
+      if (!__syncthreads_or(done ? 0 : 1)) break;
+    }
+  }
+  if (!valid_pixel) return;
+  const int64_t pixel = cam.pixel_offset + static_cast<int64_t>(pixel_y) * cam.width + pixel_x;
+  const float accumulated_alpha = 1.0f - transmittance;
+  if (args.output_srgb) {
+    red = vgr_linear_to_srgb(red);
+    green = vgr_linear_to_srgb(green);
+    blue = vgr_linear_to_srgb(blue);
+  }
+  args.rgb[pixel * 3 + 0] = red;
+  args.rgb[pixel * 3 + 1] = green;
+  args.rgb[pixel * 3 + 2] = blue;
+  if (kFull) {
+    args.alpha[pixel] = accumulated_alpha;
+    args.depth[pixel] = accumulated_alpha > 1.0e-8f ? depth_sum / accumulated_alpha : __int_as_float(0x7f800000);
+    const bool labelled = best_weight >= 0.0f && accumulated_alpha >= args.semantic_min_alpha &&
+                          best_gaussian >= 0 && best_gaussian < args.gaussian_count;
+    args.semantic[pixel] = labelled ? args.semantic_ids[best_gaussian] : -1;
+  }
+}
+
+template <int kRaster>
+void launch_ray(const CompositeArgs& args, const vgr_options& options, const float* ray_records, int active_count,
+                bool full, hipStream_t stream) {
+  const int raster_tiles_x = (options.width + kRaster - 1) / kRaster;
+  const int raster_tiles = raster_tiles_x * ((options.height + kRaster - 1) / kRaster);
+  const int64_t blocks = static_cast<int64_t>(active_count) * raster_tiles;
+  if (blocks <= 0) return;
+  const dim3 grid(static_cast<unsigned>(blocks)), block(kRaster * kRaster);
+  if (full) {
+    hipLaunchKernelGGL((composite_ray_kernel<kRaster, true>), grid, block, 0, stream, args, ray_records,
+                       options.gaussian_support_sigma, raster_tiles_x, raster_tiles, options.tile_size);
+  } else {
+    hipLaunchKernelGGL((composite_ray_kernel<kRaster, false>), grid, block, 0, stream, args, ray_records,
+                       options.gaussian_support_sigma, raster_tiles_x, raster_tiles, options.tile_size);
+  }
+}
+
 // Minimum waves per SIMD for 16x16 tiles (launch-bounds occupancy target): eight
 // means at most 64 VGPRs.  The blend loop is latency-bound, so waves in flight
 // pay more than registers.  The evaluation-counting and persistent variants
@@ -660,6 +799,20 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const v
     return VGR_E_INVALID_ARGUMENT;
   }
   const bool full = options.full_sensor_output != 0;
+  if (options.ray_evaluation != 0) {
+    // options.tile_size is the bin size here (32 bins under 16x16 raster tiles).
+    if (options.tile_size == 8) {
+      launch_ray<8>(args, options, ws.ray_records, cameras.active_count, full, stream);
+    } else {
+      launch_ray<16>(args, options, ws.ray_records, cameras.active_count, full, stream);
+    }
+    const hipError_t error = hipGetLastError();
+    if (error != hipSuccess) {
+      set_error("composite launch failed: %s", hipGetErrorString(error));
+      return VGR_E_LAUNCH;
+    }
+    return 0;
+  }
   if (options.tile_size < 8) {
     const int64_t threads = blocks * options.tile_size * options.tile_size;
     const dim3 grid(static_cast<unsigned>((threads + 255) / 256)), block(256);

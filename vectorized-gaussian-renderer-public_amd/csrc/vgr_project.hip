@@ -151,7 +151,8 @@ __global__ void build_camera_table_kernel(
 }
 
 
-template <bool kDense>
+// kMode: 0 tile rectangles, 1 dense per-pixel passes, 2 exact-ray extents.
+template <int kMode>
 __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(ProjectionArgs args,
                                                                          int64_t* __restrict__ block_counts,
                                                                          uint64_t* __restrict__ wave_masks) {
@@ -182,8 +183,15 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
     Projected p;
     TileRect rect;
     int tiles;
-    const bool seen = kDense ? evaluate_dense_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles)
-                             : evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    bool seen;
+    if constexpr (kMode == 1) {
+      seen = evaluate_dense_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    } else if constexpr (kMode == 2) {
+      RayData ray;
+      seen = evaluate_ray_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles, ray);
+    } else {
+      seen = evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    }
     if (seen) {
       visible += 1;
       intersections += tiles;
@@ -201,12 +209,13 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
   }
 }
 
-template <int kDegree, bool kDense>
+template <int kDegree, int kMode>
 __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
     ProjectionArgs args, const int64_t* __restrict__ block_bases, const uint64_t* __restrict__ wave_masks,
     vgr_record* __restrict__ records, int32_t* __restrict__ radii, uint64_t* __restrict__ keys,
     int32_t* __restrict__ values, int32_t* __restrict__ slot_records, int64_t visible_capacity,
-    int64_t intersection_capacity) {
+    int64_t intersection_capacity, float* __restrict__ ray_records) {
+  constexpr bool kDense = kMode == 1;
   int64_t block;
   uint64_t mask;
   if (!resolve_block(args, block, mask)) return;
@@ -239,9 +248,15 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
     Projected p;
     TileRect rect;
     int tiles;
-    const bool visible = kDense
-                             ? evaluate_dense_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles)
-                             : evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    RayData ray;
+    bool visible;
+    if constexpr (kMode == 1) {
+      visible = evaluate_dense_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    } else if constexpr (kMode == 2) {
+      visible = evaluate_ray_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles, ray);
+    } else {
+      visible = evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    }
     int inclusive_visible = visible ? 1 : 0;
     int inclusive_tiles = tiles;
     wave_inclusive_scan2(inclusive_visible, inclusive_tiles);
@@ -260,6 +275,12 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
         if (radii != nullptr) {
           radii[record * 2 + 0] = p.radius_x;
           radii[record * 2 + 1] = p.radius_y;
+        }
+        if constexpr (kMode == 2) {
+          float4* row = reinterpret_cast<float4*>(ray_records + record * 12);
+          row[0] = make_float4(ray.precision[0], ray.precision[1], ray.precision[2], ray.precision[3]);
+          row[1] = make_float4(ray.precision[4], ray.precision[5], ray.precision_mean[0], ray.precision_mean[1]);
+          row[2] = make_float4(ray.precision_mean[2], ray.mean_precision_mean, 0.0f, 0.0f);
         }
       }
     }
@@ -741,12 +762,12 @@ int launch_cull(const ProjectionArgs& args, const vgr_scene& scene, const Scratc
 namespace {
 
 // One count -> scan -> write pass over the projection grid (records and/or intersections).
-template <bool kDense>
+template <int kMode>
 void projection_pass(const ProjectionArgs& args, const vgr_workspace& ws, int64_t* block_counts, int64_t* block_bases,
                      uint64_t* wave_masks, int64_t* partials, int64_t blocks, hipStream_t stream) {
   const int64_t waves = blocks * kProjectionWaves;
   if (blocks > 0) {
-    hipLaunchKernelGGL(projection_count_kernel<kDense>, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0,
+    hipLaunchKernelGGL(projection_count_kernel<kMode>, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), 0,
                        stream, args, block_counts, wave_masks);
   }
   scan_pairs(block_counts, waves, block_bases, partials, ws.visible_capacity, ws.intersection_capacity,
@@ -755,24 +776,24 @@ void projection_pass(const ProjectionArgs& args, const vgr_workspace& ws, int64_
   const dim3 grid(static_cast<unsigned>(blocks)), threads(kBlockThreads);
   switch (args.sh_degree) {
     case 1:
-      hipLaunchKernelGGL((projection_write_kernel<1, kDense>), grid, threads, 0, stream, args, block_bases, wave_masks,
+      hipLaunchKernelGGL((projection_write_kernel<1, kMode>), grid, threads, 0, stream, args, block_bases, wave_masks,
                          ws.records, ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
-                         ws.intersection_capacity);
+                         ws.intersection_capacity, ws.ray_records);
       break;
     case 2:
-      hipLaunchKernelGGL((projection_write_kernel<2, kDense>), grid, threads, 0, stream, args, block_bases, wave_masks,
+      hipLaunchKernelGGL((projection_write_kernel<2, kMode>), grid, threads, 0, stream, args, block_bases, wave_masks,
                          ws.records, ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
-                         ws.intersection_capacity);
+                         ws.intersection_capacity, ws.ray_records);
       break;
     case 3:
-      hipLaunchKernelGGL((projection_write_kernel<3, kDense>), grid, threads, 0, stream, args, block_bases, wave_masks,
+      hipLaunchKernelGGL((projection_write_kernel<3, kMode>), grid, threads, 0, stream, args, block_bases, wave_masks,
                          ws.records, ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
-                         ws.intersection_capacity);
+                         ws.intersection_capacity, ws.ray_records);
       break;
     default:
-      hipLaunchKernelGGL((projection_write_kernel<0, kDense>), grid, threads, 0, stream, args, block_bases, wave_masks,
+      hipLaunchKernelGGL((projection_write_kernel<0, kMode>), grid, threads, 0, stream, args, block_bases, wave_masks,
                          ws.records, ws.radii, ws.keys_in, ws.values_in, ws.slot_records, ws.visible_capacity,
-                         ws.intersection_capacity);
+                         ws.intersection_capacity, ws.ray_records);
       break;
   }
 }
@@ -804,23 +825,25 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
     args.item_count = reinterpret_cast<const int64_t*>(scratch + layout.item_totals);
   }
 
-  if (options.tile_size != 1) {
-    projection_pass<false>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
+  if (options.ray_evaluation != 0) {
+    projection_pass<2>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
+  } else if (options.tile_size != 1) {
+    projection_pass<0>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
   } else if (options.depth_bucket_count <= 0) {
     args.dense_mode = 0;  // records and pixel intersections in one pass
-    projection_pass<true>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
+    projection_pass<1>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
   } else {
     // Records first, then the per-pixel depth cutoff from them, then the pixel
     // intersections that survive it (the second pass re-evaluates the same pairs,
     // so its scanned record indices are those of the first).
     args.dense_mode = 1;
-    projection_pass<true>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
+    projection_pass<1>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
     const int status = launch_dense_cutoff(cameras, options, ws, layout, scratch, stream);
     if (status != 0) return status;
     args.dense_mode = 2;
     args.depth_buckets = options.depth_bucket_count;
     args.depth_cutoff = reinterpret_cast<const int32_t*>(scratch + layout.dense_cutoff);
-    projection_pass<true>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
+    projection_pass<1>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
   }
   const hipError_t status = hipGetLastError();
   if (status != hipSuccess) {

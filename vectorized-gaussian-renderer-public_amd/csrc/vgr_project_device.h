@@ -42,6 +42,9 @@ struct Projected {
 // Screen-space projection of one (camera, Gaussian) pair.  Operation order
 // follows renderer_cuda.cu:614-1055 term by term (left-to-right sums, no
 // contraction), with vgr_logf in place of __logf.
+// kRay: exact-ray renders cull on the support quadric's screen ellipse instead
+// (ray_extent), so the screen-space cull of the splat is skipped here.
+template <bool kRay = false>
 __device__ __forceinline__ bool project_pair(
     const CameraParams& cam, float m0, float m1, float m2, float opacity,
     const float* cov, float near_plane, float far_plane, float support_sigma,
@@ -106,10 +109,10 @@ __device__ __forceinline__ bool project_pair(
   const int radius_x = float_to_int_clamped(ceilf(extend * sqrtf(cov00)));
   const int radius_y = float_to_int_clamped(ceilf(extend * sqrtf(cov11)));
   if (radius_x <= 0 && radius_y <= 0) return false;
-  if (mean_x + static_cast<float>(radius_x) <= 0.0f ||
+  if (!kRay && (mean_x + static_cast<float>(radius_x) <= 0.0f ||
       mean_y + static_cast<float>(radius_y) <= 0.0f ||
       mean_x - static_cast<float>(radius_x) >= static_cast<float>(cam.width) ||
-      mean_y - static_cast<float>(radius_y) >= static_cast<float>(cam.height)) {
+      mean_y - static_cast<float>(radius_y) >= static_cast<float>(cam.height))) {
     return false;
   }
   const float inverse_determinant = 1.0f / determinant;
@@ -123,6 +126,114 @@ __device__ __forceinline__ bool project_pair(
   out.radius_x = radius_x;
   out.radius_y = radius_y;
   return true;
+}
+
+// ---- exact-ray evaluation ------------------------------------------------------
+
+// Camera-space precision of a splat for the ray test: P = Sigma^-1 (6), P mu (3),
+// mu^T P mu (1), as stored in workspace.ray_records.
+struct RayData {
+  float precision[6];  // 00, 01, 02, 11, 12, 22
+  float precision_mean[3];
+  float mean_precision_mean;
+};
+
+// Exact-ray extent of a projected pair (renderer_cuda.cu:795-956): the camera-
+// space precision in double, the screen ellipse of the rays whose closest point
+// lies within support_sigma (the quadric r^T (q P - P mu mu^T P) r with
+// q = mu^T P mu - sigma^2, centre and half extents), which replaces the splat's
+// mean and radii for binning and culling; a degenerate ellipse covers the image.
+// Returns false when the covariance is singular or the ellipse misses the image.
+__device__ __forceinline__ bool ray_extent(const CameraParams& cam, float m0, float m1, float m2, const float* cov,
+                                           float support_sigma, Projected& p, RayData& ray) {
+  // Camera-space mean and covariance with project_pair's operations.
+  const float* e = cam.env;
+  const float env_x = e[0] * m0 + e[1] * m1 + e[2] * m2 + e[3];
+  const float env_y = e[4] * m0 + e[5] * m1 + e[6] * m2 + e[7];
+  const float env_z = e[8] * m0 + e[9] * m1 + e[10] * m2 + e[11];
+  const float* v = cam.view;
+  const float camera_x = v[0] * env_x + v[1] * env_y + v[2] * env_z + v[3];
+  const float camera_y = v[4] * env_x + v[5] * env_y + v[6] * env_z + v[7];
+  const float camera_z = v[8] * env_x + v[9] * env_y + v[10] * env_z + v[11];
+  const float* r = cam.rotation;
+  float rc[9];
+#pragma unroll
+  for (int row = 0; row < 3; ++row) {
+    const float r0 = r[row * 3 + 0];
+    const float r1 = r[row * 3 + 1];
+    const float r2 = r[row * 3 + 2];
+    rc[row * 3 + 0] = r0 * cov[0] + r1 * cov[1] + r2 * cov[2];
+    rc[row * 3 + 1] = r0 * cov[1] + r1 * cov[3] + r2 * cov[4];
+    rc[row * 3 + 2] = r0 * cov[2] + r1 * cov[4] + r2 * cov[5];
+  }
+  const double c00 = rc[0] * r[0] + rc[1] * r[1] + rc[2] * r[2];
+  const double c01 = rc[0] * r[3] + rc[1] * r[4] + rc[2] * r[5];
+  const double c02 = rc[0] * r[6] + rc[1] * r[7] + rc[2] * r[8];
+  const double c11 = rc[3] * r[3] + rc[4] * r[4] + rc[5] * r[5];
+  const double c12 = rc[3] * r[6] + rc[4] * r[7] + rc[5] * r[8];
+  const double c22 = rc[6] * r[6] + rc[7] * r[7] + rc[8] * r[8];
+  const double det = c00 * (c11 * c22 - c12 * c12) - c01 * (c01 * c22 - c02 * c12) + c02 * (c01 * c12 - c02 * c11);
+  if (!(det > 1.0e-48)) return false;
+  const double inverse_det = 1.0 / det;
+  const double p00 = (c11 * c22 - c12 * c12) * inverse_det;
+  const double p01 = (c02 * c12 - c01 * c22) * inverse_det;
+  const double p02 = (c01 * c12 - c02 * c11) * inverse_det;
+  const double p11 = (c00 * c22 - c02 * c02) * inverse_det;
+  const double p12 = (c01 * c02 - c00 * c12) * inverse_det;
+  const double p22 = (c00 * c11 - c01 * c01) * inverse_det;
+  const double x = camera_x, y = camera_y, z = camera_z;
+  const double pm0 = p00 * x + p01 * y + p02 * z;
+  const double pm1 = p01 * x + p11 * y + p12 * z;
+  const double pm2 = p02 * x + p12 * y + p22 * z;
+  const double mpm = x * pm0 + y * pm1 + z * pm2;
+  ray.precision[0] = static_cast<float>(p00);
+  ray.precision[1] = static_cast<float>(p01);
+  ray.precision[2] = static_cast<float>(p02);
+  ray.precision[3] = static_cast<float>(p11);
+  ray.precision[4] = static_cast<float>(p12);
+  ray.precision[5] = static_cast<float>(p22);
+  ray.precision_mean[0] = static_cast<float>(pm0);
+  ray.precision_mean[1] = static_cast<float>(pm1);
+  ray.precision_mean[2] = static_cast<float>(pm2);
+  ray.mean_precision_mean = static_cast<float>(mpm);
+
+  const double support2 = static_cast<double>(support_sigma) * static_cast<double>(support_sigma);
+  const double q = mpm - support2;
+  const double k00 = q * p00 - pm0 * pm0;
+  const double k01 = q * p01 - pm0 * pm1;
+  const double k02 = q * p02 - pm0 * pm2;
+  const double k11 = q * p11 - pm1 * pm1;
+  const double k12 = q * p12 - pm1 * pm2;
+  const double k22 = q * p22 - pm2 * pm2;
+  const double k_det = k00 * k11 - k01 * k01;
+  bool finite = false;
+  if (k00 > 0.0 && k_det > 0.0) {
+    const double inverse_k = 1.0 / k_det;
+    const double centre_x = (k01 * k12 - k11 * k02) * inverse_k;
+    const double centre_y = (k01 * k02 - k00 * k12) * inverse_k;
+    const double scale = -(k22 + k02 * centre_x + k12 * centre_y);
+    if (scale > 0.0) {
+      const double extent_x = sqrt(scale * k11 * inverse_k);
+      const double extent_y = sqrt(scale * k00 * inverse_k);
+      const double mean_x = static_cast<double>(cam.focal_x) * centre_x + static_cast<double>(cam.center_x);
+      const double mean_y = static_cast<double>(cam.focal_y) * centre_y + static_cast<double>(cam.center_y);
+      p.mean_x = static_cast<float>(mean_x);
+      p.mean_y = static_cast<float>(mean_y);
+      p.radius_x = max(1, float_to_int_clamped(ceilf(static_cast<float>(fabs(static_cast<double>(cam.focal_x)) *
+                                                                        extent_x))));
+      p.radius_y = max(1, float_to_int_clamped(ceilf(static_cast<float>(fabs(static_cast<double>(cam.focal_y)) *
+                                                                        extent_y))));
+      finite = isfinite(mean_x) && isfinite(mean_y) && isfinite(extent_x) && isfinite(extent_y);
+    }
+  }
+  if (!finite) {
+    p.mean_x = 0.5f * static_cast<float>(cam.width);
+    p.mean_y = 0.5f * static_cast<float>(cam.height);
+    p.radius_x = p.radius_y = max(cam.width, cam.height);
+  }
+  return !(p.mean_x + static_cast<float>(p.radius_x) <= 0.0f || p.mean_y + static_cast<float>(p.radius_y) <= 0.0f ||
+           p.mean_x - static_cast<float>(p.radius_x) >= static_cast<float>(cam.width) ||
+           p.mean_y - static_cast<float>(p.radius_y) >= static_cast<float>(cam.height));
 }
 
 struct TileRect {
@@ -331,6 +442,24 @@ __device__ __forceinline__ int dense_pixels(const ProjectionArgs& args, const Ca
     }
   }
   return count;
+}
+
+// evaluate_pair for exact-ray renders: the ray extent replaces the splat's mean
+// and radii (binning and culling); `ray` receives the record's precision data.
+__device__ __forceinline__ bool evaluate_ray_pair(const ProjectionArgs& args, const CameraParams& cam, int64_t g,
+                                                  bool loaded, float m0, float m1, float m2, float opacity,
+                                                  const float* cov, Projected& p, TileRect& rect, int& tiles,
+                                                  RayData& ray) {
+  tiles = 0;
+  if (!loaded || cam.scene_slot < 0 || g < cam.gaussian_begin || g >= cam.gaussian_end) return false;
+  if (!project_pair<true>(cam, m0, m1, m2, opacity, cov, args.near_plane, args.far_plane, args.support_sigma,
+                          args.covariance_epsilon, args.antialiased != 0, p)) {
+    return false;
+  }
+  if (!ray_extent(cam, m0, m1, m2, cov, args.support_sigma, p, ray)) return false;
+  rect = tile_rect(p, args.tile_size, cam.tiles_x, cam.tiles_y);
+  tiles = rect_tiles(rect);
+  return true;
 }
 
 // evaluate_pair for the dense passes: the pair's intersections are its emitted

@@ -48,6 +48,7 @@ from .native import (
     COUNTER_SLOTS,
     RECORD_FLOATS,
     SORT_FAILURE_SLOT,
+    RAY_RECORD_FLOATS,
     WORKSPACE_TENSORS,
     dense_cutoff,
     float_bits,
@@ -280,7 +281,7 @@ class CustomHipBackend:
     def _global_tiles(self) -> int:
         layout = self._layout
         assert layout is not None
-        ts = self.options.tile_size
+        ts = self.bin_tile_size
         return layout.physical_views * (-(-layout.width // ts)) * (-(-layout.height // ts))
 
     def _scratch_bytes(self, intersections: int) -> int:
@@ -297,7 +298,8 @@ class CustomHipBackend:
                          gaussians=gaussians, intersection_capacity=intersections, tile_size=o.tile_size,
                          near_plane=o.near_plane, far_plane=o.far_plane, sort_digit_bits=o.sort_digit_bits,
                          cameras_per_block=o.cameras_per_block, sort_tile_items=o.sort_tile_items,
-                         depth_bucket_count=buckets, depth_bucket_group_size=group))
+                         depth_bucket_count=buckets, depth_bucket_group_size=group,
+                         ray_evaluation=o.ray_gaussian_evaluation))
 
     def _check_native_limits(self, capacity: Capacity) -> None:
         if capacity.visible_records <= 0 or capacity.tile_intersections <= 0:
@@ -336,6 +338,9 @@ class CustomHipBackend:
             "dummy_alpha": BufferSpec((1,), torch.float32),
             "dummy_semantic": BufferSpec((1,), torch.int64),
         }
+        if o.ray_gaussian_evaluation:
+            # Per record: the camera-space precision and its products with the mean.
+            buffers["ray_records"] = BufferSpec((v, RAY_RECORD_FLOATS), torch.float32)
         if o.chunked:
             buffers.update({
                 "logical_chunk_counters": BufferSpec((layout.chunks, COUNTER_SLOTS), torch.int64),
@@ -616,10 +621,13 @@ class CustomHipBackend:
             extra["statistics"] = self.statistics
         if o.lookback_spin_limit:
             extra["lookback_spin_limit"] = o.lookback_spin_limit
+        if o.ray_gaussian_evaluation:
+            extra["ray_records"] = ws["ray_records"]
         selector = int(self._native.render(
             self._packed.native_list(), cameras, outputs, workspace, self._layout.height, self._layout.width,
             self._packed.max_scene_gaussians, o.near_plane, o.far_plane, o.gaussian_support_sigma,
-            o.covariance_epsilon, o.rasterize_mode == "antialiased", o.semantic_min_alpha, False, o.tile_size,
+            o.covariance_epsilon, o.rasterize_mode == "antialiased", o.semantic_min_alpha,
+            o.ray_gaussian_evaluation, o.tile_size,
             o.depth_bucket_count, min(o.depth_bucket_group_size, o.depth_bucket_count), o.compact_projection_cache,
             o.materialize_projected_records, reuse,
             o.output_srgb, o.deterministic, full, o.fixed_capacity_sort, sh_degree=self._sh_degree,
@@ -834,7 +842,10 @@ class CustomHipBackend:
 
     @property
     def bin_tile_size(self) -> int:
-        return self.options.tile_size
+        """Tile size of the intersection lists: exact-ray evaluation at 16x16 raster tiles
+        shares one 32x32 list among four raster tiles (cuda_backend.py:1973-1985)."""
+        o = self.options
+        return 2 * o.tile_size if o.ray_gaussian_evaluation and o.tile_size == 16 else o.tile_size
 
     @property
     def rasterize_mode(self) -> Literal["classic", "antialiased"]:

@@ -45,7 +45,8 @@ RECORD_FLOATS = 12  # 48-byte visible record, see vgr_record in include/vgr.h
 RECORD_GRAD_FLOATS = 10  # dL/d (mean_x, mean_y, conic_0..2, opacity, depth, r, g, b) per intersection slot
 CAMERA_GRAD_FLOATS = 24  # per-camera pose partials of the backward pass
 POSE_REPLICAS = 32  # copies of the pose partials (VGR_POSE_REPLICAS)
-ABI_VERSION = 6
+RAY_RECORD_FLOATS = 12  # exact-ray precision data per visible record (workspace.ray_records)
+ABI_VERSION = 7
 # Device counter block: the reference's five counters (cuda_backend.py:22-28) plus
 # one slot counting radix-sort look-back waits that gave up (the order is then wrong).
 COUNTER_SLOTS = 6
@@ -141,6 +142,7 @@ class _Workspace(ctypes.Structure):
         ("tile_capacity", ctypes.c_int64),
         ("scratch_bytes", ctypes.c_int64),
         ("slot_records", ctypes.c_void_p),
+        ("ray_records", ctypes.c_void_p),
     ]
 
 
@@ -173,6 +175,7 @@ class _Options(ctypes.Structure):
         ("ordering", ctypes.c_int32),
         ("depth_bucket_count", ctypes.c_int32),
         ("depth_bucket_group_size", ctypes.c_int32),
+        ("ray_evaluation", ctypes.c_int32),
     ]
 
 
@@ -325,7 +328,7 @@ class NativeRenderer:
                       cameras_per_block: int = 0, sort_tile_items: int = 0, chunk_culling: bool = False,
                       resolutions: Sequence[tuple[int, int]] | None = None,
                       ordering: int = ORDERING_GLOBAL_RADIX, depth_bucket_count: int = 0,
-                      depth_bucket_group_size: int = 0) -> int:
+                      depth_bucket_group_size: int = 0, ray_evaluation: bool = False) -> int:
         """Scratch bytes one render of up to ``max_views`` cameras needs (``depth_bucket_count`` /
         ``depth_bucket_group_size``: the effective cutoff of :func:`dense_cutoff`)."""
         cameras = _Cameras()
@@ -343,7 +346,8 @@ class NativeRenderer:
                                 sort_digit_bits=sort_digit_bits, cameras_per_block=cameras_per_block,
                                 sort_tile_items=sort_tile_items, chunk_culling=int(bool(chunk_culling)),
                                 ordering=int(ordering), depth_bucket_count=int(depth_bucket_count),
-                                depth_bucket_group_size=int(depth_bucket_group_size))
+                                depth_bucket_group_size=int(depth_bucket_group_size),
+                                ray_evaluation=int(bool(ray_evaluation)))
         sizes = _Sizes()
         self._check(self.lib.vgr_workspace_sizes_query(ctypes.byref(cameras), ctypes.byref(options), int(gaussians),
                                                         int(intersection_capacity), ctypes.byref(sizes)),
@@ -433,6 +437,7 @@ class NativeRenderer:
         ordering: int = ORDERING_GLOBAL_RADIX,
         max_camera_tiles: int = 0,
         slot_records: torch.Tensor | None = None,
+        ray_records: torch.Tensor | None = None,
     ) -> int:
         """One render submission; returns the sorted-buffer selector (1: *_out holds the result).
 
@@ -451,8 +456,9 @@ class NativeRenderer:
         forwards (``checkpoints``) also take ``slot_records`` (int32, one per intersection slot):
         the sort then carries slots, which :meth:`render_backward` needs.
         """
-        if ray_gaussian_evaluation:
-            raise ValueError("ray_gaussian_evaluation is not available on the gfx950 tile path.")
+        if ray_gaussian_evaluation and ray_records is None:
+            raise ValueError("ray_gaussian_evaluation needs the ray_records workspace (RAY_RECORD_FLOATS per "
+                             "visible record)")
         if (compact_projection_cache or materialize_projected_records) and int(tile_size) != 1:
             raise ValueError("compact_projection_cache requires tile_size=1.")
         # Records are always materialized here (48-byte records written by the projection
@@ -538,6 +544,12 @@ class NativeRenderer:
             tile_ends=_ptr(tile_ends), counters=_ptr(counters), scratch=_ptr(scratch),
             visible_capacity=int(records.shape[0]), intersection_capacity=int(keys_in.numel()),
             tile_capacity=int(tile_starts.numel()), scratch_bytes=int(scratch.numel()))
+        if ray_gaussian_evaluation:
+            if (ray_records.dtype != torch.float32 or ray_records.device != device or not ray_records.is_contiguous()
+                    or ray_records.numel() < RAY_RECORD_FLOATS * int(records.shape[0])):
+                raise ValueError("ray_records must be a contiguous float32 tensor of RAY_RECORD_FLOATS per visible "
+                                 "record on the render device")
+            c_workspace.ray_records = ray_records.data_ptr()
         if cutoff_buckets > 0:
             # The cutoff reads the records' radii between the records pass and the
             # emission pass; keys_out is idle until the sort, so it holds them.
@@ -561,7 +573,8 @@ class NativeRenderer:
             cameras_per_block=int(cameras_per_block), reuse_projection=int(bool(reuse_projection)),
             sort_tile_items=int(sort_tile_items), persistent_composite=int(persistent_composite),
             chunk_culling=int(chunk_bounds is not None), lookback_spin_limit=int(lookback_spin_limit),
-            ordering=int(ordering), depth_bucket_count=cutoff_buckets, depth_bucket_group_size=cutoff_group)
+            ordering=int(ordering), depth_bucket_count=cutoff_buckets, depth_bucket_group_size=cutoff_group,
+            ray_evaluation=int(bool(ray_gaussian_evaluation)))
         event_array = None
         if stage_events is not None:
             if len(stage_events) != 5:

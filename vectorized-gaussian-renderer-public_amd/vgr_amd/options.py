@@ -100,7 +100,11 @@ _RULES: tuple[tuple[Callable[[BackendOptions], bool], str], ...] = (
      "compact_projection_cache currently requires deterministic=False."),
     (lambda o: o.materialize_projected_records and not o.compact_projection_cache,
      "materialize_projected_records requires compact_projection_cache=True."),
-    (lambda o: o.ray_gaussian_evaluation, "ray_gaussian_evaluation is not available in the gfx950 build."),
+    (lambda o: o.rasterize_mode == "antialiased" and o.ray_gaussian_evaluation,
+     "rasterize_mode='antialiased' is incompatible with ray_gaussian_evaluation=True because the exact-ray path "
+     "does not use the screen-space covariance dilation being compensated."),
+    (lambda o: o.ray_gaussian_evaluation and o.tile_size not in (8, 16),
+     "ray_gaussian_evaluation renders 8x8 or 16x16 tiles on the gfx950 build."),
     (lambda o: o.fixed_capacity_sort and o.deterministic,
      "fixed_capacity_sort applies to the nondeterministic global-radix path only; it cannot be combined with "
      "deterministic=True."),
