@@ -230,7 +230,7 @@ typedef struct vgr_options {
 /* Intersection ordering of a render.
  * GLOBAL_RADIX: every intersection of the batch is sorted by (tile, depth bits)
  *   with the onesweep radix sort (the reference's DeviceRadixSort call,
- *   renderer_cuda.cu:3215-3269); intersection capacity below 2^30.
+ *   renderer_cuda.cu:3215-3269); intersection capacity up to 2^31 - 1.
  * TILE_FRONT: no reference counterpart.  Intersections are bucketed per tile
  *   (key = depth bits relative to near, value = packed Gaussian index), each
  *   tile sorts only the nearest entries of its bucket (at most 4096, chosen by

@@ -150,7 +150,7 @@ void validation() {
   workspace.counters = dummy_i;
   workspace.scratch = dummy_f;
   workspace.visible_capacity = 1;
-  workspace.intersection_capacity = 1LL << 30;  // beyond the radix ordering's limit
+  workspace.intersection_capacity = 1LL << 31;  // beyond the int32 position limit
   workspace.tile_capacity = 1200;
   workspace.scratch_bytes = 1 << 30;
   expect(vgr_render(&scene, &cameras, &outputs, &workspace, &options, nullptr, &selector) ==
@@ -183,6 +183,24 @@ void validation() {
   std::memset(&gradients, 0, sizeof(gradients));
   expect(vgr_render_backward(&scene, &cameras, &outputs, &workspace, &options, &gradients, nullptr) ==
              VGR_E_INVALID_ARGUMENT, "backward without checkpoints");
+  // Tile-front lists: checkpointed forwards take no slot records, and the backward is pose-only.
+  options.ordering = VGR_ORDERING_TILE_FRONT;
+  outputs.final_transmittance = dummy_f;
+  outputs.last_blended = dummy_v;
+  workspace.slot_records = dummy_v;
+  expect(vgr_render(&scene, &cameras, &outputs, &workspace, &options, nullptr, &selector) ==
+             VGR_E_INVALID_ARGUMENT, "front forward rejects slot records");
+  workspace.slot_records = nullptr;
+  gradients.grad_rgb = dummy_f;
+  gradients.grad_viewmats = dummy_f;
+  gradients.camera_grads = dummy_f;
+  gradients.grad_means = dummy_f;
+  expect(vgr_render_backward(&scene, &cameras, &outputs, &workspace, &options, &gradients, nullptr) ==
+             VGR_E_INVALID_ARGUMENT, "front backward is pose-only");
+  std::memset(&gradients, 0, sizeof(gradients));
+  outputs.final_transmittance = nullptr;
+  outputs.last_blended = nullptr;
+  options = default_options();
   expect(vgr_covariance_backward(nullptr, nullptr, nullptr, 4, nullptr, nullptr, nullptr) ==
              VGR_E_INVALID_ARGUMENT, "covariance backward nulls");
   expect(vgr_prepare_chunked_active_ids(nullptr, 0, nullptr, nullptr, 0, 0, nullptr) == VGR_E_INVALID_ARGUMENT,

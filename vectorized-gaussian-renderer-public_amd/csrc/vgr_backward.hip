@@ -35,6 +35,8 @@
 // thread owns one Gaussian and accumulates its gradients over the group's
 // cameras in registers (one atomic per value per group); camera-pose partials
 // are reduced over the workgroup and added per camera.
+#include <type_traits>
+
 #include "vgr_composite_device.h"
 #include "vgr_project_device.h"
 
@@ -128,6 +130,309 @@ __device__ __forceinline__ float wave_transpose_sum10(const float (&v)[10], int 
 
 __device__ __forceinline__ void atomic_add_f32(float* address, float value) {
   unsafeAtomicAdd(address, value);
+}
+
+// ---- projection backward ---------------------------------------------------------
+
+// dL/d f (accumulated) and dL/d direction (returned) of the SH colour.
+template <int kDegree>
+__device__ __forceinline__ void sh_colour_backward(const float (&f)[3 * (kDegree + 1) * (kDegree + 1)], float dx,
+                                                   float dy, float dz, const float* grad_rgb,
+                                                   float (&grad_f)[3 * (kDegree + 1) * (kDegree + 1)],
+                                                   float* grad_dir) {
+  constexpr int kBands = (kDegree + 1) * (kDegree + 1);
+  grad_dir[0] = grad_dir[1] = grad_dir[2] = 0.0f;
+  if (kDegree <= 0) {
+    // Degree 0: rgb = f[0..2] with no clamp.
+    grad_f[0] += grad_rgb[0];
+    grad_f[1] += grad_rgb[1];
+    grad_f[2] += grad_rgb[2];
+    return;
+  }
+  const float norm2 = dx * dx + dy * dy + dz * dz;
+  float x = 0.0f, y = 0.0f, z = 0.0f, inverse_norm = 0.0f;
+  if (norm2 > 0.0f) {
+    inverse_norm = 1.0f / sqrtf(norm2);
+    x = dx * inverse_norm;
+    y = dy * inverse_norm;
+    z = dz * inverse_norm;
+  }
+  float basis[16];
+  basis[0] = 1.0f;
+  basis[1] = -kShC1 * y;
+  basis[2] = kShC1 * z;
+  basis[3] = -kShC1 * x;
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float xy = x * y, yz = y * z, xz = x * z;
+  if (kDegree >= 2) {
+    basis[4] = kShC2_0 * xy;
+    basis[5] = kShC2_1 * yz;
+    basis[6] = kShC2_2 * (2.0f * zz - xx - yy);
+    basis[7] = kShC2_3 * xz;
+    basis[8] = kShC2_4 * (xx - yy);
+  }
+  if (kDegree >= 3) {
+    basis[9] = kShC3_0 * y * (3.0f * xx - yy);
+    basis[10] = kShC3_1 * xy * z;
+    basis[11] = kShC3_2 * y * (4.0f * zz - xx - yy);
+    basis[12] = kShC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+    basis[13] = kShC3_4 * x * (4.0f * zz - xx - yy);
+    basis[14] = kShC3_5 * z * (xx - yy);
+    basis[15] = kShC3_6 * x * (xx - 3.0f * yy);
+  }
+  // Clamp mask of max(0, .): recompute the forward sum.
+  float g[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 1; k < kBands; ++k) acc = fmaf(basis[k], f[3 * k + c], acc);
+    g[c] = f[c] + acc > 0.0f ? grad_rgb[c] : 0.0f;
+  }
+  // dL/d basis_k = sum_c f[3k+c] g_c; dL/d f[3k+c] = basis_k g_c.
+  float gb[16];
+#pragma unroll
+  for (int k = 0; k < kBands; ++k) {
+    gb[k] = f[3 * k + 0] * g[0] + f[3 * k + 1] * g[1] + f[3 * k + 2] * g[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) grad_f[3 * k + c] += basis[k] * g[c];
+  }
+  if (norm2 <= 0.0f) return;
+  float gx = -kShC1 * gb[3];
+  float gy = -kShC1 * gb[1];
+  float gz = kShC1 * gb[2];
+  if (kDegree >= 2) {
+    gx += kShC2_0 * y * gb[4] - 2.0f * kShC2_2 * x * gb[6] + kShC2_3 * z * gb[7] + 2.0f * kShC2_4 * x * gb[8];
+    gy += kShC2_0 * x * gb[4] + kShC2_1 * z * gb[5] - 2.0f * kShC2_2 * y * gb[6] - 2.0f * kShC2_4 * y * gb[8];
+    gz += kShC2_1 * y * gb[5] + 4.0f * kShC2_2 * z * gb[6] + kShC2_3 * x * gb[7];
+  }
+  if (kDegree >= 3) {
+    gx += kShC3_0 * 6.0f * xy * gb[9] + kShC3_1 * yz * gb[10] - kShC3_2 * 2.0f * xy * gb[11] -
+          kShC3_3 * 6.0f * xz * gb[12] + kShC3_4 * (4.0f * zz - 3.0f * xx - yy) * gb[13] +
+          kShC3_5 * 2.0f * xz * gb[14] + kShC3_6 * 3.0f * (xx - yy) * gb[15];
+    gy += kShC3_0 * 3.0f * (xx - yy) * gb[9] + kShC3_1 * xz * gb[10] +
+          kShC3_2 * (4.0f * zz - xx - 3.0f * yy) * gb[11] - kShC3_3 * 6.0f * yz * gb[12] -
+          kShC3_4 * 2.0f * xy * gb[13] - kShC3_5 * 2.0f * yz * gb[14] - kShC3_6 * 6.0f * xy * gb[15];
+    gz += kShC3_1 * xy * gb[10] + kShC3_2 * 8.0f * yz * gb[11] +
+          kShC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy) * gb[12] + kShC3_4 * 8.0f * xz * gb[13] +
+          kShC3_5 * (xx - yy) * gb[14];
+  }
+  // Through the normalization u = d / |d|: dL/dd = (g - u (u . g)) / |d|.
+  const float dot = x * gx + y * gy + z * gz;
+  grad_dir[0] = (gx - x * dot) * inverse_norm;
+  grad_dir[1] = (gy - y * dot) * inverse_norm;
+  grad_dir[2] = (gz - z * dot) * inverse_norm;
+}
+
+// Gradient of one visible pair w.r.t. the Gaussian (mean, covariance, opacity)
+// and the camera (point outer product, translation, rotation, centre), given
+// the record gradient r[10].  Recomputes the forward quantities of project_pair.
+struct PairGradients {
+  float mean[3];
+  float covariance[6];
+  float opacity;
+  float camera_point[3];  // dL/d xyz_cam
+  float rotation[9];      // dL/d W, W = R_view R_env (covariance transform)
+};
+
+__device__ __forceinline__ void project_pair_backward(const CameraParams& cam, float m0, float m1, float m2,
+                                                      float opacity, const float* cov, float covariance_epsilon,
+                                                      bool antialiased, const float* r, PairGradients& out) {
+  const float* e = cam.env;
+  const float env_x = e[0] * m0 + e[1] * m1 + e[2] * m2 + e[3];
+  const float env_y = e[4] * m0 + e[5] * m1 + e[6] * m2 + e[7];
+  const float env_z = e[8] * m0 + e[9] * m1 + e[10] * m2 + e[11];
+  const float* v = cam.view;
+  const float camera_x = v[0] * env_x + v[1] * env_y + v[2] * env_z + v[3];
+  const float camera_y = v[4] * env_x + v[5] * env_y + v[6] * env_z + v[7];
+  const float camera_z = v[8] * env_x + v[9] * env_y + v[10] * env_z + v[11];
+  const float inverse_z = 1.0f / camera_z;
+  const float* w = cam.rotation;
+  // Sigma_cam = W Sigma W^T.
+  float ws[9];  // W Sigma
+#pragma unroll
+  for (int row = 0; row < 3; ++row) {
+    ws[row * 3 + 0] = w[row * 3 + 0] * cov[0] + w[row * 3 + 1] * cov[1] + w[row * 3 + 2] * cov[2];
+    ws[row * 3 + 1] = w[row * 3 + 0] * cov[1] + w[row * 3 + 1] * cov[3] + w[row * 3 + 2] * cov[4];
+    ws[row * 3 + 2] = w[row * 3 + 0] * cov[2] + w[row * 3 + 1] * cov[4] + w[row * 3 + 2] * cov[5];
+  }
+  const float c00 = ws[0] * w[0] + ws[1] * w[1] + ws[2] * w[2];
+  const float c01 = ws[0] * w[3] + ws[1] * w[4] + ws[2] * w[5];
+  const float c02 = ws[0] * w[6] + ws[1] * w[7] + ws[2] * w[8];
+  const float c11 = ws[3] * w[3] + ws[4] * w[4] + ws[5] * w[5];
+  const float c12 = ws[3] * w[6] + ws[4] * w[7] + ws[5] * w[8];
+  const float c22 = ws[6] * w[6] + ws[7] * w[7] + ws[8] * w[8];
+  const float tx = camera_x * inverse_z;
+  const float ty = camera_y * inverse_z;
+  const float clamp_x = fminf(cam.limit_x_pos, fmaxf(-cam.limit_x_neg, tx));
+  const float clamp_y = fminf(cam.limit_y_pos, fmaxf(-cam.limit_y_neg, ty));
+  const bool inside_x = clamp_x == tx;
+  const bool inside_y = clamp_y == ty;
+  const float clamped_x = camera_z * clamp_x;
+  const float clamped_y = camera_z * clamp_y;
+  const float inverse_z_squared = inverse_z * inverse_z;
+  const float fx = cam.focal_x, fy = cam.focal_y;
+  const float j00 = fx * inverse_z;
+  const float j02 = -fx * clamped_x * inverse_z_squared;
+  const float j11 = fy * inverse_z;
+  const float j12 = -fy * clamped_y * inverse_z_squared;
+  const float raw00 = j00 * j00 * c00 + 2.0f * j00 * j02 * c02 + j02 * j02 * c22;
+  const float cov01 = j00 * j11 * c01 + j00 * j12 * c02 + j02 * j11 * c12 + j02 * j12 * c22;
+  const float raw11 = j11 * j11 * c11 + 2.0f * j11 * j12 * c12 + j12 * j12 * c22;
+  const float cov00 = raw00 + covariance_epsilon;
+  const float cov11 = raw11 + covariance_epsilon;
+  const float determinant = cov00 * cov11 - cov01 * cov01;
+  const float inverse_determinant = 1.0f / determinant;
+
+  // Record opacity = clamp(opacity * compensation, 0, 1).
+  float compensation = 1.0f, ratio = 0.0f;
+  bool compensation_live = false;
+  float original = 0.0f;
+  if (antialiased) {
+    original = raw00 * raw11 - cov01 * cov01;
+    ratio = original / determinant;
+    const float floor2 = VGR_MIN_ANTIALIAS_COMPENSATION * VGR_MIN_ANTIALIAS_COMPENSATION;
+    compensation = sqrtf(fmaxf(floor2, ratio));
+    compensation_live = ratio > floor2;
+  }
+  const float effective = opacity * compensation;
+  const float g_effective = (effective >= 0.0f && effective <= 1.0f) ? r[5] : 0.0f;
+  out.opacity = g_effective * compensation;
+
+  // conic = (cov')^-1: dL/dcov' = -M G M with G = [[g0, g1/2], [g1/2, g2]].
+  const float a = cov11 * inverse_determinant;
+  const float b = -cov01 * inverse_determinant;
+  const float c = cov00 * inverse_determinant;
+  const float g0 = r[2], g1h = 0.5f * r[3], g2 = r[4];
+  const float mg00 = a * g0 + b * g1h, mg01 = a * g1h + b * g2;
+  const float mg10 = b * g0 + c * g1h, mg11 = b * g1h + c * g2;
+  float g_raw00 = -(mg00 * a + mg01 * b);
+  float g_cov01 = -2.0f * (mg00 * b + mg01 * c);
+  float g_raw11 = -(mg10 * b + mg11 * c);
+  if (compensation_live) {
+    const float g_ratio = g_effective * opacity * 0.5f / compensation;
+    const float inverse_det2 = inverse_determinant * inverse_determinant;
+    g_raw00 += g_ratio * (raw11 * determinant - original * cov11) * inverse_det2;
+    g_raw11 += g_ratio * (raw00 * determinant - original * cov00) * inverse_det2;
+    g_cov01 += g_ratio * 2.0f * cov01 * (original - determinant) * inverse_det2;
+  }
+
+  // 2D covariance = J Sigma_cam J^T: gradients of the six Sigma_cam scalars and of J.
+  const float gc00 = g_raw00 * j00 * j00;
+  const float gc01 = g_cov01 * j00 * j11;
+  const float gc02 = g_raw00 * 2.0f * j00 * j02 + g_cov01 * j00 * j12;
+  const float gc11 = g_raw11 * j11 * j11;
+  const float gc12 = g_cov01 * j02 * j11 + g_raw11 * 2.0f * j11 * j12;
+  const float gc22 = g_raw00 * j02 * j02 + g_cov01 * j02 * j12 + g_raw11 * j12 * j12;
+  const float g_j00 = g_raw00 * (2.0f * j00 * c00 + 2.0f * j02 * c02) + g_cov01 * (j11 * c01 + j12 * c02);
+  const float g_j02 = g_raw00 * (2.0f * j00 * c02 + 2.0f * j02 * c22) + g_cov01 * (j11 * c12 + j12 * c22);
+  const float g_j11 = g_raw11 * (2.0f * j11 * c11 + 2.0f * j12 * c12) + g_cov01 * (j00 * c01 + j02 * c12);
+  const float g_j12 = g_raw11 * (2.0f * j11 * c12 + 2.0f * j12 * c22) + g_cov01 * (j00 * c02 + j02 * c22);
+
+  // Camera-space point: mean2d, depth and J.
+  float g_x = r[0] * fx * inverse_z;
+  float g_y = r[1] * fy * inverse_z;
+  float g_z = -(r[0] * fx * camera_x + r[1] * fy * camera_y) * inverse_z_squared + r[6];
+  g_z -= (g_j00 * fx + g_j11 * fy) * inverse_z_squared;
+  const float inverse_z_cubed = inverse_z_squared * inverse_z;
+  g_z += 2.0f * (g_j02 * fx * clamped_x + g_j12 * fy * clamped_y) * inverse_z_cubed;
+  const float g_clamped_x = -g_j02 * fx * inverse_z_squared;
+  const float g_clamped_y = -g_j12 * fy * inverse_z_squared;
+  if (inside_x) {
+    g_x += g_clamped_x;
+  } else {
+    g_z += g_clamped_x * clamp_x;
+  }
+  if (inside_y) {
+    g_y += g_clamped_y;
+  } else {
+    g_z += g_clamped_y * clamp_y;
+  }
+  out.camera_point[0] = g_x;
+  out.camera_point[1] = g_y;
+  out.camera_point[2] = g_z;
+  // Mean: xyz_cam = V_R (E_R m + E_t) + V_t.
+  const float ge0 = v[0] * g_x + v[4] * g_y + v[8] * g_z;
+  const float ge1 = v[1] * g_x + v[5] * g_y + v[9] * g_z;
+  const float ge2 = v[2] * g_x + v[6] * g_y + v[10] * g_z;
+  out.mean[0] = e[0] * ge0 + e[4] * ge1 + e[8] * ge2;
+  out.mean[1] = e[1] * ge0 + e[5] * ge1 + e[9] * ge2;
+  out.mean[2] = e[2] * ge0 + e[6] * ge1 + e[10] * ge2;
+
+  // Sigma_cam = W Sigma W^T: dL/dSigma = W^T G W, dL/dW = 2 G W Sigma (G full symmetric).
+  const float G[9] = {gc00, 0.5f * gc01, 0.5f * gc02, 0.5f * gc01, gc11, 0.5f * gc12,
+                      0.5f * gc02, 0.5f * gc12, gc22};
+  float gw[9];  // G W
+#pragma unroll
+  for (int row = 0; row < 3; ++row)
+#pragma unroll
+    for (int col = 0; col < 3; ++col)
+      gw[row * 3 + col] = G[row * 3 + 0] * w[0 * 3 + col] + G[row * 3 + 1] * w[1 * 3 + col] +
+                          G[row * 3 + 2] * w[2 * 3 + col];
+  // W^T (G W)
+  float s[9];
+#pragma unroll
+  for (int row = 0; row < 3; ++row)
+#pragma unroll
+    for (int col = 0; col < 3; ++col)
+      s[row * 3 + col] = w[0 * 3 + row] * gw[0 * 3 + col] + w[1 * 3 + row] * gw[1 * 3 + col] +
+                         w[2 * 3 + row] * gw[2 * 3 + col];
+  out.covariance[0] = s[0];
+  out.covariance[1] = s[1] + s[3];
+  out.covariance[2] = s[2] + s[6];
+  out.covariance[3] = s[4];
+  out.covariance[4] = s[5] + s[7];
+  out.covariance[5] = s[8];
+  const float sigma[9] = {cov[0], cov[1], cov[2], cov[1], cov[3], cov[4], cov[2], cov[4], cov[5]};
+#pragma unroll
+  for (int row = 0; row < 3; ++row)
+#pragma unroll
+    for (int col = 0; col < 3; ++col)
+      out.rotation[row * 3 + col] = 2.0f * (gw[row * 3 + 0] * sigma[0 * 3 + col] + gw[row * 3 + 1] * sigma[1 * 3 + col] +
+                                            gw[row * 3 + 2] * sigma[2 * 3 + col]);
+}
+
+// Pose partials of one visible pair from its ten record terms r (mean_x, mean_y,
+// conic_0..2, opacity, depth, r, g, b), as the projection backward forms them:
+// dL/d xyz_cam outer the environment-space point (9), dL/d xyz_cam (3),
+// dL/d W (9) and minus the SH direction gradient (the camera centre, 3).
+__device__ __forceinline__ void fill_pose_terms(const CameraParams& cam, float m0, float m1, float m2,
+                                                const PairGradients& pg, const float* grad_dir,
+                                                float (&pose)[kCameraGrads]) {
+  const float* e = cam.env;
+  const float pe[3] = {e[0] * m0 + e[1] * m1 + e[2] * m2 + e[3], e[4] * m0 + e[5] * m1 + e[6] * m2 + e[7],
+                       e[8] * m0 + e[9] * m1 + e[10] * m2 + e[11]};
+#pragma unroll
+  for (int row = 0; row < 3; ++row) {
+#pragma unroll
+    for (int col = 0; col < 3; ++col) pose[row * 3 + col] = pg.camera_point[row] * pe[col];
+    pose[9 + row] = pg.camera_point[row];
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) pose[12 + k] = pg.rotation[k];
+  pose[21] = -grad_dir[0];
+  pose[22] = -grad_dir[1];
+  pose[23] = -grad_dir[2];
+}
+
+// Pose partials of (camera, packed Gaussian g) from the pair's record terms.
+template <int kDegree>
+__device__ __forceinline__ void gaussian_pose_terms(const ProjectionArgs& pa, const CameraParams& cam, int32_t g,
+                                                    const float (&r)[kRecordGrads], float (&pose)[kCameraGrads]) {
+  bool loaded;
+  float m0, m1, m2, opacity, cov[6];
+  load_gaussian(pa, g, loaded, m0, m1, m2, opacity, cov);
+  constexpr int kChannels = 3 * (kDegree + 1) * (kDegree + 1);
+  float f[kChannels];
+  load_features<kDegree>(pa.features, g, pa.feature_width, f);
+  PairGradients pg;
+  project_pair_backward(cam, m0, m1, m2, opacity, cov, pa.covariance_epsilon, pa.antialiased != 0, r, pg);
+  float grad_f[kChannels];  // scene gradient: not needed here
+#pragma unroll
+  for (int k = 0; k < kChannels; ++k) grad_f[k] = 0.0f;
+  float grad_dir[3];
+  sh_colour_backward<kDegree>(f, m0 - cam.origin_x, m1 - cam.origin_y, m2 - cam.origin_z, r + 7, grad_f, grad_dir);
+  fill_pose_terms(cam, m0, m1, m2, pg, grad_dir, pose);
 }
 
 // ---- compositing backward --------------------------------------------------------
@@ -409,6 +714,29 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_backward_kerne
   }
 }
 
+// Tile-front lists (VGR_ORDERING_TILE_FRONT) hold packed Gaussian indices: the
+// records are recomputed from the scene for the tile's camera (as the forward
+// compositor did), and the pass computes camera-pose gradients only.  Each
+// (Gaussian, tile) pair's ten terms go straight through the pair's projection
+// backward (linear in them, so summing over tiles afterwards equals the
+// per-pair sum of the global-ordering pass up to rounding); the 24 pose
+// partials are summed per lane in LDS and added once per tile.
+struct FrontBackwardArgs : CompositeBackwardArgs {
+  ProjectionArgs projection;
+  float* camera_grads;      // [VGR_POSE_REPLICAS * active_count, 24]
+  const int64_t* counters;  // the forward's counter block
+};
+
+// Device-side guard: a forward whose intersections overflowed the buffer left
+// incomplete lists, so every pose term becomes NaN instead of silently wrong.
+__device__ __forceinline__ bool front_backward_invalid(const FrontBackwardArgs& args) {
+  return args.counters[VGR_COUNTER_INTERSECTIONS] > args.intersection_capacity ||
+         args.counters[VGR_COUNTER_SORT_FAILURES] != 0;
+}
+
+template <int kDegree>
+using BackwardArgsFor = typename std::conditional<(kDegree < 0), CompositeBackwardArgs, FrontBackwardArgs>::type;
+
 // Wave-per-tile form for 16x16 tiles: one wave64 owns the whole tile and every
 // lane four pixels (column lane % 16, rows lane / 16 + 4k).  Pixel k of every
 // lane lies in the tile's k-th band of four rows, so the per-band reach mask of
@@ -421,8 +749,10 @@ __global__ __launch_bounds__(kTileSize* kTileSize) void composite_backward_kerne
 // training step on 16 cameras of 512x512 (the step ran in 4.9 ms without them).
 // Five waves per SIMD: at most 102 VGPRs (the 6.6 KB of LDS per wave would allow 24 per CU);
 // the batch is gathered at its start rather than prefetched, to stay within them.
-template <int kBatch>
-__global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(CompositeBackwardArgs args) {
+template <int kBatch, int kDegree = -1>
+__global__ __launch_bounds__(kWave, (kDegree < 0 ? 5 : (kDegree >= 2 ? 2 : 3))) void composite_backward_wave_kernel(
+    BackwardArgsFor<kDegree> args) {
+  constexpr bool kFront = kDegree >= 0;
   constexpr int kTileSize = 16;
   constexpr int kBands = 4;
   constexpr int kPerLane = kBatch / kWave;
@@ -437,6 +767,8 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
   // whether any pixel blended it.
   __shared__ __align__(16) float s_partial[kBatch][12];
   __shared__ uint32_t s_blended[kBatch];
+  // Tile-front: per-lane sums of the 24 pose partials (column = lane).
+  __shared__ float s_pose[kFront ? kCameraGrads : 1][kWave + 1];  // padded: row reads hit distinct banks
 
   const int64_t block = args.tile_order != nullptr ? args.tile_order[blockIdx.x] : blockIdx.x;
   int active;
@@ -501,8 +833,12 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
   const int64_t range_end_full = args.tile_ends[global_tile];
   if (range_start < 0 || range_end_full <= range_start) return;
   const int64_t range_end = min(range_end_full, range_start + static_cast<int64_t>(tile_last > 0 ? tile_last : 0));
-  zero_partials(args, range_end, range_end_full, lane, kWave);
+  if constexpr (!kFront) zero_partials(args, range_end, range_end_full, lane, kWave);
   if (tile_last <= 0) return;
+  if constexpr (kFront) {
+#pragma unroll
+    for (int k = 0; k < kCameraGrads; ++k) s_pose[k][lane] = 0.0f;
+  }
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
 
   const int64_t batches = (range_end - range_start + kBatch - 1) / kBatch;
@@ -511,11 +847,31 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
     const int count = static_cast<int>(range_end - batch_start < kBatch ? range_end - batch_start : kBatch);
     f32x4 next_a[kPerLane] = {}, next_b[kPerLane] = {}, next_c[kPerLane] = {};
     int32_t next_ids[kPerLane];
-    gather_batch<kWave, kBatch, kPerLane, true>(args.sorted_values, args.records, args.visible_capacity,
-                                                batch_start, range_end, lane, next_a, next_b, next_c,
-                                                args.slot_records);
-    gather_batch_slots<kWave, kBatch, kPerLane>(args.sorted_values, args.intersection_capacity, batch_start,
-                                                range_end, lane, next_ids);
+    if constexpr (kFront) {
+      // Records recomputed from the listed Gaussians; the entry's id is the Gaussian.
+#pragma unroll
+      for (int k = 0; k < kPerLane; ++k) {
+        const int64_t index = batch_start + k * kWave + lane;
+        next_ids[k] = -1;
+        if (index < range_end) {
+          const int32_t g = args.sorted_values[index];
+          if (g >= 0 && g < args.projection.gaussian_count) {
+            record_from_gaussian<kDegree>(args.projection, cam, g, next_a[k], next_b[k], next_c[k]);
+            next_ids[k] = g;
+          } else {
+            next_a[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            next_b[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            next_c[k] = f32x4{0.0f, 0.0f, __int_as_float(-1), 0.0f};
+          }
+        }
+      }
+    } else {
+      gather_batch<kWave, kBatch, kPerLane, true>(args.sorted_values, args.records, args.visible_capacity,
+                                                  batch_start, range_end, lane, next_a, next_b, next_c,
+                                                  args.slot_records);
+      gather_batch_slots<kWave, kBatch, kPerLane>(args.sorted_values, args.intersection_capacity, batch_start,
+                                                  range_end, lane, next_ids);
+    }
     __syncthreads();  // previous batch fully consumed
     uint32_t list_count = 0;
 #pragma unroll
@@ -631,10 +987,12 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
     for (int k = 0; k < kPerLane; ++k) {
       const int slot = k * kWave + lane;
       if (slot >= count) continue;
-      const int32_t id = s_record[slot];  // intersection slot of the entry
+      const int32_t id = s_record[slot];  // intersection slot (tile-front: Gaussian) of the entry
       if (id < 0) continue;
       if (s_blended[slot] == 0u) {
-        store_partial(args.partials, id, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (!kFront) {
+          store_partial(args.partials, id, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f);
+        }
         continue;
       }
       const f32x4* partial = reinterpret_cast<const f32x4*>(s_partial[slot]);
@@ -655,269 +1013,33 @@ __global__ __launch_bounds__(kWave, 5) void composite_backward_wave_kernel(Compo
         d_c1 = -o * p0.w;
         d_c2 = -0.5f * o * p1.x;
       }
-      store_partial(args.partials, id, d_mx, d_my, d_c0, d_c1, d_c2, p1.y, p1.z, p1.w, p2.x, p2.y);
+      if constexpr (kFront) {
+        float r[kRecordGrads] = {d_mx, d_my, d_c0, d_c1, d_c2, p1.y, p1.z, p1.w, p2.x, p2.y};
+        if (front_backward_invalid(args)) {
+#pragma unroll
+          for (int t = 0; t < kRecordGrads; ++t) r[t] = __int_as_float(0x7fc00000);
+        }
+        float pose[kCameraGrads];
+        gaussian_pose_terms<kDegree>(args.projection, cam, id, r, pose);
+#pragma unroll
+        for (int t = 0; t < kCameraGrads; ++t) s_pose[t][lane] += pose[t];
+      } else {
+        store_partial(args.partials, id, d_mx, d_my, d_c0, d_c1, d_c2, p1.y, p1.z, p1.w, p2.x, p2.y);
+      }
     }
   }
-}
-
-// ---- projection backward ---------------------------------------------------------
-
-// dL/d f (accumulated) and dL/d direction (returned) of the SH colour.
-template <int kDegree>
-__device__ __forceinline__ void sh_colour_backward(const float (&f)[3 * (kDegree + 1) * (kDegree + 1)], float dx,
-                                                   float dy, float dz, const float* grad_rgb,
-                                                   float (&grad_f)[3 * (kDegree + 1) * (kDegree + 1)],
-                                                   float* grad_dir) {
-  constexpr int kBands = (kDegree + 1) * (kDegree + 1);
-  grad_dir[0] = grad_dir[1] = grad_dir[2] = 0.0f;
-  if (kDegree <= 0) {
-    // Degree 0: rgb = f[0..2] with no clamp.
-    grad_f[0] += grad_rgb[0];
-    grad_f[1] += grad_rgb[1];
-    grad_f[2] += grad_rgb[2];
-    return;
+  if constexpr (kFront) {
+    // Lane t < 24 sums pose partial t over the 64 lanes' columns and adds it to
+    // the camera's row in replica (tile % VGR_POSE_REPLICAS).
+    __syncthreads();
+    if (lane < kCameraGrads) {
+      float sum = 0.0f;
+      for (int column = 0; column < kWave; ++column) sum += s_pose[lane][column];
+      const int64_t replica = global_tile % VGR_POSE_REPLICAS;
+      if (sum != 0.0f) atomic_add_f32(args.camera_grads + (replica * args.active_count + active) * kCameraGrads + lane,
+                                      sum);
+    }
   }
-  const float norm2 = dx * dx + dy * dy + dz * dz;
-  float x = 0.0f, y = 0.0f, z = 0.0f, inverse_norm = 0.0f;
-  if (norm2 > 0.0f) {
-    inverse_norm = 1.0f / sqrtf(norm2);
-    x = dx * inverse_norm;
-    y = dy * inverse_norm;
-    z = dz * inverse_norm;
-  }
-  float basis[16];
-  basis[0] = 1.0f;
-  basis[1] = -kShC1 * y;
-  basis[2] = kShC1 * z;
-  basis[3] = -kShC1 * x;
-  const float xx = x * x, yy = y * y, zz = z * z;
-  const float xy = x * y, yz = y * z, xz = x * z;
-  if (kDegree >= 2) {
-    basis[4] = kShC2_0 * xy;
-    basis[5] = kShC2_1 * yz;
-    basis[6] = kShC2_2 * (2.0f * zz - xx - yy);
-    basis[7] = kShC2_3 * xz;
-    basis[8] = kShC2_4 * (xx - yy);
-  }
-  if (kDegree >= 3) {
-    basis[9] = kShC3_0 * y * (3.0f * xx - yy);
-    basis[10] = kShC3_1 * xy * z;
-    basis[11] = kShC3_2 * y * (4.0f * zz - xx - yy);
-    basis[12] = kShC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
-    basis[13] = kShC3_4 * x * (4.0f * zz - xx - yy);
-    basis[14] = kShC3_5 * z * (xx - yy);
-    basis[15] = kShC3_6 * x * (xx - 3.0f * yy);
-  }
-  // Clamp mask of max(0, .): recompute the forward sum.
-  float g[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    float acc = 0.0f;
-#pragma unroll
-    for (int k = 1; k < kBands; ++k) acc = fmaf(basis[k], f[3 * k + c], acc);
-    g[c] = f[c] + acc > 0.0f ? grad_rgb[c] : 0.0f;
-  }
-  // dL/d basis_k = sum_c f[3k+c] g_c; dL/d f[3k+c] = basis_k g_c.
-  float gb[16];
-#pragma unroll
-  for (int k = 0; k < kBands; ++k) {
-    gb[k] = f[3 * k + 0] * g[0] + f[3 * k + 1] * g[1] + f[3 * k + 2] * g[2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) grad_f[3 * k + c] += basis[k] * g[c];
-  }
-  if (norm2 <= 0.0f) return;
-  float gx = -kShC1 * gb[3];
-  float gy = -kShC1 * gb[1];
-  float gz = kShC1 * gb[2];
-  if (kDegree >= 2) {
-    gx += kShC2_0 * y * gb[4] - 2.0f * kShC2_2 * x * gb[6] + kShC2_3 * z * gb[7] + 2.0f * kShC2_4 * x * gb[8];
-    gy += kShC2_0 * x * gb[4] + kShC2_1 * z * gb[5] - 2.0f * kShC2_2 * y * gb[6] - 2.0f * kShC2_4 * y * gb[8];
-    gz += kShC2_1 * y * gb[5] + 4.0f * kShC2_2 * z * gb[6] + kShC2_3 * x * gb[7];
-  }
-  if (kDegree >= 3) {
-    gx += kShC3_0 * 6.0f * xy * gb[9] + kShC3_1 * yz * gb[10] - kShC3_2 * 2.0f * xy * gb[11] -
-          kShC3_3 * 6.0f * xz * gb[12] + kShC3_4 * (4.0f * zz - 3.0f * xx - yy) * gb[13] +
-          kShC3_5 * 2.0f * xz * gb[14] + kShC3_6 * 3.0f * (xx - yy) * gb[15];
-    gy += kShC3_0 * 3.0f * (xx - yy) * gb[9] + kShC3_1 * xz * gb[10] +
-          kShC3_2 * (4.0f * zz - xx - 3.0f * yy) * gb[11] - kShC3_3 * 6.0f * yz * gb[12] -
-          kShC3_4 * 2.0f * xy * gb[13] - kShC3_5 * 2.0f * yz * gb[14] - kShC3_6 * 6.0f * xy * gb[15];
-    gz += kShC3_1 * xy * gb[10] + kShC3_2 * 8.0f * yz * gb[11] +
-          kShC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy) * gb[12] + kShC3_4 * 8.0f * xz * gb[13] +
-          kShC3_5 * (xx - yy) * gb[14];
-  }
-  // Through the normalization u = d / |d|: dL/dd = (g - u (u . g)) / |d|.
-  const float dot = x * gx + y * gy + z * gz;
-  grad_dir[0] = (gx - x * dot) * inverse_norm;
-  grad_dir[1] = (gy - y * dot) * inverse_norm;
-  grad_dir[2] = (gz - z * dot) * inverse_norm;
-}
-
-// Gradient of one visible pair w.r.t. the Gaussian (mean, covariance, opacity)
-// and the camera (point outer product, translation, rotation, centre), given
-// the record gradient r[10].  Recomputes the forward quantities of project_pair.
-struct PairGradients {
-  float mean[3];
-  float covariance[6];
-  float opacity;
-  float camera_point[3];  // dL/d xyz_cam
-  float rotation[9];      // dL/d W, W = R_view R_env (covariance transform)
-};
-
-__device__ __forceinline__ void project_pair_backward(const CameraParams& cam, float m0, float m1, float m2,
-                                                      float opacity, const float* cov, float covariance_epsilon,
-                                                      bool antialiased, const float* r, PairGradients& out) {
-  const float* e = cam.env;
-  const float env_x = e[0] * m0 + e[1] * m1 + e[2] * m2 + e[3];
-  const float env_y = e[4] * m0 + e[5] * m1 + e[6] * m2 + e[7];
-  const float env_z = e[8] * m0 + e[9] * m1 + e[10] * m2 + e[11];
-  const float* v = cam.view;
-  const float camera_x = v[0] * env_x + v[1] * env_y + v[2] * env_z + v[3];
-  const float camera_y = v[4] * env_x + v[5] * env_y + v[6] * env_z + v[7];
-  const float camera_z = v[8] * env_x + v[9] * env_y + v[10] * env_z + v[11];
-  const float inverse_z = 1.0f / camera_z;
-  const float* w = cam.rotation;
-  // Sigma_cam = W Sigma W^T.
-  float ws[9];  // W Sigma
-#pragma unroll
-  for (int row = 0; row < 3; ++row) {
-    ws[row * 3 + 0] = w[row * 3 + 0] * cov[0] + w[row * 3 + 1] * cov[1] + w[row * 3 + 2] * cov[2];
-    ws[row * 3 + 1] = w[row * 3 + 0] * cov[1] + w[row * 3 + 1] * cov[3] + w[row * 3 + 2] * cov[4];
-    ws[row * 3 + 2] = w[row * 3 + 0] * cov[2] + w[row * 3 + 1] * cov[4] + w[row * 3 + 2] * cov[5];
-  }
-  const float c00 = ws[0] * w[0] + ws[1] * w[1] + ws[2] * w[2];
-  const float c01 = ws[0] * w[3] + ws[1] * w[4] + ws[2] * w[5];
-  const float c02 = ws[0] * w[6] + ws[1] * w[7] + ws[2] * w[8];
-  const float c11 = ws[3] * w[3] + ws[4] * w[4] + ws[5] * w[5];
-  const float c12 = ws[3] * w[6] + ws[4] * w[7] + ws[5] * w[8];
-  const float c22 = ws[6] * w[6] + ws[7] * w[7] + ws[8] * w[8];
-  const float tx = camera_x * inverse_z;
-  const float ty = camera_y * inverse_z;
-  const float clamp_x = fminf(cam.limit_x_pos, fmaxf(-cam.limit_x_neg, tx));
-  const float clamp_y = fminf(cam.limit_y_pos, fmaxf(-cam.limit_y_neg, ty));
-  const bool inside_x = clamp_x == tx;
-  const bool inside_y = clamp_y == ty;
-  const float clamped_x = camera_z * clamp_x;
-  const float clamped_y = camera_z * clamp_y;
-  const float inverse_z_squared = inverse_z * inverse_z;
-  const float fx = cam.focal_x, fy = cam.focal_y;
-  const float j00 = fx * inverse_z;
-  const float j02 = -fx * clamped_x * inverse_z_squared;
-  const float j11 = fy * inverse_z;
-  const float j12 = -fy * clamped_y * inverse_z_squared;
-  const float raw00 = j00 * j00 * c00 + 2.0f * j00 * j02 * c02 + j02 * j02 * c22;
-  const float cov01 = j00 * j11 * c01 + j00 * j12 * c02 + j02 * j11 * c12 + j02 * j12 * c22;
-  const float raw11 = j11 * j11 * c11 + 2.0f * j11 * j12 * c12 + j12 * j12 * c22;
-  const float cov00 = raw00 + covariance_epsilon;
-  const float cov11 = raw11 + covariance_epsilon;
-  const float determinant = cov00 * cov11 - cov01 * cov01;
-  const float inverse_determinant = 1.0f / determinant;
-
-  // Record opacity = clamp(opacity * compensation, 0, 1).
-  float compensation = 1.0f, ratio = 0.0f;
-  bool compensation_live = false;
-  float original = 0.0f;
-  if (antialiased) {
-    original = raw00 * raw11 - cov01 * cov01;
-    ratio = original / determinant;
-    const float floor2 = VGR_MIN_ANTIALIAS_COMPENSATION * VGR_MIN_ANTIALIAS_COMPENSATION;
-    compensation = sqrtf(fmaxf(floor2, ratio));
-    compensation_live = ratio > floor2;
-  }
-  const float effective = opacity * compensation;
-  const float g_effective = (effective >= 0.0f && effective <= 1.0f) ? r[5] : 0.0f;
-  out.opacity = g_effective * compensation;
-
-  // conic = (cov')^-1: dL/dcov' = -M G M with G = [[g0, g1/2], [g1/2, g2]].
-  const float a = cov11 * inverse_determinant;
-  const float b = -cov01 * inverse_determinant;
-  const float c = cov00 * inverse_determinant;
-  const float g0 = r[2], g1h = 0.5f * r[3], g2 = r[4];
-  const float mg00 = a * g0 + b * g1h, mg01 = a * g1h + b * g2;
-  const float mg10 = b * g0 + c * g1h, mg11 = b * g1h + c * g2;
-  float g_raw00 = -(mg00 * a + mg01 * b);
-  float g_cov01 = -2.0f * (mg00 * b + mg01 * c);
-  float g_raw11 = -(mg10 * b + mg11 * c);
-  if (compensation_live) {
-    const float g_ratio = g_effective * opacity * 0.5f / compensation;
-    const float inverse_det2 = inverse_determinant * inverse_determinant;
-    g_raw00 += g_ratio * (raw11 * determinant - original * cov11) * inverse_det2;
-    g_raw11 += g_ratio * (raw00 * determinant - original * cov00) * inverse_det2;
-    g_cov01 += g_ratio * 2.0f * cov01 * (original - determinant) * inverse_det2;
-  }
-
-  // 2D covariance = J Sigma_cam J^T: gradients of the six Sigma_cam scalars and of J.
-  const float gc00 = g_raw00 * j00 * j00;
-  const float gc01 = g_cov01 * j00 * j11;
-  const float gc02 = g_raw00 * 2.0f * j00 * j02 + g_cov01 * j00 * j12;
-  const float gc11 = g_raw11 * j11 * j11;
-  const float gc12 = g_cov01 * j02 * j11 + g_raw11 * 2.0f * j11 * j12;
-  const float gc22 = g_raw00 * j02 * j02 + g_cov01 * j02 * j12 + g_raw11 * j12 * j12;
-  const float g_j00 = g_raw00 * (2.0f * j00 * c00 + 2.0f * j02 * c02) + g_cov01 * (j11 * c01 + j12 * c02);
-  const float g_j02 = g_raw00 * (2.0f * j00 * c02 + 2.0f * j02 * c22) + g_cov01 * (j11 * c12 + j12 * c22);
-  const float g_j11 = g_raw11 * (2.0f * j11 * c11 + 2.0f * j12 * c12) + g_cov01 * (j00 * c01 + j02 * c12);
-  const float g_j12 = g_raw11 * (2.0f * j11 * c12 + 2.0f * j12 * c22) + g_cov01 * (j00 * c02 + j02 * c22);
-
-  // Camera-space point: mean2d, depth and J.
-  float g_x = r[0] * fx * inverse_z;
-  float g_y = r[1] * fy * inverse_z;
-  float g_z = -(r[0] * fx * camera_x + r[1] * fy * camera_y) * inverse_z_squared + r[6];
-  g_z -= (g_j00 * fx + g_j11 * fy) * inverse_z_squared;
-  const float inverse_z_cubed = inverse_z_squared * inverse_z;
-  g_z += 2.0f * (g_j02 * fx * clamped_x + g_j12 * fy * clamped_y) * inverse_z_cubed;
-  const float g_clamped_x = -g_j02 * fx * inverse_z_squared;
-  const float g_clamped_y = -g_j12 * fy * inverse_z_squared;
-  if (inside_x) {
-    g_x += g_clamped_x;
-  } else {
-    g_z += g_clamped_x * clamp_x;
-  }
-  if (inside_y) {
-    g_y += g_clamped_y;
-  } else {
-    g_z += g_clamped_y * clamp_y;
-  }
-  out.camera_point[0] = g_x;
-  out.camera_point[1] = g_y;
-  out.camera_point[2] = g_z;
-  // Mean: xyz_cam = V_R (E_R m + E_t) + V_t.
-  const float ge0 = v[0] * g_x + v[4] * g_y + v[8] * g_z;
-  const float ge1 = v[1] * g_x + v[5] * g_y + v[9] * g_z;
-  const float ge2 = v[2] * g_x + v[6] * g_y + v[10] * g_z;
-  out.mean[0] = e[0] * ge0 + e[4] * ge1 + e[8] * ge2;
-  out.mean[1] = e[1] * ge0 + e[5] * ge1 + e[9] * ge2;
-  out.mean[2] = e[2] * ge0 + e[6] * ge1 + e[10] * ge2;
-
-  // Sigma_cam = W Sigma W^T: dL/dSigma = W^T G W, dL/dW = 2 G W Sigma (G full symmetric).
-  const float G[9] = {gc00, 0.5f * gc01, 0.5f * gc02, 0.5f * gc01, gc11, 0.5f * gc12,
-                      0.5f * gc02, 0.5f * gc12, gc22};
-  float gw[9];  // G W
-#pragma unroll
-  for (int row = 0; row < 3; ++row)
-#pragma unroll
-    for (int col = 0; col < 3; ++col)
-      gw[row * 3 + col] = G[row * 3 + 0] * w[0 * 3 + col] + G[row * 3 + 1] * w[1 * 3 + col] +
-                          G[row * 3 + 2] * w[2 * 3 + col];
-  // W^T (G W)
-  float s[9];
-#pragma unroll
-  for (int row = 0; row < 3; ++row)
-#pragma unroll
-    for (int col = 0; col < 3; ++col)
-      s[row * 3 + col] = w[0 * 3 + row] * gw[0 * 3 + col] + w[1 * 3 + row] * gw[1 * 3 + col] +
-                         w[2 * 3 + row] * gw[2 * 3 + col];
-  out.covariance[0] = s[0];
-  out.covariance[1] = s[1] + s[3];
-  out.covariance[2] = s[2] + s[6];
-  out.covariance[3] = s[4];
-  out.covariance[4] = s[5] + s[7];
-  out.covariance[5] = s[8];
-  const float sigma[9] = {cov[0], cov[1], cov[2], cov[1], cov[3], cov[4], cov[2], cov[4], cov[5]};
-#pragma unroll
-  for (int row = 0; row < 3; ++row)
-#pragma unroll
-    for (int col = 0; col < 3; ++col)
-      out.rotation[row * 3 + col] = 2.0f * (gw[row * 3 + 0] * sigma[0 * 3 + col] + gw[row * 3 + 1] * sigma[1 * 3 + col] +
-                                            gw[row * 3 + 2] * sigma[2 * 3 + col]);
 }
 
 struct ProjectionBackwardArgs {
@@ -1040,22 +1162,7 @@ __global__ __launch_bounds__(kBlockThreads) void projection_backward_kernel(Proj
           for (int k = 0; k < 6; ++k) grad_cov[k] += pg.covariance[k];
           grad_opacity += pg.opacity;
         }
-        if (kPose) {
-          const float* e = cam.env;
-          const float pe[3] = {e[0] * m0 + e[1] * m1 + e[2] * m2 + e[3], e[4] * m0 + e[5] * m1 + e[6] * m2 + e[7],
-                               e[8] * m0 + e[9] * m1 + e[10] * m2 + e[11]};
-#pragma unroll
-          for (int row = 0; row < 3; ++row) {
-#pragma unroll
-            for (int col = 0; col < 3; ++col) pose[row * 3 + col] = pg.camera_point[row] * pe[col];
-            pose[9 + row] = pg.camera_point[row];
-          }
-#pragma unroll
-          for (int k = 0; k < 9; ++k) pose[12 + k] = pg.rotation[k];
-          pose[21] = -grad_dir[0];
-          pose[22] = -grad_dir[1];
-          pose[23] = -grad_dir[2];
-        }
+        if (kPose) fill_pose_terms(cam, m0, m1, m2, pg, grad_dir, pose);
       }
     }
     // Block-uniform test (the barrier below needs every wave).
@@ -1300,6 +1407,45 @@ int launch_backward(const vgr_scene& scene, const vgr_cameras& cameras, const vg
   if (tile_blocks > 0x7fffffffLL) {
     set_error("backward composite grid of %lld tiles exceeds the launch limit", static_cast<long long>(tile_blocks));
     return VGR_E_INVALID_ARGUMENT;
+  }
+  if (options.ordering == VGR_ORDERING_TILE_FRONT) {
+    // Pose gradients only, fused into the compositing backward (no record rows,
+    // no projection pass).  Grids the forward ordered longest list first reuse
+    // its order; larger ones run in natural order.
+    FrontBackwardArgs fargs;
+    static_cast<CompositeBackwardArgs&>(fargs) = cargs;
+    make_projection_args(scene, cameras, options, layout, scratch, fargs.projection);
+    fargs.camera_grads = grads.camera_grads;
+    fargs.counters = ws.counters;
+    fargs.tile_order = tile_blocks <= kOrderedCompositeBlocks
+                           ? reinterpret_cast<const int32_t*>(scratch + layout.tile_order)
+                           : nullptr;
+    if (tile_blocks > 0) {
+      const dim3 grid(static_cast<unsigned>(tile_blocks)), threads(kWave);
+      switch (scene.sh_degree) {
+        case 1:
+          hipLaunchKernelGGL((composite_backward_wave_kernel<64, 1>), grid, threads, 0, stream, fargs);
+          break;
+        case 2:
+          hipLaunchKernelGGL((composite_backward_wave_kernel<64, 2>), grid, threads, 0, stream, fargs);
+          break;
+        case 3:
+          hipLaunchKernelGGL((composite_backward_wave_kernel<64, 3>), grid, threads, 0, stream, fargs);
+          break;
+        default:
+          hipLaunchKernelGGL((composite_backward_wave_kernel<64, 0>), grid, threads, 0, stream, fargs);
+          break;
+      }
+    }
+    const int finalize_blocks = (cameras.active_count + 63) / 64;
+    hipLaunchKernelGGL(camera_grads_finalize_kernel, dim3(finalize_blocks), dim3(64), 0, stream, table,
+                       cameras.active_count, grads.camera_grads, grads.grad_viewmats);
+    const hipError_t status = hipGetLastError();
+    if (status != hipSuccess) {
+      set_error("backward launch failed: %s", hipGetErrorString(status));
+      return VGR_E_LAUNCH;
+    }
+    return 0;
   }
   if (tile_blocks > 0) {
     if (options.tile_size == 8) {

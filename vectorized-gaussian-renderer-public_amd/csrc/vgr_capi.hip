@@ -430,9 +430,9 @@ int prepare_render(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outpu
     set_error("visible capacity must be in [1, INT32_MAX]; got %lld", static_cast<long long>(ws.visible_capacity));
     return VGR_E_INVALID_ARGUMENT;
   }
-  // The radix sort's look-back flags carry 30-bit prefixes; the tile-front
-  // buckets are addressed with int32 tile starts.
-  const int64_t intersection_limit = opt.ordering == VGR_ORDERING_TILE_FRONT ? 0x7fffffffLL : (1LL << 30) - 1;
+  // Both orderings address intersections with int32 positions (tile starts and
+  // ends, the radix sort's 31-bit look-back prefixes): at most 2^31 - 1.
+  const int64_t intersection_limit = 0x7fffffffLL;
   if (ws.intersection_capacity <= 0 || ws.intersection_capacity > intersection_limit) {
     set_error("intersection capacity must be in [1, %lld]; got %lld", static_cast<long long>(intersection_limit),
               static_cast<long long>(ws.intersection_capacity));
@@ -449,6 +449,11 @@ int prepare_render(const vgr_scene& sc, const vgr_cameras& cams, const vgr_outpu
   }
   if (opt.tile_size == 1 && opt.depth_bucket_count > 0 && ws.radii == nullptr) {
     set_error("the depth cutoff (tile_size 1, depth_bucket_count > 0) needs workspace.radii");
+    return VGR_E_INVALID_ARGUMENT;
+  }
+  if (ws.slot_records != nullptr && opt.ordering == VGR_ORDERING_TILE_FRONT) {
+    set_error("tile-front lists hold Gaussians, not intersection slots: workspace.slot_records must be NULL "
+              "(its checkpointed forward feeds the pose-only backward)");
     return VGR_E_INVALID_ARGUMENT;
   }
   if (ws.slot_records != nullptr && sc.gaussian_ids != nullptr) {
@@ -676,16 +681,26 @@ extern "C" int32_t vgr_render_backward(const vgr_scene* scene, const vgr_cameras
     set_error("backward needs the forward's transmittance checkpoints, full sensor output and linear RGB");
     return VGR_E_INVALID_ARGUMENT;
   }
-  if (grads.grad_rgb == nullptr || grads.record_grads == nullptr) {
-    set_error("backward needs grad_rgb and the record_grads workspace");
-    return VGR_E_INVALID_ARGUMENT;
-  }
-  if (ws.slot_records == nullptr) {
-    set_error("backward needs the forward's workspace.slot_records (slot-sorted training forward)");
-    return VGR_E_INVALID_ARGUMENT;
-  }
+  const bool front = resolved.ordering == VGR_ORDERING_TILE_FRONT;
   const bool scene_grads = grads.grad_means != nullptr || grads.grad_covariances != nullptr ||
                            grads.grad_opacities != nullptr || grads.grad_features != nullptr;
+  if (front) {
+    // Tile-front lists carry no record slots: the pass computes pose gradients only.
+    if (grads.grad_rgb == nullptr || grads.grad_viewmats == nullptr || scene_grads) {
+      set_error("tile-front backward computes camera-pose gradients only: it needs grad_rgb and grad_viewmats, "
+                "and scene gradients must be NULL");
+      return VGR_E_INVALID_ARGUMENT;
+    }
+  } else {
+    if (grads.grad_rgb == nullptr || grads.record_grads == nullptr) {
+      set_error("backward needs grad_rgb and the record_grads workspace");
+      return VGR_E_INVALID_ARGUMENT;
+    }
+    if (ws.slot_records == nullptr) {
+      set_error("backward needs the forward's workspace.slot_records (slot-sorted training forward)");
+      return VGR_E_INVALID_ARGUMENT;
+    }
+  }
   if (scene_grads && (grads.grad_means == nullptr || grads.grad_covariances == nullptr ||
                       grads.grad_opacities == nullptr || grads.grad_features == nullptr)) {
     set_error("scene gradients are all given or all NULL");

@@ -142,34 +142,6 @@ __device__ __forceinline__ int64_t item_total(const Args& args) {
   return args.total_items;
 }
 
-// The 48-byte record of (camera, packed Gaussian g), recomputed with the
-// projection write kernel's arithmetic (same functions, same operation order),
-// so it is bit-identical to the record that kernel would store.
-template <int kDegree>
-__device__ __forceinline__ void record_from_gaussian(const ProjectionArgs& pa, const CameraParams& cam, int32_t g,
-                                                     f32x4& a, f32x4& b, f32x4& c) {
-  bool loaded;
-  float m0, m1, m2, opacity, cov[6];
-  load_gaussian(pa, g, loaded, m0, m1, m2, opacity, cov);
-  Projected p;
-  if (!loaded || !project_pair(cam, m0, m1, m2, opacity, cov, pa.near_plane, pa.far_plane, pa.support_sigma,
-                               pa.covariance_epsilon, pa.antialiased != 0, p)) {
-    a = f32x4{0.0f, 0.0f, 0.0f, 0.0f};  // unreachable for listed pairs: contributes nothing
-    b = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    c = f32x4{0.0f, 0.0f, __int_as_float(-1), 0.0f};
-    return;
-  }
-  constexpr int kChannels = 3 * (kDegree + 1) * (kDegree + 1);
-  float f[kChannels];
-  load_features<kDegree>(pa.features, g, pa.feature_width, f);
-  float rgb[3];
-  sh_colour<kDegree>(f, m0 - cam.origin_x, m1 - cam.origin_y, m2 - cam.origin_z, rgb);
-  const int32_t id = pa.gaussian_ids != nullptr ? pa.gaussian_ids[g] : g;
-  a = f32x4{p.mean_x, p.mean_y, p.conic_0, p.conic_1};
-  b = f32x4{p.conic_2, p.opacity, p.depth, rgb[0]};
-  c = f32x4{rgb[1], rgb[2], __int_as_float(id), __int_as_float(cam.camera)};
-}
-
 // Gathers one batch: stored records (kDegree < 0) or records recomputed from
 // the listed Gaussians (kDegree = SH degree, tile-front lists).
 template <int kThreads, int kBatch, int kPerThread, int kDegree, bool kSlots = false>
@@ -502,6 +474,9 @@ __global__ __launch_bounds__(kRaster* kRaster) void composite_ray_kernel(Composi
   __shared__ f32x4 s_b[kThreads];
   __shared__ f32x4 s_c[kThreads];
   __shared__ f32x4 s_colour[kThreads];  // red, green, blue, gaussian id bits
+  __shared__ uint32_t s_waves[kThreads];  // waves whose pixel rows the entry's cull box reaches
+  constexpr int kRowsPerWave = kWave / kRaster;
+  const int wave = static_cast<int>(threadIdx.x) / kWave;
   const int64_t block = blockIdx.x;
   const int active = static_cast<int>(block / raster_tiles);
   if (active >= args.active_count) return;
@@ -535,21 +510,37 @@ __global__ __launch_bounds__(kRaster* kRaster) void composite_ray_kernel(Composi
         if (record >= 0 && record < args.visible_capacity) {
           const f32x4* rec = reinterpret_cast<const f32x4*>(args.records + record);
           const f32x4* ray = reinterpret_cast<const f32x4*>(ray_records + static_cast<int64_t>(record) * 12);
-          const f32x4 r1 = rec[1], r2 = rec[2], q2 = ray[2];
+          const f32x4 r0 = rec[0], r1 = rec[1], r2 = rec[2], q2 = ray[2];
           s_a[threadIdx.x] = ray[0];
           s_b[threadIdx.x] = ray[1];
           s_c[threadIdx.x] = f32x4{q2.x, q2.y, r1.y, r1.z};
           s_colour[threadIdx.x] = f32x4{r1.w, r2.x, r2.y, r2.z};
+          // Rows of pixel centres each wave covers against the entry's cull box
+          // (centre r0.xy, half extents q2.zw): a wave outside it has alpha 0 at
+          // every pixel, so it skips the entry.
+          const float x0 = static_cast<float>(tile_x * kRaster) + 0.5f;
+          uint32_t waves = 0u;
+          if (fabsf(r0.x - fminf(fmaxf(r0.x, x0), x0 + static_cast<float>(kRaster - 1))) <= q2.z) {
+#pragma unroll
+            for (int w = 0; w < kThreads / kWave; ++w) {
+              const float y0 = static_cast<float>(tile_y * kRaster + w * kRowsPerWave) + 0.5f;
+              const float y1 = y0 + static_cast<float>(kRowsPerWave - 1);
+              if (fabsf(r0.y - fminf(fmaxf(r0.y, y0), y1)) <= q2.w) waves |= 1u << w;
+            }
+          }
+          s_waves[threadIdx.x] = waves;
         } else {
           // Only reachable with a corrupted list: an entry that never blends.
           s_a[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
           s_b[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
           s_c[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
           s_colour[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, __int_as_float(-1)};
+          s_waves[threadIdx.x] = 0u;
         }
       }
       __syncthreads();
       for (int item = 0; !done && item < count; ++item) {
+        if (((s_waves[item] >> wave) & 1u) == 0u) continue;  // uniform over the wave
         const f32x4 c = s_c[item];
         const float alpha = ray_alpha(ray_x, ray_y, s_a[item], s_b[item], c, support_sigma);
         if (!(alpha > 0.0f)) continue;
@@ -663,7 +654,10 @@ void launch_variant(const CompositeArgsFor<kDegree>& args, int64_t blocks, int p
 template <int kDegree>
 void launch_projected_degree(bool full, const FrontCompositeArgs& args, int64_t blocks, int persistent,
                              hipStream_t stream) {
-  if (args.statistics != nullptr) {
+  if (args.final_transmittance != nullptr) {
+    // Checkpointed (pose-training) forward: final transmittance and last blended position.
+    launch_variant<16, 256, true, false, true, kDegree>(args, blocks, persistent, stream);
+  } else if (args.statistics != nullptr) {
     if (full) {
       launch_variant<16, 256, true, true, false, kDegree>(args, blocks, persistent, stream);
     } else {
@@ -769,13 +763,13 @@ int launch_composite(const vgr_scene& scene, const vgr_cameras& cameras, const v
   args.final_transmittance = outputs.final_transmittance;
   args.last_blended = outputs.last_blended;
   args.slot_records = ws.slot_records;
-  if (args.final_transmittance != nullptr && args.slot_records == nullptr) {
+  const bool front = options.ordering == VGR_ORDERING_TILE_FRONT;
+  if (args.final_transmittance != nullptr && args.slot_records == nullptr && !front) {
     set_error("training forwards (transmittance checkpoints) need workspace.slot_records");
     return VGR_E_INVALID_ARGUMENT;
   }
-  const bool front = options.ordering == VGR_ORDERING_TILE_FRONT;
-  if (front && (args.final_transmittance != nullptr || options.tile_size != 16)) {
-    set_error("tile-front ordering composites 16x16 tiles without training checkpoints");
+  if (front && options.tile_size != 16) {
+    set_error("tile-front ordering composites 16x16 tiles");
     return VGR_E_INVALID_ARGUMENT;
   }
   if (args.final_transmittance != nullptr &&

@@ -7,6 +7,7 @@
 #define VGR_COMPOSITE_DEVICE_H
 
 #include "vgr_internal.h"
+#include "vgr_project_device.h"
 
 namespace vgr {
 namespace dev {
@@ -212,6 +213,34 @@ __device__ __forceinline__ void build_wave_lists(WaveLists<kTileSize, kBatch>& s
     }
   }
   __syncthreads();
+}
+
+// The 48-byte record of (camera, packed Gaussian g), recomputed with the
+// projection write kernel's arithmetic (same functions, same operation order),
+// so it is bit-identical to the record that kernel would store.
+template <int kDegree>
+__device__ __forceinline__ void record_from_gaussian(const ProjectionArgs& pa, const CameraParams& cam, int32_t g,
+                                                     f32x4& a, f32x4& b, f32x4& c) {
+  bool loaded;
+  float m0, m1, m2, opacity, cov[6];
+  load_gaussian(pa, g, loaded, m0, m1, m2, opacity, cov);
+  Projected p;
+  if (!loaded || !project_pair(cam, m0, m1, m2, opacity, cov, pa.near_plane, pa.far_plane, pa.support_sigma,
+                               pa.covariance_epsilon, pa.antialiased != 0, p)) {
+    a = f32x4{0.0f, 0.0f, 0.0f, 0.0f};  // unreachable for listed pairs: contributes nothing
+    b = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    c = f32x4{0.0f, 0.0f, __int_as_float(-1), 0.0f};
+    return;
+  }
+  constexpr int kChannels = 3 * (kDegree + 1) * (kDegree + 1);
+  float f[kChannels];
+  load_features<kDegree>(pa.features, g, pa.feature_width, f);
+  float rgb[3];
+  sh_colour<kDegree>(f, m0 - cam.origin_x, m1 - cam.origin_y, m2 - cam.origin_z, rgb);
+  const int32_t id = pa.gaussian_ids != nullptr ? pa.gaussian_ids[g] : g;
+  a = f32x4{p.mean_x, p.mean_y, p.conic_0, p.conic_1};
+  b = f32x4{p.conic_2, p.opacity, p.depth, rgb[0]};
+  c = f32x4{rgb[1], rgb[2], __int_as_float(id), __int_as_float(cam.camera)};
 }
 
 }  // namespace dev

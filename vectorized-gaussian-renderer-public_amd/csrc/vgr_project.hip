@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
           float4* row = reinterpret_cast<float4*>(ray_records + record * 12);
           row[0] = make_float4(ray.precision[0], ray.precision[1], ray.precision[2], ray.precision[3]);
           row[1] = make_float4(ray.precision[4], ray.precision[5], ray.precision_mean[0], ray.precision_mean[1]);
-          row[2] = make_float4(ray.precision_mean[2], ray.mean_precision_mean, 0.0f, 0.0f);
+          row[2] = make_float4(ray.precision_mean[2], ray.mean_precision_mean, ray.cull_half[0], ray.cull_half[1]);
         }
       }
     }

@@ -136,7 +136,37 @@ struct RayData {
   float precision[6];  // 00, 01, 02, 11, 12, 22
   float precision_mean[3];
   float mean_precision_mean;
+  // Half extents (pixels) of a box around the binning centre holding every pixel
+  // whose float-evaluated m^2 can reach the support: the screen ellipse of a
+  // slightly larger support, sigma_c^2 = sigma^2 (1 + 1e-4) + 1e-5 mu^T P mu, plus
+  // one pixel.  The compositor skips rows of pixels outside it.
+  float cull_half[2];
 };
+
+// Screen ellipse {r = (x, y, 1): r^T (q P - P mu mu^T P) r >= 0} of the rays whose
+// closest approach to the mean lies within the support (q = mu^T P mu - sigma^2):
+// centre and half extents in normalized ray coordinates; false when unbounded.
+__device__ __forceinline__ bool support_ellipse(double p00, double p01, double p02, double p11, double p12,
+                                                double p22, double pm0, double pm1, double pm2, double q,
+                                                double& centre_x, double& centre_y, double& extent_x,
+                                                double& extent_y) {
+  const double k00 = q * p00 - pm0 * pm0;
+  const double k01 = q * p01 - pm0 * pm1;
+  const double k02 = q * p02 - pm0 * pm2;
+  const double k11 = q * p11 - pm1 * pm1;
+  const double k12 = q * p12 - pm1 * pm2;
+  const double k22 = q * p22 - pm2 * pm2;
+  const double k_det = k00 * k11 - k01 * k01;
+  if (!(k00 > 0.0 && k_det > 0.0)) return false;
+  const double inverse_k = 1.0 / k_det;
+  centre_x = (k01 * k12 - k11 * k02) * inverse_k;
+  centre_y = (k01 * k02 - k00 * k12) * inverse_k;
+  const double scale = -(k22 + k02 * centre_x + k12 * centre_y);
+  if (!(scale > 0.0)) return false;
+  extent_x = sqrt(scale * k11 * inverse_k);
+  extent_y = sqrt(scale * k00 * inverse_k);
+  return true;
+}
 
 // Exact-ray extent of a projected pair (renderer_cuda.cu:795-956): the camera-
 // space precision in double, the screen ellipse of the rays whose closest point
@@ -198,38 +228,38 @@ __device__ __forceinline__ bool ray_extent(const CameraParams& cam, float m0, fl
   ray.mean_precision_mean = static_cast<float>(mpm);
 
   const double support2 = static_cast<double>(support_sigma) * static_cast<double>(support_sigma);
-  const double q = mpm - support2;
-  const double k00 = q * p00 - pm0 * pm0;
-  const double k01 = q * p01 - pm0 * pm1;
-  const double k02 = q * p02 - pm0 * pm2;
-  const double k11 = q * p11 - pm1 * pm1;
-  const double k12 = q * p12 - pm1 * pm2;
-  const double k22 = q * p22 - pm2 * pm2;
-  const double k_det = k00 * k11 - k01 * k01;
-  bool finite = false;
-  if (k00 > 0.0 && k_det > 0.0) {
-    const double inverse_k = 1.0 / k_det;
-    const double centre_x = (k01 * k12 - k11 * k02) * inverse_k;
-    const double centre_y = (k01 * k02 - k00 * k12) * inverse_k;
-    const double scale = -(k22 + k02 * centre_x + k12 * centre_y);
-    if (scale > 0.0) {
-      const double extent_x = sqrt(scale * k11 * inverse_k);
-      const double extent_y = sqrt(scale * k00 * inverse_k);
-      const double mean_x = static_cast<double>(cam.focal_x) * centre_x + static_cast<double>(cam.center_x);
-      const double mean_y = static_cast<double>(cam.focal_y) * centre_y + static_cast<double>(cam.center_y);
-      p.mean_x = static_cast<float>(mean_x);
-      p.mean_y = static_cast<float>(mean_y);
-      p.radius_x = max(1, float_to_int_clamped(ceilf(static_cast<float>(fabs(static_cast<double>(cam.focal_x)) *
-                                                                        extent_x))));
-      p.radius_y = max(1, float_to_int_clamped(ceilf(static_cast<float>(fabs(static_cast<double>(cam.focal_y)) *
-                                                                        extent_y))));
-      finite = isfinite(mean_x) && isfinite(mean_y) && isfinite(extent_x) && isfinite(extent_y);
-    }
+  const double fx = fabs(static_cast<double>(cam.focal_x)), fy = fabs(static_cast<double>(cam.focal_y));
+  double centre_x, centre_y, extent_x, extent_y;
+  bool finite = support_ellipse(p00, p01, p02, p11, p12, p22, pm0, pm1, pm2, mpm - support2, centre_x, centre_y,
+                                extent_x, extent_y);
+  if (finite) {
+    const double mean_x = static_cast<double>(cam.focal_x) * centre_x + static_cast<double>(cam.center_x);
+    const double mean_y = static_cast<double>(cam.focal_y) * centre_y + static_cast<double>(cam.center_y);
+    p.mean_x = static_cast<float>(mean_x);
+    p.mean_y = static_cast<float>(mean_y);
+    p.radius_x = max(1, float_to_int_clamped(ceilf(static_cast<float>(fx * extent_x))));
+    p.radius_y = max(1, float_to_int_clamped(ceilf(static_cast<float>(fy * extent_y))));
+    finite = isfinite(mean_x) && isfinite(mean_y) && isfinite(extent_x) && isfinite(extent_y);
   }
   if (!finite) {
     p.mean_x = 0.5f * static_cast<float>(cam.width);
     p.mean_y = 0.5f * static_cast<float>(cam.height);
     p.radius_x = p.radius_y = max(cam.width, cam.height);
+  }
+  // Cull box of the compositor (never used for binning): the ellipse of the
+  // enlarged support, measured from the binning centre; unbounded covers all.
+  ray.cull_half[0] = ray.cull_half[1] = 3.0e38f;
+  double cc_x, cc_y, ce_x, ce_y;
+  if (finite && support_ellipse(p00, p01, p02, p11, p12, p22, pm0, pm1, pm2,
+                                mpm - (support2 * (1.0 + 1.0e-4) + 1.0e-5 * fabs(mpm)), cc_x, cc_y, ce_x, ce_y)) {
+    const double half_x = fabs(static_cast<double>(cam.focal_x) * cc_x + static_cast<double>(cam.center_x) -
+                               static_cast<double>(p.mean_x)) + fx * ce_x + 1.0;
+    const double half_y = fabs(static_cast<double>(cam.focal_y) * cc_y + static_cast<double>(cam.center_y) -
+                               static_cast<double>(p.mean_y)) + fy * ce_y + 1.0;
+    if (half_x < 1.0e30 && half_y < 1.0e30) {
+      ray.cull_half[0] = static_cast<float>(half_x) * (1.0f + 1.0e-6f);
+      ray.cull_half[1] = static_cast<float>(half_y) * (1.0f + 1.0e-6f);
+    }
   }
   return !(p.mean_x + static_cast<float>(p.radius_x) <= 0.0f || p.mean_y + static_cast<float>(p.radius_y) <= 0.0f ||
            p.mean_x - static_cast<float>(p.radius_x) >= static_cast<float>(cam.width) ||

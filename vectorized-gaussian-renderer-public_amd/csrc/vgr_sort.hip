@@ -36,9 +36,11 @@
 namespace vgr {
 namespace {
 
-constexpr uint32_t kFlagAggregate = 1u << 30;
-constexpr uint32_t kFlagPrefix = 2u << 30;
-constexpr uint32_t kFlagValueMask = (1u << 30) - 1u;
+// Look-back flag words: 0 = not published; bit 31 set = inclusive prefix in bits
+// 0-30 (up to 2^31 - 1 keys); otherwise the tile's aggregate plus one (a tile
+// holds at most 8192 keys, so an aggregate word is never 0 nor has bit 31).
+constexpr uint32_t kFlagPrefix = 1u << 31;
+constexpr uint32_t kFlagValueMask = kFlagPrefix - 1u;
 constexpr uint32_t kDefaultSpinLimit = 1u << 22;
 constexpr uint32_t kForceGiveUp = 0xffffffffu;  // test hook: every look-back gives up at once
 
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
   using Count = typename std::conditional<(kRadix > 256), uint16_t, uint32_t>::type;
   __shared__ Count s_count[kWaves][kRadix];
   __shared__ uint32_t s_local_start[kRadix];
-  __shared__ int32_t s_global_start[kRadix];  // capacities are below 2^30
+  __shared__ int32_t s_global_start[kRadix];  // capacities are below 2^31
   __shared__ uint32_t s_ticket;
 
   // The grid covers the capacity; the device-side key count needs its first
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
     }
     total = running;
     // Publish this tile's aggregate (tile 0 publishes its inclusive prefix).
-    const uint32_t flag = (tile == 0 ? kFlagPrefix : kFlagAggregate) | total;
+    const uint32_t flag = tile == 0 ? (kFlagPrefix | total) : total + 1u;
     __hip_atomic_store(&flags[tile * kRadix + threadIdx.x], flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   // Exclusive scan of the digit totals (kRadix <= kSortThreads: one per thread).
@@ -320,11 +322,12 @@ __global__ __launch_bounds__(kSortThreads) void sort_onesweep_pass_kernel(
 #pragma unroll
         for (int w = 0; w < kWindow; ++w) {
           if (found || consumed < w) continue;  // stopped at an earlier entry
-          const uint32_t state = window[w] & ~kFlagValueMask;
-          if (state == 0u || predecessor - w < 0) continue;  // not published yet
-          exclusive += window[w] & kFlagValueMask;
+          const uint32_t word = window[w];
+          if (word == 0u || predecessor - w < 0) continue;  // not published yet
+          const bool prefix = (word & kFlagPrefix) != 0u;
+          exclusive += prefix ? (word & kFlagValueMask) : word - 1u;
           consumed = w + 1;
-          if (state == kFlagPrefix || predecessor - w == 0) found = true;
+          if (prefix || predecessor - w == 0) found = true;
         }
         if (found) break;
         predecessor -= consumed;

@@ -167,7 +167,7 @@ class DifferentiableRenderer:
         self._calibrated = False
         self._pending: list[tuple[torch.cuda.Event, torch.Tensor, int, int]] = []
         self.max_visible = 1 << 27
-        self.max_intersections = (1 << 30) - 1
+        self.max_intersections = (1 << 31) - 1
         self.camera_chunks = 1
 
     # -- workspace ------------------------------------------------------------------------
@@ -182,7 +182,7 @@ class DifferentiableRenderer:
         if self.intersection_capacity is None:
             self.intersection_capacity = max(1, int(math.ceil(self.visible_capacity * self.intersections_per_visible)))
         visible = min(int(self.visible_capacity), 0x7FFFFFFF)
-        intersections = min(int(self.intersection_capacity), (1 << 30) - 1)
+        intersections = min(int(self.intersection_capacity), (1 << 31) - 1)
         device = self.device
         scratch_bytes = self.native.scratch_bytes(
             max_views=batch, height=self.height, width=self.width, gaussians=gaussians,
@@ -261,10 +261,10 @@ class DifferentiableRenderer:
             if visible > workspace.records.shape[0]:
                 self.visible_capacity = min(int(math.ceil(visible * _GROWTH)), 0x7FFFFFFF)
                 self.intersection_capacity = max(int(self.intersection_capacity or 0),
-                                                 min(int(math.ceil(intersections * _GROWTH)), (1 << 30) - 1))
+                                                 min(int(math.ceil(intersections * _GROWTH)), (1 << 31) - 1))
                 grow = True
             if intersections > workspace.keys_in.numel():
-                self.intersection_capacity = min(int(math.ceil(intersections * _GROWTH)), (1 << 30) - 1)
+                self.intersection_capacity = min(int(math.ceil(intersections * _GROWTH)), (1 << 31) - 1)
                 grow = True
             if sort_failures and not grow:
                 continue  # a sort look-back wait gave up: the order is wrong, render again
@@ -303,7 +303,7 @@ class DifferentiableRenderer:
         self._calibrated = False  # the next call renders synchronously with grown buffers
         self.visible_capacity = min(max(int(self.visible_capacity or 0), math.ceil(visible * _GROWTH)), 0x7FFFFFFF)
         self.intersection_capacity = min(max(int(self.intersection_capacity or 0),
-                                             math.ceil(intersections * _GROWTH)), (1 << 30) - 1)
+                                             math.ceil(intersections * _GROWTH)), (1 << 31) - 1)
         from .hip_backend import RendererCapacityError
 
         raise RendererCapacityError(
@@ -359,7 +359,7 @@ class DifferentiableRenderer:
 
 
 class _SplitCameras(Exception):
-    """One call's demand exceeds the int32 / 2^30 workspace limits: render fewer cameras per call."""
+    """One call's demand exceeds the int32 workspace limits: render fewer cameras per call."""
 
     def __init__(self, factor: int) -> None:
         super().__init__(f"split cameras by {factor}")

@@ -31,8 +31,9 @@ from typing import Any
 import torch
 
 from .hip_backend import COUNTER_NAMES, RendererCapacityError
-from .native import (COUNTER_SLOTS, FRONT_MAX_CAMERA_TILES, MAX_INTERSECTIONS, ORDERING_GLOBAL_RADIX,
-                     ORDERING_TILE_FRONT, RECORD_FLOATS, SORT_FAILURE_SLOT, infer_sh_degree, load_native_module)
+from .native import (CAMERA_GRAD_FLOATS, COUNTER_SLOTS, FRONT_MAX_CAMERA_TILES, MAX_INTERSECTIONS,
+                     ORDERING_GLOBAL_RADIX, ORDERING_TILE_FRONT, POSE_REPLICAS, RECORD_FLOATS, SORT_FAILURE_SLOT,
+                     infer_sh_degree, load_native_module)
 
 ORDERINGS = {"global": ORDERING_GLOBAL_RADIX, "front": ORDERING_TILE_FRONT}
 
@@ -93,6 +94,7 @@ class BatchedRenderer:
         max_intersections_per_submission: int = (1 << 31) - 1,
         ordering: str = "auto",
         front_open_limit: float = 0.05,
+        pose_gradients: bool = False,
         native_module: Any | None = None,
     ) -> None:
         self.device = torch.device(device)
@@ -124,6 +126,12 @@ class BatchedRenderer:
             raise ValueError("ordering must be 'global', 'front' or 'auto'.")
         self.ordering_request = ordering
         self.front_open_limit = float(front_open_limit)
+        if pose_gradients and (ordering == "global" or output_srgb or not full_sensor_output):
+            raise ValueError("pose_gradients needs the tile-front ordering ('front' or 'auto'), linear RGB and full "
+                             "sensor output.")
+        self.pose_gradients = bool(pose_gradients)
+        self.checkpoints: tuple[torch.Tensor, torch.Tensor] | None = None
+        self._last_call = None
         self.ordering = "global"  # resolved by calibrate()
         self.plain_per_submission = 0
         self._scratch_floor = 0
@@ -256,6 +264,13 @@ class BatchedRenderer:
             "alpha": torch.zeros((self.total_pixels, 1), device=dev, dtype=torch.float32),
             "semantic_id": torch.full((self.total_pixels, 1), -1, device=dev, dtype=torch.int64),
         }
+        if self.pose_gradients:
+            # Per-pixel checkpoints of the forward and the pose-gradient buffers.
+            self.checkpoints = (torch.ones(self.total_pixels, device=dev, dtype=torch.float32),
+                                torch.zeros(self.total_pixels, device=dev, dtype=torch.int32))
+            self.camera_grads = torch.zeros((POSE_REPLICAS * b, CAMERA_GRAD_FLOATS), device=dev,
+                                            dtype=torch.float32)
+            self.grad_viewmats = torch.zeros((b, 4, 4), device=dev, dtype=torch.float32)
         g = packed["means"].shape[0]
         if visible_capacity is None:
             visible_capacity = min(b * g, max(1, b * 200_000))
@@ -364,6 +379,8 @@ class BatchedRenderer:
         extra: dict[str, Any] = {}
         if stage_events is not None:
             extra["stage_events"] = stage_events
+        if self.pose_gradients and self.ordering == "front":
+            extra["checkpoints"] = self.checkpoints
         chunk_statistics = None
         if statistics is not None:
             statistics.zero_()
@@ -387,6 +404,7 @@ class BatchedRenderer:
             self.chunk_counters[index].copy_(self.workspace["counters"])
             if chunk_statistics is not None:
                 statistics.add_(chunk_statistics)
+        self._last_call = self.native.last_call
 
     def read_counters(self) -> dict[str, int]:
         """Counters summed over the submissions of the plan (``max_*`` keys: the largest one)."""
@@ -443,6 +461,9 @@ class BatchedRenderer:
         self.ordering = "global"
         counters = self._calibrate_capacities(max_retries)
         self.plain_per_submission = max(stop - start for start, stop, _ in self.chunks)
+        if self.pose_gradients and not self.front_eligible():
+            raise ValueError("pose gradients need the tile-front ordering, which this plan cannot use "
+                             f"(16x16 tiles and at most {FRONT_MAX_CAMERA_TILES} tiles per camera).")
         if self.ordering_request == "global" or not self.front_eligible():
             return counters
         global_chunks = self.chunk_count
@@ -451,7 +472,7 @@ class BatchedRenderer:
         # One submission now spans more tiles (and needs other scratch): size the buffers for it.
         self._allocate(self.visible_capacity, self.intersection_capacity)
         front_counters = self._calibrate_capacities(max_retries)
-        if self.ordering_request == "auto":
+        if self.ordering_request == "auto" and not self.pose_gradients:
             statistics = torch.zeros(4, dtype=torch.int64, device=self.device)
             self._submit(statistics=statistics)
             opened = int(statistics[3])
@@ -518,6 +539,37 @@ class BatchedRenderer:
             self._submit()
         return self.outputs
 
+    def backward_pose(self, grad_rgb: torch.Tensor, grad_depth: torch.Tensor | None = None,
+                      grad_alpha: torch.Tensor | None = None) -> torch.Tensor:
+        """dL/d viewmats [B,4,4] of the last render from the flat upstream gradients
+        (``grad_rgb`` [P,3], ``grad_depth`` / ``grad_alpha`` [P,1] or None).
+
+        Needs ``pose_gradients=True`` and the tile-front ordering in one submission: the pass
+        walks each tile's list back to front from the checkpointed transmittance, recomputing
+        the records, and feeds every (Gaussian, tile) pair's terms through the pair's projection
+        backward into per-camera pose partials (no scene gradients).  The returned tensor is the
+        renderer's own buffer, overwritten by the next call."""
+        if not self.pose_gradients:
+            raise RuntimeError("backward_pose needs a renderer built with pose_gradients=True.")
+        if self.ordering != "front" or self.chunk_count != 1 or self._last_call is None:
+            raise RuntimeError("backward_pose follows a calibrated tile-front render in one submission.")
+        for name, tensor, width in (("grad_rgb", grad_rgb, 3), ("grad_depth", grad_depth, 1),
+                                    ("grad_alpha", grad_alpha, 1)):
+            if tensor is not None and tensor.numel() != self.total_pixels * width:
+                raise ValueError(f"{name} must hold {width} value(s) per pixel of the plan ([P, {width}]).")
+        self.grad_viewmats.zero_()
+        self.native.render_backward(self._last_call, grad_rgb.contiguous(),
+                                    None if grad_depth is None else grad_depth.contiguous(),
+                                    None if grad_alpha is None else grad_alpha.contiguous(), None,
+                                    scene_grads=None, grad_viewmats=self.grad_viewmats,
+                                    camera_grads=self.camera_grads)
+        return self.grad_viewmats
+
+    def render_pose(self, viewmats: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        """Differentiable render w.r.t. ``viewmats`` [B,4,4]: returns copies of the flat rgb, depth
+        and alpha outputs; their backward calls :meth:`backward_pose`."""
+        return _BatchedPoseRender.apply(viewmats, self)
+
     def camera_outputs(self, index: int, outputs: dict[str, torch.Tensor] | None = None) -> dict[str, torch.Tensor]:
         """[H, W, C] views of camera ``index`` in the flat outputs."""
         outputs = outputs if outputs is not None else self.outputs
@@ -554,3 +606,21 @@ class BatchedRenderer:
                     4096, 64, False, False, False, self.options["output_srgb"], True, self.full_sensor_output, False,
                     sh_degree=self.sh_degree, **self.plain_tuning)
         return self.plain_groups
+
+
+class _BatchedPoseRender(torch.autograd.Function):
+    """Batched forward (graph replay when captured) with camera-pose gradients."""
+
+    @staticmethod
+    def forward(ctx, viewmats: torch.Tensor, renderer: BatchedRenderer):
+        outputs = renderer.render(viewmats.detach())
+        ctx.renderer = renderer
+        return outputs["rgb"].clone(), outputs["depth"].clone(), outputs["alpha"].clone()
+
+    @staticmethod
+    def backward(ctx, grad_rgb, grad_depth, grad_alpha):
+        renderer: BatchedRenderer = ctx.renderer
+        if grad_rgb is None:
+            grad_rgb = torch.zeros((renderer.total_pixels, 3), device=renderer.device, dtype=torch.float32)
+        grad = renderer.backward_pose(grad_rgb, grad_depth, grad_alpha)
+        return grad.clone(), None

@@ -65,7 +65,7 @@ __all__ = ["COUNTER_NAMES", "CustomCudaBackend", "CustomHipBackend", "DEFAULT_GA
 FULL_SENSOR_OUTPUTS = ("rgb", "depth", "alpha", "semantic_id")
 RGB_ONLY_OUTPUTS = ("rgb",)
 _INT32_MAX = 2_147_483_647
-_SORT_CAPACITY_LIMIT = (1 << 30) - 1  # look-back flags carry 30-bit prefixes
+_SORT_CAPACITY_LIMIT = (1 << 31) - 1  # int32 positions; look-back flags carry 31-bit prefixes
 _RECORD_BYTES = 4 * RECORD_FLOATS
 _CAPACITY_ERROR_SCHEMA = "renderer-capacity-error/v1"
 _ALLOCATION_SCHEMA = "renderer-capacity-allocation/v1"

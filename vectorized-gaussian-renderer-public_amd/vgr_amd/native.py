@@ -55,8 +55,8 @@ SORT_FAILURE_SLOT = 5
 ORDERING_GLOBAL_RADIX = 0
 ORDERING_TILE_FRONT = 1
 # Largest intersection capacity of one submission per ordering: the radix sort's
-# look-back flags carry 30-bit prefixes; tile-front buckets use int32 offsets.
-MAX_INTERSECTIONS = {ORDERING_GLOBAL_RADIX: (1 << 30) - 1, ORDERING_TILE_FRONT: (1 << 31) - 1}
+# look-back flags carry 31-bit prefixes; tile-front buckets use int32 offsets.
+MAX_INTERSECTIONS = {ORDERING_GLOBAL_RADIX: (1 << 31) - 1, ORDERING_TILE_FRONT: (1 << 31) - 1}
 # Tile-front ordering limits: tiles per camera (16x16 tiles).
 FRONT_MAX_CAMERA_TILES = 2048
 
@@ -598,7 +598,7 @@ class NativeRenderer:
         return int(selector.value)
 
     def render_backward(self, call: "RenderCall", grad_rgb: torch.Tensor, grad_depth: torch.Tensor | None,
-                        grad_alpha: torch.Tensor | None, record_grads: torch.Tensor, *,
+                        grad_alpha: torch.Tensor | None, record_grads: torch.Tensor | None, *,
                         scene_grads: tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor] | None,
                         grad_viewmats: torch.Tensor | None = None,
                         camera_grads: torch.Tensor | None = None) -> None:
@@ -607,7 +607,9 @@ class NativeRenderer:
         ``scene_grads`` = (means [G,3], covariances [G,6], opacities [G],
         features [G,F]) float32, zero-filled by the caller and added to.
         ``grad_viewmats`` [B,4,4] float32 (zeroed, added to) with ``camera_grads``
-        workspace [active, 24] requests camera-pose gradients.
+        workspace [POSE_REPLICAS * active, 24] requests camera-pose gradients.  After a
+        tile-front forward (``ordering=ORDERING_TILE_FRONT`` with checkpoints) the pass computes
+        pose gradients only: ``record_grads`` and ``scene_grads`` are None there.
         """
         if call.outputs.final_transmittance is None:
             raise ValueError("render_backward needs a forward rendered with checkpoints")
@@ -619,10 +621,15 @@ class NativeRenderer:
                 continue
             if tensor.dtype != torch.float32 or tensor.device != device or not tensor.is_contiguous():
                 raise ValueError(f"{name} must be a contiguous float32 tensor on {device}")
-        if call.workspace.slot_records is None:
-            raise ValueError("render_backward needs a forward rendered with slot_records")
-        if record_grads.numel() < call.workspace.intersection_capacity * RECORD_GRAD_FLOATS:
-            raise ValueError("record_grads must hold 10 floats per intersection slot")
+        if int(call.options.ordering) == ORDERING_TILE_FRONT:
+            if scene_grads is not None or grad_viewmats is None:
+                raise ValueError("the tile-front backward computes camera-pose gradients only: pass grad_viewmats "
+                                 "and no scene_grads")
+        else:
+            if call.workspace.slot_records is None:
+                raise ValueError("render_backward needs a forward rendered with slot_records")
+            if record_grads is None or record_grads.numel() < call.workspace.intersection_capacity * RECORD_GRAD_FLOATS:
+                raise ValueError("record_grads must hold 10 floats per intersection slot")
         gradients = _Gradients(grad_rgb=_ptr(grad_rgb), grad_depth=_ptr(grad_depth), grad_alpha=_ptr(grad_alpha),
                                record_grads=_ptr(record_grads), sorted_selector=call.selector)
         if scene_grads is not None:
