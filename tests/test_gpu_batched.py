@@ -121,8 +121,9 @@ def test_chunk_culling_drops_work_in_a_room_and_keeps_results():
         np.testing.assert_array_equal(out[name], reference[name], err_msg=name)
 
 
-def test_spatial_order_keeps_gaussian_id_tie_order():
-    count = 48
+@pytest.mark.parametrize("count", [48, 400])
+def test_spatial_order_keeps_gaussian_id_tie_order(count):
+    """Equal depths give runs of equal keys: insertion sort up to 32, heap sort beyond."""
     means = torch.zeros((count, 3))
     means[:, 0] = torch.linspace(-0.05, 0.05, count)  # spread so Morton order differs from ID order
     means[:, 2] = 5.0

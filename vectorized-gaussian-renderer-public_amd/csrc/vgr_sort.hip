@@ -434,15 +434,27 @@ __global__ __launch_bounds__(kBlockThreads) void ranges_kernel(const uint64_t* _
   }
 }
 
+// Order key of a run member: its record's Gaussian ID, then the record index
+// (IDs are unique within one camera's records; the index only orders records
+// that a corrupted list could repeat).
+__device__ __forceinline__ uint64_t tie_key(const vgr_record* __restrict__ records, int64_t visible_capacity,
+                                            int32_t value) {
+  const int32_t id = (value >= 0 && value < visible_capacity) ? records[value].gaussian_id : 0;
+  return (static_cast<uint64_t>(static_cast<uint32_t>(id)) << 32) | static_cast<uint32_t>(value);
+}
+
 // Restores Gaussian-ID order inside runs of equal keys (same tile, same depth
 // bits).  Needed only when the packed scene is stored in a spatial order, in
 // which case records of a camera are not in registration-id order.  Every
-// run is owned by the thread that sees its first element.
+// run is owned by the thread that sees its first element: insertion sort for
+// short runs, in-place heap sort (O(n log n)) for runs longer than 32, such as
+// the many equal depths of a fronto-parallel plane.
 __global__ __launch_bounds__(kBlockThreads) void tie_order_kernel(const uint64_t* __restrict__ keys,
                                                                   int32_t* __restrict__ values,
                                                                   const vgr_record* __restrict__ records,
                                                                   const int64_t* __restrict__ counters,
                                                                   int64_t capacity, int64_t visible_capacity) {
+  constexpr int64_t kInsertionRun = 32;
   const int64_t n = element_count(counters, capacity);
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlockThreads + threadIdx.x; i + 1 < n;
        i += static_cast<int64_t>(gridDim.x) * kBlockThreads) {
@@ -450,18 +462,48 @@ __global__ __launch_bounds__(kBlockThreads) void tie_order_kernel(const uint64_t
     if (key == kSentinelKey || keys[i + 1] != key || (i > 0 && keys[i - 1] == key)) continue;
     int64_t end = i + 1;
     while (end < n && keys[end] == key) ++end;
-    for (int64_t j = i + 1; j < end; ++j) {
-      const int32_t moving = values[j];
-      const int32_t moving_id = (moving >= 0 && moving < visible_capacity) ? records[moving].gaussian_id : 0;
-      int64_t k = j - 1;
-      while (k >= i) {
-        const int32_t other = values[k];
-        const int32_t other_id = (other >= 0 && other < visible_capacity) ? records[other].gaussian_id : 0;
-        if (other_id <= moving_id) break;
-        values[k + 1] = other;
-        --k;
+    int32_t* run = values + i;
+    const int64_t length = end - i;
+    if (length <= kInsertionRun) {
+      for (int64_t j = 1; j < length; ++j) {
+        const int32_t moving = run[j];
+        const uint64_t moving_key = tie_key(records, visible_capacity, moving);
+        int64_t k = j - 1;
+        while (k >= 0 && tie_key(records, visible_capacity, run[k]) > moving_key) {
+          run[k + 1] = run[k];
+          --k;
+        }
+        run[k + 1] = moving;
       }
-      values[k + 1] = moving;
+      continue;
+    }
+    // Max-heap on the keys, then repeated extraction to the back.
+    auto sift_down = [&](int64_t root, int64_t size) {
+      const int32_t item = run[root];
+      const uint64_t item_key = tie_key(records, visible_capacity, item);
+      for (;;) {
+        int64_t child = 2 * root + 1;
+        if (child >= size) break;
+        uint64_t child_key = tie_key(records, visible_capacity, run[child]);
+        if (child + 1 < size) {
+          const uint64_t right_key = tie_key(records, visible_capacity, run[child + 1]);
+          if (right_key > child_key) {
+            ++child;
+            child_key = right_key;
+          }
+        }
+        if (child_key <= item_key) break;
+        run[root] = run[child];
+        root = child;
+      }
+      run[root] = item;
+    };
+    for (int64_t root = length / 2 - 1; root >= 0; --root) sift_down(root, length);
+    for (int64_t last = length - 1; last > 0; --last) {
+      const int32_t top = run[0];
+      run[0] = run[last];
+      run[last] = top;
+      sift_down(0, last);
     }
   }
 }
