@@ -348,6 +348,11 @@ typedef struct vgr_gradients {
  * its slots' rows (per-pair results are deterministic).  Scene gradients of a
  * Gaussian seen by several camera groups, and pose gradients, are merged with
  * float atomics (order-dependent to rounding).  Needs workspace.slot_records.
+ * After a checkpointed VGR_ORDERING_TILE_FRONT forward (no slot_records) the
+ * pass computes camera-pose gradients only: each tile list is walked with the
+ * records recomputed from the scene, and every (Gaussian, tile) pair's terms go
+ * straight through the pair's projection backward into camera_grads; scene
+ * gradients and record_grads must be NULL there.
  * No reference counterpart (the reference is forward-only,
  * KERNEL_ARCHITECTURE.md:454-460). */
 int32_t vgr_render_backward(const vgr_scene* scene, const vgr_cameras* cameras,
