@@ -474,9 +474,17 @@ __global__ __launch_bounds__(kRaster* kRaster) void composite_ray_kernel(Composi
   __shared__ f32x4 s_b[kThreads];
   __shared__ f32x4 s_c[kThreads];
   __shared__ f32x4 s_colour[kThreads];  // red, green, blue, gaussian id bits
-  __shared__ uint32_t s_waves[kThreads];  // waves whose pixel rows the entry's cull box reaches
-  constexpr int kRowsPerWave = kWave / kRaster;
+  // Each wave owns an 8x8 pixel block (16x16 raster tiles: four quadrants) and
+  // walks the batch entries whose cull box reaches it, compacted in list order;
+  // with 8x8 raster tiles the bin is the tile, whose box every entry reaches.
+  constexpr int kWaves = kThreads / kWave;
+  constexpr bool kCull = kWaves > 1;
+  constexpr int kBlocksX = kRaster / 8;
+  __shared__ uint16_t s_list[kWaves][kThreads];
+  __shared__ uint32_t s_totals[kWaves][kWaves];  // [source wave][list]
   const int wave = static_cast<int>(threadIdx.x) / kWave;
+  const int wave_lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+  const int block_x0 = (wave % kBlocksX) * 8, block_y0 = (wave / kBlocksX) * 8;
   const int64_t block = blockIdx.x;
   const int active = static_cast<int>(block / raster_tiles);
   if (active >= args.active_count) return;
@@ -485,8 +493,8 @@ __global__ __launch_bounds__(kRaster* kRaster) void composite_ray_kernel(Composi
   const int raster_tile = static_cast<int>(block - static_cast<int64_t>(active) * raster_tiles);
   const int tile_x = raster_tile % raster_tiles_x;
   const int tile_y = raster_tile / raster_tiles_x;
-  const int pixel_x = tile_x * kRaster + static_cast<int>(threadIdx.x) % kRaster;
-  const int pixel_y = tile_y * kRaster + static_cast<int>(threadIdx.x) / kRaster;
+  const int pixel_x = tile_x * kRaster + block_x0 + (wave_lane & 7);
+  const int pixel_y = tile_y * kRaster + block_y0 + (wave_lane >> 3);
   const bool valid_pixel = pixel_x < cam.width && pixel_y < cam.height;
   const int bin_x = tile_x * kRaster / bin_size;
   const int bin_y = tile_y * kRaster / bin_size;
@@ -505,6 +513,7 @@ __global__ __launch_bounds__(kRaster* kRaster) void composite_ray_kernel(Composi
     for (int64_t batch_start = range_start; batch_start < range_end; batch_start += kThreads) {
       const int count = static_cast<int>(range_end - batch_start < kThreads ? range_end - batch_start : kThreads);
       __syncthreads();  // previous batch fully consumed
+      uint32_t waves = 0u;
       if (static_cast<int>(threadIdx.x) < count) {
         const int32_t record = args.sorted_values[batch_start + threadIdx.x];
         if (record >= 0 && record < args.visible_capacity) {
@@ -515,32 +524,57 @@ __global__ __launch_bounds__(kRaster* kRaster) void composite_ray_kernel(Composi
           s_b[threadIdx.x] = ray[1];
           s_c[threadIdx.x] = f32x4{q2.x, q2.y, r1.y, r1.z};
           s_colour[threadIdx.x] = f32x4{r1.w, r2.x, r2.y, r2.z};
-          // Rows of pixel centres each wave covers against the entry's cull box
-          // (centre r0.xy, half extents q2.zw): a wave outside it has alpha 0 at
-          // every pixel, so it skips the entry.
-          const float x0 = static_cast<float>(tile_x * kRaster) + 0.5f;
-          uint32_t waves = 0u;
-          if (fabsf(r0.x - fminf(fmaxf(r0.x, x0), x0 + static_cast<float>(kRaster - 1))) <= q2.z) {
+          // Each wave's block of pixel centres against the entry's cull box (centre
+          // r0.xy, half extents q2.zw): a wave outside it has alpha 0 at every
+          // pixel, so the entry is left out of its list.
+          waves = kCull ? 0u : 1u;
+          if constexpr (kCull) {
 #pragma unroll
-            for (int w = 0; w < kThreads / kWave; ++w) {
-              const float y0 = static_cast<float>(tile_y * kRaster + w * kRowsPerWave) + 0.5f;
-              const float y1 = y0 + static_cast<float>(kRowsPerWave - 1);
-              if (fabsf(r0.y - fminf(fmaxf(r0.y, y0), y1)) <= q2.w) waves |= 1u << w;
+            for (int w = 0; w < kWaves; ++w) {
+              const float x0 = static_cast<float>(tile_x * kRaster + (w % kBlocksX) * 8) + 0.5f;
+              const float y0 = static_cast<float>(tile_y * kRaster + (w / kBlocksX) * 8) + 0.5f;
+              if (fabsf(r0.x - fminf(fmaxf(r0.x, x0), x0 + 7.0f)) <= q2.z &&
+                  fabsf(r0.y - fminf(fmaxf(r0.y, y0), y0 + 7.0f)) <= q2.w) {
+                waves |= 1u << w;
+              }
             }
           }
-          s_waves[threadIdx.x] = waves;
         } else {
           // Only reachable with a corrupted list: an entry that never blends.
           s_a[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
           s_b[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
           s_c[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
           s_colour[threadIdx.x] = f32x4{0.0f, 0.0f, 0.0f, __int_as_float(-1)};
-          s_waves[threadIdx.x] = 0u;
         }
       }
+      // Per-wave lists in batch order: ballot prefix within the source wave, then
+      // the totals of the source waves before it.
+      int list_count = count;
+      if constexpr (kCull) {
+        const uint64_t below = wave_lane == 0 ? 0ull : (~0ull >> (kWave - wave_lane));
+        uint32_t prefix[kWaves];
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+          const uint64_t vote = __ballot((waves >> w) & 1u);
+          prefix[w] = static_cast<uint32_t>(__popcll(vote & below));
+          if (wave_lane == 0) s_totals[wave][w] = static_cast<uint32_t>(__popcll(vote));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+          if ((waves >> w) & 1u) {
+            uint32_t position = prefix[w];
+            for (int q = 0; q < wave; ++q) position += s_totals[q][w];
+            s_list[w][position] = static_cast<uint16_t>(threadIdx.x);
+          }
+        }
+        list_count = 0;
+#pragma unroll
+        for (int q = 0; q < kWaves; ++q) list_count += static_cast<int>(s_totals[q][wave]);
+      }
       __syncthreads();
-      for (int item = 0; !done && item < count; ++item) {
-        if (((s_waves[item] >> wave) & 1u) == 0u) continue;  // uniform over the wave
+      for (int i = 0; !done && i < list_count; ++i) {
+        const int item = kCull ? static_cast<int>(s_list[wave][i]) : i;
         const f32x4 c = s_c[item];
         const float alpha = ray_alpha(ray_x, ray_y, s_a[item], s_b[item], c, support_sigma);
         if (!(alpha > 0.0f)) continue;
