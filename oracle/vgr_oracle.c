@@ -557,8 +557,9 @@ static int compare_tau_entries(const void* a, const void* b) {
 
 /* Per-pixel cutoff bucket of one camera (advance_depth_cutoff_kernel,
  * renderer_cuda.cu:1658-1690, over all buckets at once): the first bucket where
- * the running thickness reaches the threshold, else the last bucket.  Per
- * (pixel, bucket) sums wrap in 32 bits, as the device table's do. */
+ * the running thickness reaches the threshold, else the last bucket.  Sums are
+ * 64-bit integers (1/65536 fixed point), so no grouping of the buckets changes
+ * them. */
 static int32_t* depth_cutoffs(const camera_job* job, int buckets) {
   const camera_setup* c = job->cam;
   const oracle_options* opt = job->opt;
@@ -594,7 +595,7 @@ static int32_t* depth_cutoffs(const camera_job* job, int buckets) {
     int found = -1;
     while (i < entries && list[i].pixel == pixel) {
       const int32_t bucket = list[i].bucket;
-      uint32_t bucket_sum = 0;
+      uint64_t bucket_sum = 0;
       while (i < entries && list[i].pixel == pixel && list[i].bucket == bucket) bucket_sum += list[i++].tau;
       running += bucket_sum;
       if (found < 0 && running >= ORACLE_TAU_THRESHOLD_FIXED) found = bucket;

@@ -61,13 +61,6 @@ vgr_options binning_options(const vgr_options& options) {
   return binned;
 }
 
-// Depth-cutoff buckets per pass of the dense path: 0 = 64 (the reference default),
-// never more than the bucket count.
-int32_t resolve_depth_group(const vgr_options& options) {
-  const int32_t group = options.depth_bucket_group_size == 0 ? 64 : options.depth_bucket_group_size;
-  return group < options.depth_bucket_count ? group : options.depth_bucket_count;
-}
-
 // Tile-front ordering: LDS budgets of the bucketed emission (count kernel
 // histogram over a camera group's tiles; write kernel tables over one camera's
 // tiles).
@@ -190,7 +183,10 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   offset = align_up(offset + sizeof(int32_t) * front_tiles);
   const bool cutoff = options.tile_size == 1 && options.depth_bucket_count > 0;
   layout.dense_pixels = cutoff ? global_tiles : 0;
-  layout.dense_group_size = cutoff ? resolve_depth_group(options) : 0;
+  // Two-level cutoff: columns of up to 64 buckets.  The cutoffs do not depend on
+  // this split (nor on depth_bucket_group_size): they are prefix sums of integers.
+  layout.dense_fine = cutoff ? (options.depth_bucket_count < 64 ? options.depth_bucket_count : 64) : 0;
+  layout.dense_columns = cutoff ? (options.depth_bucket_count + layout.dense_fine - 1) / layout.dense_fine : 0;
   const size_t dense_pixels = static_cast<size_t>(layout.dense_pixels);
   layout.dense_camera_map = offset;
   offset = align_up(offset + sizeof(int32_t) * (cutoff ? static_cast<size_t>(batch > 0 ? batch : 1) : 0u));
@@ -198,8 +194,11 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   offset = align_up(offset + sizeof(int32_t) * dense_pixels);
   layout.dense_accumulated = offset;
   offset = align_up(offset + sizeof(uint64_t) * dense_pixels);
+  layout.dense_crossing = offset;
+  offset = align_up(offset + sizeof(int32_t) * dense_pixels);
   layout.dense_tau = offset;
-  offset = align_up(offset + sizeof(uint32_t) * dense_pixels * static_cast<size_t>(layout.dense_group_size));
+  offset = align_up(offset + sizeof(uint64_t) * dense_pixels *
+                                 static_cast<size_t>(layout.dense_columns + layout.dense_fine));
   layout.total = offset;
   return layout;
 }

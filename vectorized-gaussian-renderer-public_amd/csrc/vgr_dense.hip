@@ -8,16 +8,20 @@
 // thickness is a lower bound of -log T), so the cutoff only shrinks the
 // intersection count; images do not change.
 //
-// This build's form: the records pass writes the visible records and their
-// radii; then, for each group of depth_bucket_group_size buckets, one kernel
-// walks the records of the group's buckets and adds their per-pixel thickness
-// into a [pixels x group] table of 1/65536 fixed-point integers (integer adds
-// commute, so the sums and hence the cutoffs do not depend on the order in
-// which the GPU performs them; the reference sums floats with atomics), and a
-// per-pixel kernel advances each open pixel's running sum over the group's
-// buckets, records its cutoff bucket and clears its row for the next group.
-// Pixels that have a cutoff are skipped by later groups.  The emission pass
-// then drops pixel intersections whose bucket lies behind the pixel's cutoff.
+// This build's form: the cutoff of a pixel is the first bucket where the
+// bucket-ordered prefix sum of its thickness reaches the threshold, and integer
+// sums (1/65536 fixed point, 64-bit) do not depend on the order the GPU adds
+// them in, so any grouping of the buckets gives the same cutoffs (the reference
+// sums floats with atomics, group by group).  Two levels instead of one pass per
+// group: the records pass writes the visible records and their radii; a coarse
+// pass adds every (pixel, record) thickness into [pixels x C] columns of F
+// buckets each; a per-pixel scan finds the column where the running sum
+// reaches the threshold; a fine pass adds only the thickness that falls in that
+// column, per bucket, into [pixels x F]; a second scan finds the bucket.  The
+// records are read twice and no per-group table is swept (the per-group form
+// read every record and swept a [pixels x group] table once per group of
+// buckets).  The emission pass then drops pixel intersections whose bucket
+// lies behind the pixel's cutoff.
 #include <hip/hip_runtime.h>
 
 #include "vgr_internal.h"
@@ -45,69 +49,118 @@ __global__ void dense_camera_map_kernel(const CameraParams* __restrict__ table, 
   if (camera >= 0 && camera < batch) map[camera] = a;
 }
 
-// Thickness of the records whose depth bucket lies in [group_start, group_start + group_size).
-__global__ __launch_bounds__(kDenseThreads) void dense_tau_kernel(
-    const vgr_record* __restrict__ records, const int32_t* __restrict__ radii, const int64_t* __restrict__ counters,
-    int64_t visible_capacity, const CameraParams* __restrict__ table, const int32_t* __restrict__ camera_map,
-    int batch, float near_plane, float far_plane, int buckets, int group_start, int group_size,
-    const int32_t* __restrict__ cutoff, uint32_t* __restrict__ tau) {
-  const int64_t visible = counters[VGR_COUNTER_VISIBLE] < visible_capacity ? counters[VGR_COUNTER_VISIBLE]
-                                                                          : visible_capacity;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < visible; r += stride) {
-    const float4* row = reinterpret_cast<const float4*>(records + r);
-    const float4 a = row[0], b = row[1], c = row[2];
-    const int local = depth_bucket(b.z, near_plane, far_plane, buckets) - group_start;
-    if (local < 0 || local >= group_size) continue;
-    const int camera = __float_as_int(c.w);
-    if (camera < 0 || camera >= batch) continue;
-    const int active = camera_map[camera];
-    if (active < 0) continue;
-    const CameraParams& cam = table[active];
-    Projected p;
-    p.mean_x = a.x;
-    p.mean_y = a.y;
-    p.conic_0 = a.z;
-    p.conic_1 = a.w;
-    p.conic_2 = b.x;
-    p.opacity = b.y;
-    p.depth = b.z;
-    p.radius_x = radii[2 * r + 0];
-    p.radius_y = radii[2 * r + 1];
-    const TileRect rect = tile_rect(p, 1, cam.tiles_x, cam.tiles_y);
-    for (int y = rect.min_y; y < rect.max_y; ++y) {
-      for (int x = rect.min_x; x < rect.max_x; ++x) {
-        const int64_t pixel = static_cast<int64_t>(cam.tile_base) + static_cast<int64_t>(y) * cam.tiles_x + x;
-        if (cutoff[pixel] >= 0) continue;
-        const float alpha = pixel_alpha(p, x, y);
-        if (alpha == 0.0f) continue;
-        atomicAdd(&tau[pixel * group_size + local], optical_thickness_fixed(alpha));
-      }
+// Calls visit(pixel, bucket, thickness) for every pixel a visible record reaches
+// with nonzero alpha.
+template <typename Visit>
+__device__ __forceinline__ void for_record_pixels(const vgr_record* __restrict__ records,
+                                                  const int32_t* __restrict__ radii, int64_t r,
+                                                  const CameraParams* __restrict__ table,
+                                                  const int32_t* __restrict__ camera_map, int batch, float near_plane,
+                                                  float far_plane, int buckets, Visit&& visit) {
+  const float4* row = reinterpret_cast<const float4*>(records + r);
+  const float4 a = row[0], b = row[1], c = row[2];
+  const int camera = __float_as_int(c.w);
+  if (camera < 0 || camera >= batch) return;
+  const int active = camera_map[camera];
+  if (active < 0) return;
+  const CameraParams& cam = table[active];
+  const int bucket = depth_bucket(b.z, near_plane, far_plane, buckets);
+  Projected p;
+  p.mean_x = a.x;
+  p.mean_y = a.y;
+  p.conic_0 = a.z;
+  p.conic_1 = a.w;
+  p.conic_2 = b.x;
+  p.opacity = b.y;
+  p.depth = b.z;
+  p.radius_x = radii[2 * r + 0];
+  p.radius_y = radii[2 * r + 1];
+  const TileRect rect = tile_rect(p, 1, cam.tiles_x, cam.tiles_y);
+  for (int y = rect.min_y; y < rect.max_y; ++y) {
+    for (int x = rect.min_x; x < rect.max_x; ++x) {
+      const float alpha = pixel_alpha(p, x, y);
+      if (alpha == 0.0f) continue;
+      visit(static_cast<int64_t>(cam.tile_base) + static_cast<int64_t>(y) * cam.tiles_x + x, bucket,
+            optical_thickness_fixed(alpha));
     }
   }
 }
 
-// Running sums of the open pixels over the group's buckets; the first bucket that
-// reaches the threshold becomes the pixel's cutoff (the last bucket when none
-// does by the end).  The group rows are cleared for the next group.
-__global__ __launch_bounds__(kDenseThreads) void dense_advance_kernel(uint32_t* __restrict__ tau,
-                                                                      uint64_t* __restrict__ accumulated,
-                                                                      int32_t* __restrict__ cutoff, int64_t pixels,
-                                                                      int group_start, int group_count,
-                                                                      int group_size, int buckets) {
-  const int64_t pixel = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (pixel >= pixels || cutoff[pixel] >= 0) return;
-  uint64_t sum = accumulated[pixel];
-  int found = -1;
-  uint32_t* row = tau + pixel * group_size;
-  for (int b = 0; b < group_count; ++b) {
-    sum += row[b];
-    row[b] = 0u;
-    if (found < 0 && sum >= kTauThresholdFixed) found = group_start + b;
+__device__ __forceinline__ int64_t visible_records(const int64_t* counters, int64_t visible_capacity) {
+  return counters[VGR_COUNTER_VISIBLE] < visible_capacity ? counters[VGR_COUNTER_VISIBLE] : visible_capacity;
+}
+
+// Coarse level: thickness per (pixel, column of F buckets).
+__global__ __launch_bounds__(kDenseThreads) void dense_coarse_kernel(
+    const vgr_record* __restrict__ records, const int32_t* __restrict__ radii, const int64_t* __restrict__ counters,
+    int64_t visible_capacity, const CameraParams* __restrict__ table, const int32_t* __restrict__ camera_map,
+    int batch, float near_plane, float far_plane, int buckets, int fine, int columns,
+    unsigned long long* __restrict__ coarse) {
+  const int64_t visible = visible_records(counters, visible_capacity);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < visible; r += stride) {
+    for_record_pixels(records, radii, r, table, camera_map, batch, near_plane, far_plane, buckets,
+                      [&](int64_t pixel, int bucket, uint32_t tau) {
+                        atomicAdd(&coarse[pixel * columns + bucket / fine], static_cast<unsigned long long>(tau));
+                      });
   }
-  if (found < 0 && group_start + group_count >= buckets) found = buckets - 1;
-  accumulated[pixel] = sum;
-  cutoff[pixel] = found;
+}
+
+// Per pixel: the column where the running sum reaches the threshold and the sum
+// before it; a pixel that never reaches it keeps every bucket.
+__global__ __launch_bounds__(kDenseThreads) void dense_coarse_scan_kernel(
+    const unsigned long long* __restrict__ coarse, int64_t pixels, int columns, int buckets,
+    int32_t* __restrict__ crossing, uint64_t* __restrict__ before, int32_t* __restrict__ cutoff) {
+  const int64_t pixel = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (pixel >= pixels) return;
+  uint64_t sum = 0;
+  int found = -1;
+  for (int c = 0; c < columns; ++c) {
+    const uint64_t next = sum + coarse[pixel * columns + c];
+    if (next >= kTauThresholdFixed) {
+      found = c;
+      break;
+    }
+    sum = next;
+  }
+  crossing[pixel] = found;
+  before[pixel] = sum;
+  cutoff[pixel] = buckets - 1;  // refined by the fine scan where a column crosses
+}
+
+// Fine level: thickness per bucket of the pixel's crossing column only.
+__global__ __launch_bounds__(kDenseThreads) void dense_fine_kernel(
+    const vgr_record* __restrict__ records, const int32_t* __restrict__ radii, const int64_t* __restrict__ counters,
+    int64_t visible_capacity, const CameraParams* __restrict__ table, const int32_t* __restrict__ camera_map,
+    int batch, float near_plane, float far_plane, int buckets, int fine, const int32_t* __restrict__ crossing,
+    unsigned long long* __restrict__ cells) {
+  const int64_t visible = visible_records(counters, visible_capacity);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < visible; r += stride) {
+    for_record_pixels(records, radii, r, table, camera_map, batch, near_plane, far_plane, buckets,
+                      [&](int64_t pixel, int bucket, uint32_t tau) {
+                        const int column = bucket / fine;
+                        if (crossing[pixel] != column) return;
+                        atomicAdd(&cells[pixel * fine + (bucket - column * fine)], static_cast<unsigned long long>(tau));
+                      });
+  }
+}
+
+__global__ __launch_bounds__(kDenseThreads) void dense_fine_scan_kernel(
+    const unsigned long long* __restrict__ cells, int64_t pixels, int fine, int buckets,
+    const int32_t* __restrict__ crossing, const uint64_t* __restrict__ before, int32_t* __restrict__ cutoff) {
+  const int64_t pixel = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (pixel >= pixels) return;
+  const int column = crossing[pixel];
+  if (column < 0) return;
+  uint64_t sum = before[pixel];
+  for (int b = 0; b < fine && column * fine + b < buckets; ++b) {
+    sum += cells[pixel * fine + b];
+    if (sum >= kTauThresholdFixed) {
+      cutoff[pixel] = column * fine + b;
+      return;
+    }
+  }
 }
 
 }  // namespace
@@ -117,19 +170,20 @@ int launch_dense_cutoff(const vgr_cameras& cameras, const vgr_options& options, 
   const CameraParams* table = reinterpret_cast<const CameraParams*>(scratch + layout.camera_table);
   int32_t* camera_map = reinterpret_cast<int32_t*>(scratch + layout.dense_camera_map);
   int32_t* cutoff = reinterpret_cast<int32_t*>(scratch + layout.dense_cutoff);
-  uint64_t* accumulated = reinterpret_cast<uint64_t*>(scratch + layout.dense_accumulated);
-  uint32_t* tau = reinterpret_cast<uint32_t*>(scratch + layout.dense_tau);
+  uint64_t* before = reinterpret_cast<uint64_t*>(scratch + layout.dense_accumulated);
+  int32_t* crossing = reinterpret_cast<int32_t*>(scratch + layout.dense_crossing);
+  unsigned long long* coarse = reinterpret_cast<unsigned long long*>(scratch + layout.dense_tau);
   const int64_t pixels = layout.dense_pixels;
   const int buckets = options.depth_bucket_count;
-  const int group_size = layout.dense_group_size;
+  const int fine = layout.dense_fine;
+  const int columns = layout.dense_columns;
+  unsigned long long* cells = coarse + pixels * columns;
   if (ws.radii == nullptr) {
     set_error("the depth cutoff needs workspace.radii");
     return VGR_E_INVALID_ARGUMENT;
   }
   int status = launch_fill_u32(reinterpret_cast<uint32_t*>(camera_map), cameras.batch, 0xffffffffu, stream);
-  if (status == 0) status = launch_fill_u32(reinterpret_cast<uint32_t*>(cutoff), pixels, 0xffffffffu, stream);
-  if (status == 0) status = launch_fill_i64(reinterpret_cast<int64_t*>(accumulated), pixels, 0, stream);
-  if (status == 0) status = launch_fill_u32(tau, pixels * group_size, 0u, stream);
+  if (status == 0) status = launch_fill_i64(reinterpret_cast<int64_t*>(coarse), pixels * (columns + fine), 0, stream);
   if (status != 0) {
     set_error("depth cutoff fills failed");
     return VGR_E_LAUNCH;
@@ -142,15 +196,17 @@ int launch_dense_cutoff(const vgr_cameras& cameras, const vgr_options& options, 
   int64_t record_blocks = (ws.visible_capacity + kDenseThreads - 1) / kDenseThreads;
   record_blocks = record_blocks < 1 ? 1 : (record_blocks > 65536 ? 65536 : record_blocks);
   const int64_t pixel_blocks = (pixels + kDenseThreads - 1) / kDenseThreads;
-  for (int start = 0; start < buckets; start += group_size) {
-    const int count = buckets - start < group_size ? buckets - start : group_size;
-    hipLaunchKernelGGL(dense_tau_kernel, dim3(static_cast<unsigned>(record_blocks)), dim3(kDenseThreads), 0, stream,
+  if (pixel_blocks > 0) {
+    hipLaunchKernelGGL(dense_coarse_kernel, dim3(static_cast<unsigned>(record_blocks)), dim3(kDenseThreads), 0,
+                       stream, ws.records, ws.radii, ws.counters, ws.visible_capacity, table, camera_map,
+                       cameras.batch, options.near_plane, options.far_plane, buckets, fine, columns, coarse);
+    hipLaunchKernelGGL(dense_coarse_scan_kernel, dim3(static_cast<unsigned>(pixel_blocks)), dim3(kDenseThreads), 0,
+                       stream, coarse, pixels, columns, buckets, crossing, before, cutoff);
+    hipLaunchKernelGGL(dense_fine_kernel, dim3(static_cast<unsigned>(record_blocks)), dim3(kDenseThreads), 0, stream,
                        ws.records, ws.radii, ws.counters, ws.visible_capacity, table, camera_map, cameras.batch,
-                       options.near_plane, options.far_plane, buckets, start, group_size, cutoff, tau);
-    if (pixel_blocks > 0) {
-      hipLaunchKernelGGL(dense_advance_kernel, dim3(static_cast<unsigned>(pixel_blocks)), dim3(kDenseThreads), 0,
-                         stream, tau, accumulated, cutoff, pixels, start, count, group_size, buckets);
-    }
+                       options.near_plane, options.far_plane, buckets, fine, crossing, cells);
+    hipLaunchKernelGGL(dense_fine_scan_kernel, dim3(static_cast<unsigned>(pixel_blocks)), dim3(kDenseThreads), 0,
+                       stream, cells, pixels, fine, buckets, crossing, before, cutoff);
   }
   const hipError_t error = hipGetLastError();
   if (error != hipSuccess) {

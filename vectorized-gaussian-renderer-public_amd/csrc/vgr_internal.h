@@ -71,10 +71,12 @@ struct ScratchLayout {
   // Dense (tile size 1) depth cutoff only (zero-sized otherwise).
   size_t dense_camera_map;   // int32[batch]: camera -> active index
   size_t dense_cutoff;       // int32[pixels]: last depth bucket emitted per pixel (-1 while open)
-  size_t dense_accumulated;  // uint64[pixels]: running fixed-point optical thickness
-  size_t dense_tau;          // uint32[pixels * group]: per-bucket thickness of the current group
+  size_t dense_accumulated;  // uint64[pixels]: thickness of the buckets before the crossing column
+  size_t dense_crossing;     // int32[pixels]: column where the running thickness reaches the threshold
+  size_t dense_tau;          // uint64[pixels * (columns + fine)]: coarse column sums, then fine bucket sums
   int64_t dense_pixels;
-  int32_t dense_group_size;
+  int32_t dense_fine;        // buckets per coarse column
+  int32_t dense_columns;     // coarse columns: ceil(buckets / dense_fine)
   size_t total;
   bool culling;
   bool front;
