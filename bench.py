@@ -429,6 +429,8 @@ def bench_c4_fwd_bwd(args, torch, device) -> dict:
     return {"workload": f"{cameras} cameras {size}x{size}, 500k Gaussians SH3 FP32, L1 + depth-L1 loss, grads of "
                         "means/log-scales/quats/opacity logits/SH and the 6-DoF camera pose (se(3) twist on the base views)",
             "fwd_ms": round(fwd, 3), "fwd_bwd_ms": round(both, 3), "fwd_bwd_over_fwd": round(both / fwd, 3),
+            "gpu_and_wall_ms": {"fwd": [round(fwd_ms, 3), round(fwd_wall, 3)],
+                                "fwd_bwd": [round(step_ms, 3), round(step_wall, 3)]},
             "fwd_mpix_s": round(pixels / 1e6 / (fwd / 1e3), 1),
             "fwd_bwd_mpix_s": round(pixels / 1e6 / (both / 1e3), 1),
             "counters": renderer.last_counters, "grads_finite": finite, "oracle_parity": parity,
