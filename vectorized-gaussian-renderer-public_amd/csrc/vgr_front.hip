@@ -214,9 +214,16 @@ __global__ __launch_bounds__(kBlockThreads) void front_write_kernel(ProjectionAr
                                                                     uint32_t depth_base,
                                                                     int64_t* __restrict__ counters) {
   extern __shared__ uint32_t s_dynamic[];
-  uint32_t* s_count = s_dynamic;                      // [camera_tile_limit] entries per tile of this workgroup
-  uint32_t* s_base = s_dynamic + camera_tile_limit;   // [camera_tile_limit] first slot of the workgroup's run
-  constexpr int kRanks = 4;  // splats covering up to four tiles take LDS ranks; larger ones reserve per tile
+  uint32_t* s_count = s_dynamic;                          // [camera_tile_limit] ranked entries per tile
+  uint32_t* s_base = s_dynamic + camera_tile_limit;       // [camera_tile_limit] first slot of the workgroup's run
+  uint32_t* s_large = s_dynamic + 2 * camera_tile_limit;  // [camera_tile_limit] entries of larger splats
+  uint32_t* s_cursor = s_dynamic + 3 * camera_tile_limit;  // [camera_tile_limit] next slot for larger splats
+  // Splats covering up to four tiles keep their LDS ranks in registers; larger
+  // ones are counted per tile and take their slots from an LDS cursor after the
+  // ranked entries of the run, so every intersection of the workgroup is placed
+  // with one global reservation per (camera, tile) and no per-intersection
+  // global atomic.
+  constexpr int kRanks = 4;
   int64_t block;
   uint64_t mask;
   if (!resolve_block(args, block, mask)) return;
@@ -229,7 +236,10 @@ __global__ __launch_bounds__(kBlockThreads) void front_write_kernel(ProjectionAr
   // evaluated by the wave; it still joins the workgroup's barriers.
   const uint64_t wave_cameras = wave_masks[static_cast<int64_t>(blockIdx.x) * (kBlockThreads / kWave) +
                                            threadIdx.x / kWave];
-  for (int i = threadIdx.x; i < camera_tile_limit; i += kBlockThreads) s_count[i] = 0u;
+  for (int i = threadIdx.x; i < camera_tile_limit; i += kBlockThreads) {
+    s_count[i] = 0u;
+    s_large[i] = 0u;
+  }
   __syncthreads();
   bool loaded;
   float m0, m1, m2, opacity, cov[6];
@@ -264,14 +274,20 @@ __global__ __launch_bounds__(kBlockThreads) void front_write_kernel(ProjectionAr
           ++k;
         }
       }
+    } else if (seen) {
+      for (int ty = rect.min_y; ty < rect.max_y; ++ty) {
+        for (int tx = rect.min_x; tx < rect.max_x; ++tx) atomicAdd(&s_large[ty * cam.tiles_x + tx], 1u);
+      }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < camera_tiles; i += kBlockThreads) {
-      const uint32_t c = s_count[i];
+      const uint32_t c = s_count[i] + s_large[i];
       if (c != 0u) {
         const unsigned long long base = atomicAdd(&cursors[cam.tile_base + i], static_cast<unsigned long long>(c));
         s_base[i] = base < 0xffffffffull ? static_cast<uint32_t>(base) : 0xffffffffu;
+        s_cursor[i] = s_count[i];  // larger splats fill the run after the ranked entries
         s_count[i] = 0u;
+        s_large[i] = 0u;
       }
     }
     __syncthreads();
@@ -282,17 +298,16 @@ __global__ __launch_bounds__(kBlockThreads) void front_write_kernel(ProjectionAr
       for (int ty = rect.min_y; ty < rect.max_y; ++ty) {
         for (int tx = rect.min_x; tx < rect.max_x; ++tx) {
           const int local = ty * cam.tiles_x + tx;
-          uint64_t slot;
+          uint32_t r = 0u;
           if (ranked) {
-            uint32_t r = 0u;
 #pragma unroll
             for (int q = 0; q < kRanks; ++q) {
               if (q == k) r = rank[q];
             }
-            slot = static_cast<uint64_t>(s_base[local]) + r;
           } else {
-            slot = atomicAdd(&cursors[cam.tile_base + local], 1ull);
+            r = atomicAdd(&s_cursor[local], 1u);
           }
+          const uint64_t slot = static_cast<uint64_t>(s_base[local]) + r;
           if (slot < static_cast<uint64_t>(capacity)) {
             keys[slot] = key;
             values[slot] = value;
@@ -725,7 +740,7 @@ int launch_front_projection(const vgr_scene& scene, const vgr_cameras& cameras, 
   hipLaunchKernelGGL(front_totals_kernel, dim3(1), dim3(1024), 0, stream, block_visible, blocks,
                      culled ? args.item_count : nullptr, lengths, global_tiles, ws.counters);
   if (blocks > 0) {
-    const size_t write_lds = 2 * sizeof(uint32_t) * static_cast<size_t>(layout.front_camera_tiles);
+    const size_t write_lds = 4 * sizeof(uint32_t) * static_cast<size_t>(layout.front_camera_tiles);
     hipLaunchKernelGGL(front_write_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlockThreads), write_lds,
                        stream, args, static_cast<int>(layout.front_camera_tiles), block_masks, wave_masks, cursors,
                        reinterpret_cast<uint32_t*>(ws.keys_in), ws.values_in, ws.intersection_capacity,
