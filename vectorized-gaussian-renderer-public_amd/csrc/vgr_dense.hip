@@ -49,59 +49,77 @@ __global__ void dense_camera_map_kernel(const CameraParams* __restrict__ table, 
   if (camera >= 0 && camera < batch) map[camera] = a;
 }
 
-// Calls visit(pixel, bucket, thickness) for every pixel a visible record reaches
-// with nonzero alpha.
-template <typename Visit>
-__device__ __forceinline__ void for_record_pixels(const vgr_record* __restrict__ records,
-                                                  const int32_t* __restrict__ radii, int64_t r,
-                                                  const CameraParams* __restrict__ table,
-                                                  const int32_t* __restrict__ camera_map, int batch, float near_plane,
-                                                  float far_plane, int buckets, Visit&& visit) {
+// One visible record per lane, rebuilt for the cooperative pixel walk: its
+// camera's tile base and width, the splat, its pixel rectangle and depth bucket.
+struct RecordSplat {
+  bool valid;
+  uint32_t tile_base;
+  int tiles_x;
+  int bucket;
+  Projected p;
+  TileRect rect;
+};
+
+__device__ __forceinline__ RecordSplat load_record_splat(const vgr_record* __restrict__ records,
+                                                         const int32_t* __restrict__ radii, int64_t r,
+                                                         int64_t visible, const CameraParams* __restrict__ table,
+                                                         const int32_t* __restrict__ camera_map, int batch,
+                                                         float near_plane, float far_plane, int buckets) {
+  RecordSplat s;
+  s.valid = false;
+  s.tile_base = 0;
+  s.tiles_x = 0;
+  s.bucket = 0;
+  if (r >= visible) return s;
   const float4* row = reinterpret_cast<const float4*>(records + r);
   const float4 a = row[0], b = row[1], c = row[2];
   const int camera = __float_as_int(c.w);
-  if (camera < 0 || camera >= batch) return;
+  if (camera < 0 || camera >= batch) return s;
   const int active = camera_map[camera];
-  if (active < 0) return;
+  if (active < 0) return s;
   const CameraParams& cam = table[active];
-  const int bucket = depth_bucket(b.z, near_plane, far_plane, buckets);
-  Projected p;
-  p.mean_x = a.x;
-  p.mean_y = a.y;
-  p.conic_0 = a.z;
-  p.conic_1 = a.w;
-  p.conic_2 = b.x;
-  p.opacity = b.y;
-  p.depth = b.z;
-  p.radius_x = radii[2 * r + 0];
-  p.radius_y = radii[2 * r + 1];
-  const TileRect rect = tile_rect(p, 1, cam.tiles_x, cam.tiles_y);
-  for (int y = rect.min_y; y < rect.max_y; ++y) {
-    for (int x = rect.min_x; x < rect.max_x; ++x) {
-      const float alpha = pixel_alpha(p, x, y);
-      if (alpha == 0.0f) continue;
-      visit(static_cast<int64_t>(cam.tile_base) + static_cast<int64_t>(y) * cam.tiles_x + x, bucket,
-            optical_thickness_fixed(alpha));
-    }
-  }
+  s.p.mean_x = a.x;
+  s.p.mean_y = a.y;
+  s.p.conic_0 = a.z;
+  s.p.conic_1 = a.w;
+  s.p.conic_2 = b.x;
+  s.p.opacity = b.y;
+  s.p.depth = b.z;
+  s.p.radius_x = radii[2 * r + 0];
+  s.p.radius_y = radii[2 * r + 1];
+  s.rect = tile_rect(s.p, 1, cam.tiles_x, cam.tiles_y);
+  s.tile_base = cam.tile_base;
+  s.tiles_x = cam.tiles_x;
+  s.bucket = depth_bucket(b.z, near_plane, far_plane, buckets);
+  s.valid = true;
+  return s;
 }
 
 __device__ __forceinline__ int64_t visible_records(const int64_t* counters, int64_t visible_capacity) {
   return counters[VGR_COUNTER_VISIBLE] < visible_capacity ? counters[VGR_COUNTER_VISIBLE] : visible_capacity;
 }
 
-// Coarse level: thickness per (pixel, column of F buckets).
+// Coarse level: thickness per (pixel, column of F buckets).  Each wave takes 64
+// records at a time and walks their pixels together.
 __global__ __launch_bounds__(kDenseThreads) void dense_coarse_kernel(
     const vgr_record* __restrict__ records, const int32_t* __restrict__ radii, const int64_t* __restrict__ counters,
     int64_t visible_capacity, const CameraParams* __restrict__ table, const int32_t* __restrict__ camera_map,
     int batch, float near_plane, float far_plane, int buckets, int fine, int columns,
     unsigned long long* __restrict__ coarse) {
   const int64_t visible = visible_records(counters, visible_capacity);
+  const int lane = threadIdx.x & (kWave - 1);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < visible; r += stride) {
-    for_record_pixels(records, radii, r, table, camera_map, batch, near_plane, far_plane, buckets,
-                      [&](int64_t pixel, int bucket, uint32_t tau) {
-                        atomicAdd(&coarse[pixel * columns + bucket / fine], static_cast<unsigned long long>(tau));
+  for (int64_t first = static_cast<int64_t>(blockIdx.x) * blockDim.x + (threadIdx.x - lane); first < visible;
+       first += stride) {  // uniform over the wave
+    const RecordSplat s = load_record_splat(records, radii, first + lane, visible, table, camera_map, batch,
+                                            near_plane, far_plane, buckets);
+    dense_wave_pixels(s.valid, s.p, s.rect, s.tile_base, s.tiles_x, s.bucket, nullptr,
+                      [&](bool keep, int, int owner, uint32_t pixel, float alpha) {
+                        const int column = __shfl(s.bucket, owner, kWave) / fine;
+                        if (keep) {
+                          atomicAdd(&coarse[static_cast<int64_t>(pixel) * columns + column],
+                                    static_cast<unsigned long long>(optical_thickness_fixed(alpha)));
+                        }
                       });
   }
 }
@@ -135,13 +153,20 @@ __global__ __launch_bounds__(kDenseThreads) void dense_fine_kernel(
     int batch, float near_plane, float far_plane, int buckets, int fine, const int32_t* __restrict__ crossing,
     unsigned long long* __restrict__ cells) {
   const int64_t visible = visible_records(counters, visible_capacity);
+  const int lane = threadIdx.x & (kWave - 1);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < visible; r += stride) {
-    for_record_pixels(records, radii, r, table, camera_map, batch, near_plane, far_plane, buckets,
-                      [&](int64_t pixel, int bucket, uint32_t tau) {
+  for (int64_t first = static_cast<int64_t>(blockIdx.x) * blockDim.x + (threadIdx.x - lane); first < visible;
+       first += stride) {  // uniform over the wave
+    const RecordSplat s = load_record_splat(records, radii, first + lane, visible, table, camera_map, batch,
+                                            near_plane, far_plane, buckets);
+    dense_wave_pixels(s.valid, s.p, s.rect, s.tile_base, s.tiles_x, s.bucket, nullptr,
+                      [&](bool keep, int, int owner, uint32_t pixel, float alpha) {
+                        const int bucket = __shfl(s.bucket, owner, kWave);
                         const int column = bucket / fine;
-                        if (crossing[pixel] != column) return;
-                        atomicAdd(&cells[pixel * fine + (bucket - column * fine)], static_cast<unsigned long long>(tau));
+                        if (keep && crossing[pixel] == column) {
+                          atomicAdd(&cells[static_cast<int64_t>(pixel) * fine + (bucket - column * fine)],
+                                    static_cast<unsigned long long>(optical_thickness_fixed(alpha)));
+                        }
                       });
   }
 }

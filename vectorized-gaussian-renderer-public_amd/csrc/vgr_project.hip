@@ -184,13 +184,23 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
     TileRect rect;
     int tiles;
     bool seen;
-    if constexpr (kMode == 1) {
-      seen = evaluate_dense_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
-    } else if constexpr (kMode == 2) {
+    if constexpr (kMode == 2) {
       RayData ray;
       seen = evaluate_ray_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles, ray);
     } else {
       seen = evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+    }
+    if constexpr (kMode == 1) {
+      // Pixel intersections, counted by the wave together (added once, on lane 0).
+      tiles = 0;
+      if (args.dense_mode != 1) {
+        const int bucket = seen && args.depth_cutoff != nullptr
+                               ? depth_bucket(p.depth, args.near_plane, args.far_plane, args.depth_buckets)
+                               : 0;
+        const int wave_pixels = dense_wave_pixels(seen, p, rect, cam.tile_base, cam.tiles_x, bucket,
+                                                  args.depth_cutoff, [](bool, int, int, uint32_t, float) {});
+        if ((threadIdx.x & (kWave - 1)) == 0) intersections += wave_pixels;
+      }
     }
     if (seen) {
       visible += 1;
@@ -251,7 +261,8 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
     RayData ray;
     bool visible;
     if constexpr (kMode == 1) {
-      visible = evaluate_dense_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+      visible = evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles);
+      tiles = 0;  // pixel intersections are placed by the wave together below
     } else if constexpr (kMode == 2) {
       visible = evaluate_ray_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles, ray);
     } else {
@@ -285,21 +296,30 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
       }
     }
     if constexpr (kDense) {
-      // Pixel intersections: each lane writes its own pixels from its scanned offset.
-      if (visible && tiles > 0) {
+      // Pixel intersections, placed by the wave together in (lane, row-major)
+      // order from the wave's scanned offset: the slots of a per-lane loop.
+      if (args.dense_mode != 1) {
         const int64_t record = visible_base + inclusive_visible - 1;
-        const int32_t value = record < visible_capacity ? static_cast<int32_t>(record) : -1;
-        const uint32_t depth_bits = vgr_float_bits(p.depth);
-        const int64_t first_slot = intersection_base + inclusive_tiles - tiles;
-        dense_pixels(args, cam, p, rect, [&](int k, uint32_t pixel) {
-          const int64_t slot = first_slot + k;
-          if (slot >= intersection_capacity) return;
-          keys[slot] = value >= 0 ? ((static_cast<uint64_t>(pixel) << 32) | depth_bits) : kSentinelKey;
-          values[slot] = value;
-        });
+        const int32_t value = visible && record < visible_capacity ? static_cast<int32_t>(record) : -1;
+        const uint32_t depth_bits = visible ? vgr_float_bits(p.depth) : 0u;
+        const int bucket = visible && args.depth_cutoff != nullptr
+                               ? depth_bucket(p.depth, args.near_plane, args.far_plane, args.depth_buckets)
+                               : 0;
+        const int wave_pixels = dense_wave_pixels(
+            visible, p, rect, cam.tile_base, cam.tiles_x, bucket, args.depth_cutoff,
+            [&](bool keep, int index, int owner, uint32_t pixel, float) {
+              const int32_t owner_value = __shfl(value, owner, kWave);
+              const uint32_t owner_depth = static_cast<uint32_t>(__shfl(static_cast<int>(depth_bits), owner, kWave));
+              const int64_t slot = intersection_base + index;
+              if (keep && slot < intersection_capacity) {
+                keys[slot] =
+                    owner_value >= 0 ? ((static_cast<uint64_t>(pixel) << 32) | owner_depth) : kSentinelKey;
+                values[slot] = owner_value;
+              }
+            });
+        intersection_base += wave_pixels;
       }
       visible_base += total_visible;
-      intersection_base += total_tiles;
       continue;
     }
     // Tile intersections of the wave: the wave writes its total_tiles slots as

@@ -452,26 +452,74 @@ __device__ __forceinline__ bool evaluate_pair(const ProjectionArgs& args, const 
   return true;
 }
 
-// Pixel intersections of a visible pair at tile size 1: the pixels of its rectangle
-// where its alpha reaches 1/255 and (with a cutoff) whose cutoff bucket is not in
-// front of the pair's depth bucket (renderer_cuda.cu:1764-1805).  `emit(slot, pixel)`
-// receives them in row-major order.
-template <typename Emit>
-__device__ __forceinline__ int dense_pixels(const ProjectionArgs& args, const CameraParams& cam, const Projected& p,
-                                            const TileRect& rect, Emit&& emit) {
-  const int bucket =
-      args.depth_cutoff != nullptr ? depth_bucket(p.depth, args.near_plane, args.far_plane, args.depth_buckets) : 0;
-  int count = 0;
-  for (int y = rect.min_y; y < rect.max_y; ++y) {
-    for (int x = rect.min_x; x < rect.max_x; ++x) {
-      if (pixel_alpha(p, x, y) == 0.0f) continue;
-      const uint32_t pixel = cam.tile_base + static_cast<uint32_t>(y * cam.tiles_x + x);
-      if (args.depth_cutoff != nullptr && bucket > args.depth_cutoff[pixel]) continue;
-      emit(count, pixel);
-      ++count;
-    }
+
+// Per-pixel intersections of a wave's splats at tile size 1 (renderer_cuda.cu:1764-1805:
+// the pixels of a splat's rectangle where its alpha reaches 1/255), found cooperatively:
+// the wave walks the union of its splats' rectangles 64 pixels at a time in
+// (lane, row-major) order, pixel slot j going to the lane whose scanned
+// rectangle range holds it (binary search over the lanes' inclusive areas), so
+// large and small splats share the work.  visit(keep, index, owner, pixel, alpha)
+// is called by every lane for every 64-slot step (uniform control flow, so it
+// may shuffle): keep marks a pixel where the owner's alpha reaches 1/255 and that
+// lies in front of the pixel's depth cutoff (when cutoff is not null), index is
+// its position among the wave's kept pixels in that order — the order a per-lane
+// row-major loop would emit them in.  Per-lane inputs: the splat (seen), its
+// pixel rectangle, its camera's tile base and width in tiles, its depth bucket.
+// Returns the wave's kept total.
+template <typename Visit>
+__device__ __forceinline__ int dense_wave_pixels(bool seen, const Projected& p, const TileRect& rect,
+                                                 uint32_t tile_base, int tiles_x, int bucket,
+                                                 const int32_t* __restrict__ cutoff, Visit&& visit) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int width = seen ? rect.max_x - rect.min_x : 0;
+  const int area = seen ? width * (rect.max_y - rect.min_y) : 0;
+  int inclusive = area;
+#pragma unroll
+  for (int offset = 1; offset < kWave; offset <<= 1) {
+    const int t = __shfl_up(inclusive, offset, kWave);
+    if (lane >= offset) inclusive += t;
   }
-  return count;
+  const int total = __shfl(inclusive, kWave - 1, kWave);
+  const int exclusive = inclusive - area;
+  const int min_x = seen ? rect.min_x : 0;
+  const int min_y = seen ? rect.min_y : 0;
+  const float mean_x = seen ? p.mean_x : 0.0f, mean_y = seen ? p.mean_y : 0.0f;
+  const float conic_0 = seen ? p.conic_0 : 0.0f, conic_1 = seen ? p.conic_1 : 0.0f;
+  const float conic_2 = seen ? p.conic_2 : 0.0f, opacity = seen ? p.opacity : 0.0f;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  int kept = 0;
+  for (int run = 0; run < total; run += kWave) {  // uniform over the wave
+    const int j = run + lane;
+    int owner = 0;
+#pragma unroll
+    for (int step = kWave / 2; step > 0; step >>= 1) {
+      if (__shfl(inclusive, owner + step - 1, kWave) <= j) owner += step;
+    }
+    const bool in = j < total;
+    const int local = j - __shfl(exclusive, owner, kWave);
+    const int row_width = __shfl(width, owner, kWave);
+    const int row = row_width > 0 ? local / row_width : 0;
+    const int x = __shfl(min_x, owner, kWave) + (local - row * row_width);
+    const int y = __shfl(min_y, owner, kWave) + row;
+    Projected q;
+    q.mean_x = __shfl(mean_x, owner, kWave);
+    q.mean_y = __shfl(mean_y, owner, kWave);
+    q.conic_0 = __shfl(conic_0, owner, kWave);
+    q.conic_1 = __shfl(conic_1, owner, kWave);
+    q.conic_2 = __shfl(conic_2, owner, kWave);
+    q.opacity = __shfl(opacity, owner, kWave);
+    const uint32_t owner_base = static_cast<uint32_t>(__shfl(static_cast<int>(tile_base), owner, kWave));
+    const int owner_tiles_x = __shfl(tiles_x, owner, kWave);
+    const int owner_bucket = __shfl(bucket, owner, kWave);
+    const float alpha = in ? pixel_alpha(q, x, y) : 0.0f;
+    const uint32_t pixel = in ? owner_base + static_cast<uint32_t>(y * owner_tiles_x + x) : 0u;
+    bool keep = alpha != 0.0f;
+    if (keep && cutoff != nullptr && owner_bucket > cutoff[pixel]) keep = false;
+    const uint64_t vote = __ballot(keep);
+    visit(keep, kept + static_cast<int>(__popcll(vote & below)), owner, pixel, alpha);
+    kept += static_cast<int>(__popcll(vote));
+  }
+  return kept;
 }
 
 // evaluate_pair for exact-ray renders: the ray extent replaces the splat's mean
@@ -492,15 +540,6 @@ __device__ __forceinline__ bool evaluate_ray_pair(const ProjectionArgs& args, co
   return true;
 }
 
-// evaluate_pair for the dense passes: the pair's intersections are its emitted
-// pixels (none in the records-only pass).
-__device__ __forceinline__ bool evaluate_dense_pair(const ProjectionArgs& args, const CameraParams& cam, int64_t g,
-                                                    bool loaded, float m0, float m1, float m2, float opacity,
-                                                    const float* cov, Projected& p, TileRect& rect, int& tiles) {
-  if (!evaluate_pair(args, cam, g, loaded, m0, m1, m2, opacity, cov, p, rect, tiles)) return false;
-  tiles = args.dense_mode == 1 ? 0 : dense_pixels(args, cam, p, rect, [](int, uint32_t) {});
-  return true;
-}
 
 __device__ __forceinline__ void load_gaussian(const ProjectionArgs& args, int64_t g, bool& loaded, float& m0,
                                               float& m1, float& m2, float& opacity, float* cov) {
