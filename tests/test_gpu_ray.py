@@ -34,6 +34,9 @@ def test_exact_ray_matches_oracle(tile_size):
 
 
 def test_exact_ray_images_do_not_depend_on_the_bin_size():
+    # Bins come from the support ellipse (double, integer radius) and the support test at a pixel
+    # runs in float: a pixel within rounding of a boundary can depend on the bin size, as in the
+    # reference. This scene has no such pixel.
     scene = synthetic.gaussian_scene(20_000, seed=3, sh_degree=1)
     rig = synthetic.orbit_rig(2, 128, 96, 110.0, seed=3, radius=(8.0, 11.0))
     sixteen, _ = render_service([(0, scene)], rig, [0, 0], ray_gaussian_evaluation=True, tile_size=16)

@@ -94,6 +94,8 @@ def test_opaque_splat_stops_after_blending():
 
 
 def test_images_do_not_depend_on_the_bin_size():
+    # Only pixels within rounding of a support boundary (ellipse in double, support test in float)
+    # can depend on the bin size; this scene has none.
     scene = synthetic.gaussian_scene(8000, seed=9, sh_degree=1)
     rig = synthetic.orbit_rig(2, 96, 72, 83.0, seed=9, radius=(9.0, 11.0))
     kwargs = dict(height=72, width=96, sh_degree=1, ray_evaluation=True)
