@@ -14,10 +14,10 @@
 // them in, so any grouping of the buckets gives the same cutoffs (the reference
 // sums floats with atomics, group by group).  Two levels instead of one pass per
 // group: the records pass writes the visible records and their radii; a coarse
-// pass adds every (pixel, record) thickness into [pixels x C] columns of F
+// pass adds every (pixel, record) thickness into [C x pixels] columns of F
 // buckets each; a per-pixel scan finds the column where the running sum
 // reaches the threshold; a fine pass adds only the thickness that falls in that
-// column, per bucket, into [pixels x F]; a second scan finds the bucket.  The
+// column, per bucket, into [F x pixels]; a second scan finds the bucket.  The
 // records are read twice and no per-group table is swept (the per-group form
 // read every record and swept a [pixels x group] table once per group of
 // buckets).  The emission pass then drops pixel intersections whose bucket
@@ -99,12 +99,14 @@ __device__ __forceinline__ int64_t visible_records(const int64_t* counters, int6
   return counters[VGR_COUNTER_VISIBLE] < visible_capacity ? counters[VGR_COUNTER_VISIBLE] : visible_capacity;
 }
 
-// Coarse level: thickness per (pixel, column of F buckets).  Each wave takes 64
-// records at a time and walks their pixels together.
+// Coarse level: thickness per (column of F buckets, pixel), column-major so the
+// lanes of a wave, which walk neighbouring pixels of one record (one column),
+// add into neighbouring words of few cache lines.  Each wave takes 64 records at
+// a time and walks their pixels together.
 __global__ __launch_bounds__(kDenseThreads) void dense_coarse_kernel(
     const vgr_record* __restrict__ records, const int32_t* __restrict__ radii, const int64_t* __restrict__ counters,
     int64_t visible_capacity, const CameraParams* __restrict__ table, const int32_t* __restrict__ camera_map,
-    int batch, float near_plane, float far_plane, int buckets, int fine, int columns,
+    int batch, float near_plane, float far_plane, int buckets, int fine, int64_t pixels,
     unsigned long long* __restrict__ coarse) {
   const int64_t visible = visible_records(counters, visible_capacity);
   const int lane = threadIdx.x & (kWave - 1);
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(kDenseThreads) void dense_coarse_kernel(
                       [&](bool keep, int, int owner, uint32_t pixel, float alpha) {
                         const int column = __shfl(s.bucket, owner, kWave) / fine;
                         if (keep) {
-                          atomicAdd(&coarse[static_cast<int64_t>(pixel) * columns + column],
+                          atomicAdd(&coarse[static_cast<int64_t>(column) * pixels + pixel],
                                     static_cast<unsigned long long>(optical_thickness_fixed(alpha)));
                         }
                       });
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(kDenseThreads) void dense_coarse_scan_kernel(
   uint64_t sum = 0;
   int found = -1;
   for (int c = 0; c < columns; ++c) {
-    const uint64_t next = sum + coarse[pixel * columns + c];
+    const uint64_t next = sum + coarse[static_cast<int64_t>(c) * pixels + pixel];
     if (next >= kTauThresholdFixed) {
       found = c;
       break;
@@ -146,12 +148,13 @@ __global__ __launch_bounds__(kDenseThreads) void dense_coarse_scan_kernel(
   cutoff[pixel] = buckets - 1;  // refined by the fine scan where a column crosses
 }
 
-// Fine level: thickness per bucket of the pixel's crossing column only.
+// Fine level: thickness per (bucket of the pixel's crossing column, pixel),
+// column-major like the coarse level.
 __global__ __launch_bounds__(kDenseThreads) void dense_fine_kernel(
     const vgr_record* __restrict__ records, const int32_t* __restrict__ radii, const int64_t* __restrict__ counters,
     int64_t visible_capacity, const CameraParams* __restrict__ table, const int32_t* __restrict__ camera_map,
-    int batch, float near_plane, float far_plane, int buckets, int fine, const int32_t* __restrict__ crossing,
-    unsigned long long* __restrict__ cells) {
+    int batch, float near_plane, float far_plane, int buckets, int fine, int64_t pixels,
+    const int32_t* __restrict__ crossing, unsigned long long* __restrict__ cells) {
   const int64_t visible = visible_records(counters, visible_capacity);
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -164,7 +167,7 @@ __global__ __launch_bounds__(kDenseThreads) void dense_fine_kernel(
                         const int bucket = __shfl(s.bucket, owner, kWave);
                         const int column = bucket / fine;
                         if (keep && crossing[pixel] == column) {
-                          atomicAdd(&cells[static_cast<int64_t>(pixel) * fine + (bucket - column * fine)],
+                          atomicAdd(&cells[static_cast<int64_t>(bucket - column * fine) * pixels + pixel],
                                     static_cast<unsigned long long>(optical_thickness_fixed(alpha)));
                         }
                       });
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(kDenseThreads) void dense_fine_scan_kernel(
   if (column < 0) return;
   uint64_t sum = before[pixel];
   for (int b = 0; b < fine && column * fine + b < buckets; ++b) {
-    sum += cells[pixel * fine + b];
+    sum += cells[static_cast<int64_t>(b) * pixels + pixel];
     if (sum >= kTauThresholdFixed) {
       cutoff[pixel] = column * fine + b;
       return;
@@ -224,12 +227,12 @@ int launch_dense_cutoff(const vgr_cameras& cameras, const vgr_options& options, 
   if (pixel_blocks > 0) {
     hipLaunchKernelGGL(dense_coarse_kernel, dim3(static_cast<unsigned>(record_blocks)), dim3(kDenseThreads), 0,
                        stream, ws.records, ws.radii, ws.counters, ws.visible_capacity, table, camera_map,
-                       cameras.batch, options.near_plane, options.far_plane, buckets, fine, columns, coarse);
+                       cameras.batch, options.near_plane, options.far_plane, buckets, fine, pixels, coarse);
     hipLaunchKernelGGL(dense_coarse_scan_kernel, dim3(static_cast<unsigned>(pixel_blocks)), dim3(kDenseThreads), 0,
                        stream, coarse, pixels, columns, buckets, crossing, before, cutoff);
     hipLaunchKernelGGL(dense_fine_kernel, dim3(static_cast<unsigned>(record_blocks)), dim3(kDenseThreads), 0, stream,
                        ws.records, ws.radii, ws.counters, ws.visible_capacity, table, camera_map, cameras.batch,
-                       options.near_plane, options.far_plane, buckets, fine, crossing, cells);
+                       options.near_plane, options.far_plane, buckets, fine, pixels, crossing, cells);
     hipLaunchKernelGGL(dense_fine_scan_kernel, dim3(static_cast<unsigned>(pixel_blocks)), dim3(kDenseThreads), 0,
                        stream, cells, pixels, fine, buckets, crossing, before, cutoff);
   }
