@@ -158,6 +158,17 @@ def main() -> int:
         for index, name in enumerate(STAGES):
             stage_ms[name] += step_events[index].elapsed_time(step_events[index + 1]) / args.steps
     ms_per_step = max(gpu_ms, wall * 1000.0 / args.steps)
+    # Distribution over the timed renders: each render's span from its first stage
+    # event to its last (HIP events on the render stream), beside the bracketing
+    # event pair and the synchronized wall clock.
+    spans = sorted(step_events[0].elapsed_time(step_events[-1]) for step_events in events)
+    span_mean = sum(spans) / len(spans)
+    timing = {"renders": len(spans), "gpu_ms_mean": round(gpu_ms, 4), "wall_ms_mean": round(wall * 1000.0 / args.steps, 4),
+              "span_ms": {"mean": round(span_mean, 4),
+                          "std": round(math.sqrt(sum((x - span_mean) ** 2 for x in spans) / len(spans)), 4),
+                          "p50": round(spans[len(spans) // 2], 4),
+                          "p95": round(spans[min(len(spans) - 1, math.ceil(0.95 * len(spans)) - 1)], 4),
+                          "min": round(spans[0], 4), "max": round(spans[-1], 4)}}
     pixels = args.cameras * width * height
     value = pixels / 1.0e6 / (ms_per_step / 1000.0)
 
@@ -268,6 +279,7 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4),
+        "timing": timing,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
