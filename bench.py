@@ -19,6 +19,7 @@ import json
 import math
 import os
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -75,8 +76,8 @@ def parse_args() -> argparse.Namespace:
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: every box here holds one MI355X (asked for --gpus {args.gpus}). "
-              "The multi-GPU layout is described in DESIGN.md, not measured.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): cameras would shard across GPUs "
+              "with no exchange in the forward pass, and that layout is not measured.", file=sys.stderr)
         return 2
     import numpy as np
     import torch
@@ -238,7 +239,7 @@ def main() -> int:
                         "sample": f"first {len(sample)} of {args.cameras} cameras over the full "
                                   f"{args.gaussians}-Gaussian SH3 scene ({cpu_seconds:.1f} s)",
                         "nproc": os.cpu_count(), "cpu_model": cpu_model(),
-                        "threads_note": "16 threads: the host-CPU share of a one-GPU box",
+                        "threads_note": "16 threads: the host-CPU share for one GPU",
                         "one_core": {"value": round(len(one) * width * height / 1e6 / one_seconds, 4),
                                      "sample": f"first {len(one)} cameras, 1 thread ({one_seconds:.1f} s)"},
                         "config1_full": cpu_config1(threads)}
@@ -333,11 +334,11 @@ def oracle_parity(scene, rig, cameras: list[int], width: int, height: int, sh_de
 
 def autotune_cache() -> Path:
     """Per-run autotuner cache (the measurement itself is what the bench reports)."""
-    return ROOT / "gpurun_out" / "autotune_bench.json"
+    return Path(tempfile.gettempdir()) / f"vgr_autotune_bench_{os.getpid()}.json"
 
 
 def cpu_threads() -> int:
-    """Host threads for the CPU baseline: the box's share for one GPU (16) or fewer if absent."""
+    """Host threads for the CPU baseline: the host-CPU share for one GPU (16), or fewer if the host has fewer."""
     return max(1, min(16, os.cpu_count() or 1))
 
 
