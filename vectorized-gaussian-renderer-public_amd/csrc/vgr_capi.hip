@@ -196,6 +196,10 @@ ScratchLayout compute_layout(int32_t active_count, int64_t gaussian_count, const
   offset = align_up(offset + sizeof(uint64_t) * dense_pixels);
   layout.dense_crossing = offset;
   offset = align_up(offset + sizeof(int32_t) * dense_pixels);
+  layout.dense_block_cutoff = offset;
+  offset = align_up(offset + sizeof(int32_t) * dense_pixels);
+  layout.dense_block_crossing = offset;
+  offset = align_up(offset + sizeof(int32_t) * 2 * dense_pixels);
   layout.dense_tau = offset;
   offset = align_up(offset + sizeof(uint64_t) * dense_pixels *
                                  static_cast<size_t>(layout.dense_columns + layout.dense_fine));

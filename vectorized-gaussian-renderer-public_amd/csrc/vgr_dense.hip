@@ -148,13 +148,58 @@ __global__ __launch_bounds__(kDenseThreads) void dense_coarse_scan_kernel(
   cutoff[pixel] = buckets - 1;  // refined by the fine scan where a column crosses
 }
 
+// Per 8x8 pixel block of every active camera: the largest cutoff of its pixels
+// (kCrossing false), or the smallest and largest crossing column of its pixels
+// that cross (kCrossing true; a block with none keeps smallest > largest).  One
+// thread per block, found from a global pixel index: the camera whose pixel
+// range holds it (binary search over the ascending first pixels), then the
+// block at that offset if the camera has that many blocks.
+template <bool kCrossing>
+__global__ __launch_bounds__(kDenseThreads) void dense_block_bounds_kernel(
+    const CameraParams* __restrict__ table, int active_count, int64_t pixels, const int32_t* __restrict__ per_pixel,
+    int32_t* __restrict__ bounds) {
+  const int64_t index = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (index >= pixels || active_count <= 0) return;
+  int lo = 0, hi = active_count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (static_cast<int64_t>(table[mid].tile_base) <= index) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  const CameraParams& cam = table[lo];
+  const int64_t local = index - static_cast<int64_t>(cam.tile_base);
+  const int blocks_x = dev::dense_blocks_x(cam.tiles_x);
+  const int blocks_y = (cam.tiles_y + 7) >> 3;
+  if (cam.camera < 0 || local < 0 || local >= static_cast<int64_t>(blocks_x) * blocks_y) return;
+  const int bx = static_cast<int>(local % blocks_x), by = static_cast<int>(local / blocks_x);
+  int smallest = 0x7fffffff, largest = -1;
+  for (int y = by * 8; y < min(by * 8 + 8, cam.tiles_y); ++y) {
+    for (int x = bx * 8; x < min(bx * 8 + 8, cam.tiles_x); ++x) {
+      const int v = per_pixel[cam.tile_base + static_cast<uint32_t>(y * cam.tiles_x + x)];
+      if (kCrossing && v < 0) continue;  // the pixel never reaches the threshold
+      smallest = min(smallest, v);
+      largest = max(largest, v);
+    }
+  }
+  if (kCrossing) {
+    bounds[2 * index + 0] = smallest;
+    bounds[2 * index + 1] = largest;
+  } else {
+    bounds[index] = largest;
+  }
+}
+
 // Fine level: thickness per (bucket of the pixel's crossing column, pixel),
 // column-major like the coarse level.
 __global__ __launch_bounds__(kDenseThreads) void dense_fine_kernel(
     const vgr_record* __restrict__ records, const int32_t* __restrict__ radii, const int64_t* __restrict__ counters,
     int64_t visible_capacity, const CameraParams* __restrict__ table, const int32_t* __restrict__ camera_map,
     int batch, float near_plane, float far_plane, int buckets, int fine, int64_t pixels,
-    const int32_t* __restrict__ crossing, unsigned long long* __restrict__ cells) {
+    const int32_t* __restrict__ crossing, const int32_t* __restrict__ block_crossing,
+    unsigned long long* __restrict__ cells) {
   const int64_t visible = visible_records(counters, visible_capacity);
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -162,7 +207,23 @@ __global__ __launch_bounds__(kDenseThreads) void dense_fine_kernel(
        first += stride) {  // uniform over the wave
     const RecordSplat s = load_record_splat(records, radii, first + lane, visible, table, camera_map, batch,
                                             near_plane, far_plane, buckets);
-    dense_wave_pixels(s.valid, s.p, s.rect, s.tile_base, s.tiles_x, s.bucket, nullptr,
+    // Splats whose column is no crossing column of any block they reach add
+    // nothing here and are not walked.
+    bool walk = false;
+    if (s.valid) {
+      const int column = s.bucket / fine;
+      const int blocks_x = dev::dense_blocks_x(s.tiles_x);
+      for (int by = s.rect.min_y >> 3; by <= (s.rect.max_y - 1) >> 3 && !walk; ++by) {
+        for (int bx = s.rect.min_x >> 3; bx <= (s.rect.max_x - 1) >> 3; ++bx) {
+          const int64_t b = static_cast<int64_t>(s.tile_base) + by * blocks_x + bx;
+          if (block_crossing[2 * b] <= column && column <= block_crossing[2 * b + 1]) {
+            walk = true;
+            break;
+          }
+        }
+      }
+    }
+    dense_wave_pixels(walk, s.p, s.rect, s.tile_base, s.tiles_x, s.bucket, nullptr,
                       [&](bool keep, int, int owner, uint32_t pixel, float alpha) {
                         const int bucket = __shfl(s.bucket, owner, kWave);
                         const int column = bucket / fine;
@@ -206,6 +267,8 @@ int launch_dense_cutoff(const vgr_cameras& cameras, const vgr_options& options, 
   const int fine = layout.dense_fine;
   const int columns = layout.dense_columns;
   unsigned long long* cells = coarse + pixels * columns;
+  int32_t* block_cutoff = reinterpret_cast<int32_t*>(scratch + layout.dense_block_cutoff);
+  int32_t* block_crossing = reinterpret_cast<int32_t*>(scratch + layout.dense_block_crossing);
   if (ws.radii == nullptr) {
     set_error("the depth cutoff needs workspace.radii");
     return VGR_E_INVALID_ARGUMENT;
@@ -230,11 +293,15 @@ int launch_dense_cutoff(const vgr_cameras& cameras, const vgr_options& options, 
                        cameras.batch, options.near_plane, options.far_plane, buckets, fine, pixels, coarse);
     hipLaunchKernelGGL(dense_coarse_scan_kernel, dim3(static_cast<unsigned>(pixel_blocks)), dim3(kDenseThreads), 0,
                        stream, coarse, pixels, columns, buckets, crossing, before, cutoff);
+    hipLaunchKernelGGL(dense_block_bounds_kernel<true>, dim3(static_cast<unsigned>(pixel_blocks)),
+                       dim3(kDenseThreads), 0, stream, table, cameras.active_count, pixels, crossing, block_crossing);
     hipLaunchKernelGGL(dense_fine_kernel, dim3(static_cast<unsigned>(record_blocks)), dim3(kDenseThreads), 0, stream,
                        ws.records, ws.radii, ws.counters, ws.visible_capacity, table, camera_map, cameras.batch,
-                       options.near_plane, options.far_plane, buckets, fine, pixels, crossing, cells);
+                       options.near_plane, options.far_plane, buckets, fine, pixels, crossing, block_crossing, cells);
     hipLaunchKernelGGL(dense_fine_scan_kernel, dim3(static_cast<unsigned>(pixel_blocks)), dim3(kDenseThreads), 0,
                        stream, cells, pixels, fine, buckets, crossing, before, cutoff);
+    hipLaunchKernelGGL(dense_block_bounds_kernel<false>, dim3(static_cast<unsigned>(pixel_blocks)),
+                       dim3(kDenseThreads), 0, stream, table, cameras.active_count, pixels, cutoff, block_cutoff);
   }
   const hipError_t error = hipGetLastError();
   if (error != hipSuccess) {

@@ -73,6 +73,8 @@ struct ScratchLayout {
   size_t dense_cutoff;       // int32[pixels]: last depth bucket emitted per pixel (-1 while open)
   size_t dense_accumulated;  // uint64[pixels]: thickness of the buckets before the crossing column
   size_t dense_crossing;     // int32[pixels]: column where the running thickness reaches the threshold
+  size_t dense_block_cutoff;    // int32 per 8x8 pixel block (indexed from the camera's first pixel): largest cutoff
+  size_t dense_block_crossing;  // int32[2] per 8x8 pixel block: smallest and largest crossing column of its pixels
   size_t dense_tau;          // uint64[(columns + fine) * pixels]: coarse column sums, then fine bucket sums, column-major
   int64_t dense_pixels;
   int32_t dense_fine;        // buckets per coarse column

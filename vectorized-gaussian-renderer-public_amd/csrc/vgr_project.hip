@@ -197,7 +197,10 @@ __global__ __launch_bounds__(kBlockThreads) void projection_count_kernel(Project
         const int bucket = seen && args.depth_cutoff != nullptr
                                ? depth_bucket(p.depth, args.near_plane, args.far_plane, args.depth_buckets)
                                : 0;
-        const int wave_pixels = dense_wave_pixels(seen, p, rect, cam.tile_base, cam.tiles_x, bucket,
+        // Splats behind the cutoff of every pixel they reach are not walked (the
+        // write pass applies the same test, so the counts and slots agree).
+        const bool walk = seen && dense_rect_reaches(rect, cam.tile_base, cam.tiles_x, bucket, args.dense_block_cutoff);
+        const int wave_pixels = dense_wave_pixels(walk, p, rect, cam.tile_base, cam.tiles_x, bucket,
                                                   args.depth_cutoff, [](bool, int, int, uint32_t, float) {});
         if ((threadIdx.x & (kWave - 1)) == 0) intersections += wave_pixels;
       }
@@ -305,8 +308,10 @@ __global__ __launch_bounds__(kBlockThreads, 4) void projection_write_kernel(
         const int bucket = visible && args.depth_cutoff != nullptr
                                ? depth_bucket(p.depth, args.near_plane, args.far_plane, args.depth_buckets)
                                : 0;
+        const bool walk =
+            visible && dense_rect_reaches(rect, cam.tile_base, cam.tiles_x, bucket, args.dense_block_cutoff);
         const int wave_pixels = dense_wave_pixels(
-            visible, p, rect, cam.tile_base, cam.tiles_x, bucket, args.depth_cutoff,
+            walk, p, rect, cam.tile_base, cam.tiles_x, bucket, args.depth_cutoff,
             [&](bool keep, int index, int owner, uint32_t pixel, float) {
               const int32_t owner_value = __shfl(value, owner, kWave);
               const uint32_t owner_depth = static_cast<uint32_t>(__shfl(static_cast<int>(depth_bits), owner, kWave));
@@ -738,6 +743,7 @@ void make_projection_args(const vgr_scene& scene, const vgr_cameras& cameras, co
   args.dense_mode = 0;
   args.depth_buckets = 0;
   args.depth_cutoff = nullptr;
+  args.dense_block_cutoff = nullptr;
   args.antialiased = options.antialiased;
   args.near_plane = options.near_plane;
   args.far_plane = options.far_plane;
@@ -863,6 +869,7 @@ int launch_projection(const vgr_scene& scene, const vgr_cameras& cameras, const 
     args.dense_mode = 2;
     args.depth_buckets = options.depth_bucket_count;
     args.depth_cutoff = reinterpret_cast<const int32_t*>(scratch + layout.dense_cutoff);
+    args.dense_block_cutoff = reinterpret_cast<const int32_t*>(scratch + layout.dense_block_cutoff);
     projection_pass<1>(args, ws, block_counts, block_bases, wave_masks, partials, blocks, stream);
   }
   const hipError_t status = hipGetLastError();

@@ -383,6 +383,7 @@ struct ProjectionArgs {
   int dense_mode;
   int depth_buckets;             // depth buckets of the cutoff (depth_cutoff non-null)
   const int32_t* depth_cutoff;   // per global pixel: last depth bucket emitted, or null (no cutoff)
+  const int32_t* dense_block_cutoff;  // per 8x8 pixel block: largest cutoff of its pixels (with depth_cutoff)
 };
 
 // ---- dense (tile size 1) per-pixel decisions --------------------------------------
@@ -520,6 +521,25 @@ __device__ __forceinline__ int dense_wave_pixels(bool seen, const Projected& p, 
     kept += static_cast<int>(__popcll(vote));
   }
   return kept;
+}
+
+// Per-block bounds of the depth cutoff (tile size 1).  A camera's 8x8 pixel
+// blocks are stored from its first pixel on (index tile_base + block_y *
+// blocks_x + block_x; a camera has at least as many pixels as blocks).
+__device__ __forceinline__ int dense_blocks_x(int tiles_x) { return (tiles_x + 7) >> 3; }
+
+// False only when every pixel of the rectangle has its cutoff in front of
+// `bucket` (the block maxima say so), so a pixel walk would keep none of it.
+__device__ __forceinline__ bool dense_rect_reaches(const TileRect& rect, uint32_t tile_base, int tiles_x, int bucket,
+                                                   const int32_t* __restrict__ block_cutoff) {
+  if (block_cutoff == nullptr) return true;
+  const int blocks_x = dense_blocks_x(tiles_x);
+  for (int by = rect.min_y >> 3; by <= (rect.max_y - 1) >> 3; ++by) {
+    for (int bx = rect.min_x >> 3; bx <= (rect.max_x - 1) >> 3; ++bx) {
+      if (block_cutoff[tile_base + static_cast<uint32_t>(by * blocks_x + bx)] >= bucket) return true;
+    }
+  }
+  return false;
 }
 
 // evaluate_pair for exact-ray renders: the ray extent replaces the splat's mean
